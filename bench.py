@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: SpMM / SpMV / SDDMM HBM throughput on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): A is a
+1,000,000 x 1,000,000 CSR matrix with ~16M stored FP32 values, Zipf(1.5) row
+lengths (truncated at 100k, scaled to 16M total), uniform column indices, N(0,1)
+values, Int32 ptr/idx; B (= Y) and X are dense 1M x 64 FP32 N(0,1), row-major.
+A "step" is one SpMM C = A*B (the headline kernel); SpMV y = A*x and SDDMM
+out = A .* (X*Y^T) are timed the same way and reported beside it.
+
+`value` is algorithmic bytes / time for SpMM (GB/s) with every operand resident
+in HBM before the timed region; `roofline` compares it with the 8 TB/s HBM
+peak.  `cpu_baseline` times the oracle's threaded CSR SpMM (the C restatement
+of Finch's row-parallel loop) on this host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "finch.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0
+M = N = 1_000_000
+NNZ_TARGET = 16_000_000
+K = 64
+
+
+def algorithmic_bytes(m, n, nnz, k, kind, sv=4, si=4, sp=4, sd=4):
+    """Bytes a kernel must move at least once (each operand read/written once)."""
+    if kind == "spmv":
+        return nnz * (sv + si) + (m + 1) * sp + n * sd + m * sd
+    if kind == "spmm":
+        return nnz * (sv + si) + (m + 1) * sp + n * k * sd + m * k * sd
+    if kind == "sddmm":
+        return nnz * (2 * sv + si) + (m + 1) * sp + (m + n) * k * sd
+    raise ValueError(kind)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="approximate CPU time budget of the baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only; the sparse kernels are "
+                                   "measured at their full single-GPU shape (see DESIGN.md for "
+                                   "how rows would shard across GPUs)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from finchgpu import ops, synth
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    ptr, idx, val = synth.powerlaw_csr(M, N, NNZ_TARGET, seed=0)
+    nnz = int(idx.size)
+    print(f"# generated A: nnz={nnz} in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    A = ops.Compressed(torch.from_numpy(ptr.astype(np.int32)).to(dev),
+                       torch.from_numpy(idx.astype(np.int32)).to(dev),
+                       torch.from_numpy(val).to(dev), M, N, base=1)
+    B = torch.from_numpy(synth.dense(N, K, seed=2)).to(dev)
+    X = torch.from_numpy(synth.dense(M, K, seed=3)).to(dev)
+    x = B[:, 0].contiguous()
+    C = torch.empty((M, K), dtype=torch.float32, device=dev)
+    y = torch.empty(M, dtype=torch.float32, device=dev)
+    out = torch.empty(nnz, dtype=torch.float32, device=dev)
+
+    def run_timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(steps)]
+        t = time.perf_counter()
+        for s, e in ev:
+            s.record(stream)
+            fn()
+            e.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        per = [s.elapsed_time(e) for s, e in ev]
+        return wall / steps * 1e3, float(np.mean(per)), float(np.median(per))
+
+    spmm_ms, spmm_ev_ms, spmm_med = run_timed(lambda: ops.spmm_csr(A, B, C), args.steps,
+                                              args.warmup)
+    print(f"# spmm done: {spmm_ms:.4f} ms/step", file=sys.stderr, flush=True)
+    spmv_ms, spmv_ev_ms, _ = run_timed(lambda: ops.spmv_csr(A, x, y), args.steps, args.warmup)
+    sddmm_ms, sddmm_ev_ms, _ = run_timed(lambda: ops.sddmm_csr(A, X, B, out), args.steps,
+                                         args.warmup)
+    print("# spmv/sddmm done", file=sys.stderr, flush=True)
+
+    by_spmm = algorithmic_bytes(M, N, nnz, K, "spmm")
+    by_spmv = algorithmic_bytes(M, N, nnz, K, "spmv")
+    by_sddmm = algorithmic_bytes(M, N, nnz, K, "sddmm")
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    value = gbs(by_spmm, spmm_ms)
+    achieved = gbs(by_spmm, spmm_ev_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or oracle.max_threads()
+        Bh = B.cpu().numpy()
+        # Bounded sample: the first `rows` rows of A (whole B), sized so the
+        # threaded restatement runs for about --cpu-seconds in total.
+        rows = M // 8
+        q1 = int(ptr[rows] - 1)
+        sp = ptr[: rows + 1]
+        t = time.perf_counter()
+        oracle.spmm_csr(sp, idx[:q1], val[:q1], Bh, nthreads=threads)
+        one = time.perf_counter() - t
+        reps = max(1, min(10, int(args.cpu_seconds / max(one, 1e-3))))
+        times = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            oracle.spmm_csr(sp, idx[:q1], val[:q1], Bh, nthreads=threads)
+            times.append(time.perf_counter() - t)
+        tmed = float(np.median(times))
+        sample_bytes = algorithmic_bytes(rows, N, q1, K, "spmm")
+        cpu = {"value": round(gbs(sample_bytes, tmed * 1e3), 3), "unit": "GB/s",
+               "cores": threads, "kind": "port",
+               "sample": f"oracle CSR SpMM (FP32, Finch's static row-block threading) on rows "
+                         f"0..{rows} of A ({q1} nnz) x the full 1M x 64 B; median of {reps} "
+                         f"reps, {tmed * 1e3:.1f} ms each"}
+
+    line = {
+        "metric": "SpMM/SpMV HBM GB/s (% of peak); served req/s at p99, batched vs plain",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(spmm_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy PCG64; Zipf(1.5) rows, uniform columns, N(0,1))",
+        "config": {"workload": "config2 SpMM C=A*B: A 1Mx1M CSR FP32 Int32 idx, "
+                               f"nnz={nnz}, Zipf(1.5) rows; B 1Mx64 FP32 row-major",
+                   "m": M, "n": N, "nnz": nnz, "k": K, "index": "int32"},
+        "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "spmm_csr_kernel + fixup_kernel (one fgx_spmm_csr call)",
+                     "algorithmic_bytes": by_spmm, "launch_ms": round(spmm_ev_ms, 5)},
+        "spmv": {"gbs": round(gbs(by_spmv, spmv_ms), 2), "ms": round(spmv_ms, 5),
+                 "event_ms": round(spmv_ev_ms, 5), "algorithmic_bytes": by_spmv,
+                 "pct_of_hbm_peak": round(100 * gbs(by_spmv, spmv_ms) / HBM_PEAK_GBS, 2)},
+        "sddmm": {"gbs": round(gbs(by_sddmm, sddmm_ms), 2), "ms": round(sddmm_ms, 5),
+                  "event_ms": round(sddmm_ev_ms, 5), "algorithmic_bytes": by_sddmm,
+                  "gflops": round(2 * nnz * K / (sddmm_ms * 1e-3) / 1e9, 1),
+                  "pct_of_hbm_peak": round(100 * gbs(by_sddmm, sddmm_ms) / HBM_PEAK_GBS, 2)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,344 @@
+// fgx_csr_tiles.hpp — nnz-balanced ("merge-path") tiles over a CSR operand.
+//
+// The stored positions [0, nnz) are cut into equal wave tiles, so a wave's
+// work does not depend on how the positions are spread over rows: a 100k-nnz
+// row is split over many waves and a run of empty rows costs one store each.
+// Within a tile, each lane (SpMV) or lane group (SpMM) walks a contiguous run
+// of positions and the rows they belong to:
+//
+//   * a lane finalises every row whose end pos(r+1) falls in its run (a, b];
+//     rows that end exactly at a run boundary, and empty rows, are finalised
+//     by the run that contains that boundary, so every row is written once;
+//   * the first row a run finalises may have started in earlier runs: its
+//     value is held back and completed by a segmented scan of the runs'
+//     carries (in position order), then stored;
+//   * a row that started in an earlier *wave* is completed by a second, tiny
+//     fix-up launch that adds the earlier waves' carries in wave order.
+//
+// Sums are therefore deterministic (no atomics) and independent of what
+// else runs on the device, which the serving path relies on.
+#pragma once
+#include "fgx_device.hpp"
+
+namespace fgx {
+
+constexpr int kWavesPerBlock = 4;
+constexpr int kWinCap = 512;  // fibre ends cached in LDS per wave
+
+// Fibres that a wave tile [q0, qend) touches: r_lo owns q0 (0 for the first
+// tile), r_hi owns qend (the row a carry continues into; m if none).
+template <typename TP>
+__device__ __forceinline__ void tile_fibers(const TP* __restrict__ ptr, int64_t base, int64_t m,
+                                            int64_t q0, int64_t qend, int64_t* r_lo,
+                                            int64_t* r_hi) {
+  const int lane = lane_id();
+  const int64_t target = lane < 32 ? q0 : qend;
+  int64_t r = group_find_fiber<32>(ptr, base, 0, m, target);
+  int64_t lo = shfl(r, 0);
+  int64_t hi = shfl(r, 32);
+  *r_lo = q0 == 0 ? 0 : lo;
+  *r_hi = hi;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV tile: lane-level runs of V positions.  Writes y for every row the tile
+// finalises and (carry_row, carry_val) of the row continuing into the next
+// tile (carry_row = -1 when no positions of this tile belong to it).
+template <typename TP, typename TI, typename TV, typename TA, int V>
+__device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
+                                          const TV* __restrict__ val, int64_t base, int64_t m,
+                                          int64_t nnz, const TA* __restrict__ x,
+                                          TA* __restrict__ y, int64_t tile, bool vec,
+                                          int64_t* lds_win, int64_t* carry_row_out,
+                                          TA* carry_val_out) {
+  constexpr int W = kWave * V;
+  const int lane = lane_id();
+  const int64_t q0 = tile * W;
+  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  int64_t r_lo, r_hi;
+  tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
+  const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
+  FiberWindow<TP> win = load_window(ptr, base, r_lo, r_win_last - r_lo + 1, lds_win, kWinCap);
+
+  const int64_t a = q0 + static_cast<int64_t>(lane) * V;
+  const int64_t b = (a + V < qend) ? a + V : qend;
+  const bool active = a < qend || (a == 0 && m > 0);
+
+  // Stream the run's values and coordinates.
+  TV vv[V];
+  TI jj[V];
+  if (active && b - a == V) {
+    ld_run<TV, V>(val + a, vec, vv);
+    ld_run<TI, V>(idx + a, vec, jj);
+  } else {
+#pragma unroll
+    for (int s = 0; s < V; s++) {
+      const bool in = a + s < b;
+      vv[s] = in ? ld_stream(val + a + s) : TV{};
+      jj[s] = in ? ld_stream(idx + a + s) : TI{};
+    }
+  }
+  // Gather x (independent loads, issued together).
+  TA prod[V];
+#pragma unroll
+  for (int s = 0; s < V; s++) {
+    const bool in = a + s < b;
+    prod[s] = TA(0);
+    if (in) prod[s] = widen<TA>(vv[s]) * x[static_cast<int64_t>(jj[s]) - base];
+  }
+
+  TA acc = 0;
+  bool has_emit = false;
+  int64_t first_row = -1;
+  TA first_val = 0;
+  int64_t r = active ? (a == 0 ? 0 : win.find(a)) : r_hi;
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < V; s++) {
+      const int64_t q = a + s;
+      if (q < b) {
+        while (r < m && r <= r_win_last && win.end_of(r) <= q) {
+          if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
+          else y[r] = acc;
+          acc = 0;
+          r++;
+        }
+        acc += prod[s];
+      }
+    }
+    while (r < m && r <= r_win_last && win.end_of(r) <= b) {
+      if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
+      else y[r] = acc;
+      acc = 0;
+      r++;
+    }
+  }
+  // Segmented inclusive scan of the runs' carries; a run with an emit starts
+  // a new segment.
+  TA v = acc;
+  bool f = has_emit;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    TA v_up = shfl_up(v, d);
+    int f_up = shfl_up(static_cast<int>(f), d);
+    if (lane >= d) {
+      if (!f) v = v_up + v;
+      f = f || f_up;
+    }
+  }
+  TA carry_in = shfl_up(v, 1);
+  if (lane == 0) carry_in = 0;
+  if (has_emit) y[first_row] = first_val + carry_in;
+
+  if (lane == kWave - 1) {
+    // Does the continuing row r_hi hold positions of this tile?
+    bool carries = false;
+    if (r_hi < m && qend > q0) {
+      const int64_t start_hi = (r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
+      carries = start_hi < qend;
+    }
+    *carry_row_out = carries ? r_hi : -1;
+    *carry_val_out = carries ? v : TA(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// SpMM tile: groups of G lanes, each lane owning KV consecutive dense columns
+// of the [kofs, kofs + G*KV) slice; each group walks E consecutive positions.
+// Rows of B and C are `ldb`/`ldc` apart (row-major).
+template <int KV, typename TA>
+struct accv {
+  TA v[KV];
+};
+
+template <int KV, typename TA>
+__device__ __forceinline__ void ld_dense(const TA* __restrict__ p, int valid_cols, TA (&o)[KV]) {
+  if constexpr (KV == 4 && sizeof(TA) == 4) {
+    if (valid_cols >= 4) {
+      float4 t = *reinterpret_cast<const float4*>(p);
+      o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+      return;
+    }
+  }
+  if constexpr (KV == 2 && sizeof(TA) == 8) {
+    if (valid_cols >= 2) {
+      double2 t = *reinterpret_cast<const double2*>(p);
+      o[0] = t.x; o[1] = t.y;
+      return;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < KV; c++) o[c] = c < valid_cols ? p[c] : TA(0);
+}
+
+template <int KV, typename TA>
+__device__ __forceinline__ void st_dense(TA* __restrict__ p, int valid_cols, const TA (&o)[KV]) {
+  if constexpr (KV == 4 && sizeof(TA) == 4) {
+    if (valid_cols >= 4) {
+      *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+      return;
+    }
+  }
+  if constexpr (KV == 2 && sizeof(TA) == 8) {
+    if (valid_cols >= 2) {
+      *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+      return;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < KV; c++)
+    if (c < valid_cols) p[c] = o[c];
+}
+
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
+__device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
+                                          const TV* __restrict__ val, int64_t base, int64_t m,
+                                          int64_t nnz, int64_t k, int64_t kofs,
+                                          const TA* __restrict__ B, int64_t ldb,
+                                          TA* __restrict__ C, int64_t ldc, int64_t tile,
+                                          int64_t* lds_win, int64_t* carry_row_out,
+                                          TA* carry_val_out /* G*KV entries */) {
+  constexpr int NG = kWave / G;
+  constexpr int W = NG * E;
+  static_assert(E % G == 0, "E must be a multiple of G");
+  static_assert(G % U == 0 || U % G == 0, "U and G must nest");
+  const int lane = lane_id();
+  const int g = lane / G;
+  const int gl = lane % G;
+  const int gbase = g * G;
+  const int64_t q0 = tile * W;
+  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  int64_t r_lo, r_hi;
+  tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
+  const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
+  FiberWindow<TP> win = load_window(ptr, base, r_lo, r_win_last - r_lo + 1, lds_win, kWinCap);
+
+  const int64_t col0 = kofs + static_cast<int64_t>(gl) * KV;
+  const int valid_cols = static_cast<int>(col0 < k ? (k - col0 < KV ? k - col0 : KV) : 0);
+
+  const int64_t a = q0 + static_cast<int64_t>(g) * E;
+  const int64_t b = (a + E < qend) ? a + E : qend;
+  const bool active = a < qend || (a == 0 && m > 0);
+
+  TA acc[KV];
+#pragma unroll
+  for (int c = 0; c < KV; c++) acc[c] = 0;
+  bool has_emit = false;
+  int64_t first_row = -1;
+  TA first_val[KV];
+#pragma unroll
+  for (int c = 0; c < KV; c++) first_val[c] = 0;
+  int64_t r = active ? (a == 0 ? 0 : win.find(a)) : r_hi;
+
+  auto emit = [&](int64_t row) {
+    if (!has_emit) {
+      has_emit = true;
+      first_row = row;
+#pragma unroll
+      for (int c = 0; c < KV; c++) first_val[c] = acc[c];
+    } else if (valid_cols > 0) {
+      st_dense<KV, TA>(C + row * ldc + col0, valid_cols, acc);
+    }
+#pragma unroll
+    for (int c = 0; c < KV; c++) acc[c] = 0;
+  };
+
+  // Chunks of G positions: lane gl of the group loads position a+c0+gl.
+#pragma unroll 1
+  for (int c0 = 0; c0 < E; c0 += G) {
+    const int64_t qc = a + c0 + gl;
+    const bool inc = qc < b;
+    const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + qc)) - base : 0;
+    const TA vc = inc ? widen<TA>(ld_stream(val + qc)) : TA(0);
+#pragma unroll
+    for (int e0 = 0; e0 < G; e0 += U) {
+      TA bv[U][KV];
+      TA ve[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int src = gbase + ((e0 + u) % G);
+        const int64_t je = shfl(jc, src);
+        ve[u] = shfl(vc, src);
+        const bool in = (e0 + u < G) && (a + c0 + e0 + u < b) && valid_cols > 0;
+        if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
+        else {
+#pragma unroll
+          for (int c = 0; c < KV; c++) bv[u][c] = 0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t q = a + c0 + e0 + u;
+        if (e0 + u < G && q < b) {
+          while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
+#pragma unroll
+          for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
+        }
+      }
+    }
+  }
+  if (active) {
+    while (r < m && r <= r_win_last && win.end_of(r) <= b) { emit(r); r++; }
+  }
+
+  // Segmented scan of the groups' carries (same rule as spmv_tile).
+  TA v[KV];
+#pragma unroll
+  for (int c = 0; c < KV; c++) v[c] = acc[c];
+  bool f = has_emit;
+#pragma unroll
+  for (int d = G; d < kWave; d <<= 1) {
+    int f_up = shfl_up(static_cast<int>(f), d);
+    TA v_up[KV];
+#pragma unroll
+    for (int c = 0; c < KV; c++) v_up[c] = shfl_up(v[c], d);
+    if (lane >= d) {
+      if (!f) {
+#pragma unroll
+        for (int c = 0; c < KV; c++) v[c] = v_up[c] + v[c];
+      }
+      f = f || f_up;
+    }
+  }
+  TA cin[KV];
+#pragma unroll
+  for (int c = 0; c < KV; c++) {
+    cin[c] = shfl_up(v[c], G);
+    if (lane < G) cin[c] = 0;
+  }
+  if (has_emit && valid_cols > 0) {
+    TA o[KV];
+#pragma unroll
+    for (int c = 0; c < KV; c++) o[c] = first_val[c] + cin[c];
+    st_dense<KV, TA>(C + first_row * ldc + col0, valid_cols, o);
+  }
+  bool carries = false;
+  if (r_hi < m && qend > q0) {
+    const int64_t start_hi = (r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
+    carries = start_hi < qend;
+  }
+  if (lane == kWave - 1) *carry_row_out = carries ? r_hi : -1;
+  if (g == NG - 1 && carries) {
+#pragma unroll
+    for (int c = 0; c < KV; c++) carry_val_out[gl * KV + c] = v[c];
+  }
+}
+
+// Adds the carries of runs of consecutive tiles to the row they continue:
+// one thread (SpMV) or lane group (SpMM) per tile; only the first tile of a
+// run does the work, summing in tile order.
+template <typename TA>
+__device__ __forceinline__ void fixup_run(const int64_t* __restrict__ carry_row,
+                                          const TA* __restrict__ carry_val, int64_t ntiles,
+                                          int64_t t, int width, int col, TA* __restrict__ out_row_base,
+                                          int64_t ld, int64_t kofs, int64_t k) {
+  const int64_t r = carry_row[t];
+  if (r < 0) return;
+  if (t > 0 && carry_row[t - 1] == r) return;
+  if (col >= width || kofs + col >= k) return;
+  TA s = 0;
+  for (int64_t t2 = t; t2 < ntiles && carry_row[t2] == r; t2++) s += carry_val[t2 * width + col];
+  out_row_base[r * ld + kofs + col] += s;
+}
+
+}  // namespace fgx

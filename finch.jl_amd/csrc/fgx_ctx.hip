@@ -1,0 +1,196 @@
+// fgx_ctx.hip — context lifecycle, error reporting, scratch arena and the
+// value-precision conversion kernel.
+#include "fgx_internal.hpp"
+#include "fgx_device.hpp"
+
+#include <cstring>
+
+namespace fgx {
+
+static thread_local char g_last_error[512] = "";
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+void clear_error() { g_last_error[0] = '\0'; }
+
+int scratch(fgx_ctx* ctx, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 256;
+  if (bytes > ctx->scratch_bytes) {
+    if (ctx->scratch) {
+      // The stream may still use the old arena.
+      FGX_HIP(hipStreamSynchronize(ctx->stream));
+      FGX_HIP(hipDeviceSynchronize());
+      FGX_HIP(hipFree(ctx->scratch));
+      ctx->scratch = nullptr;
+      ctx->scratch_bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    want = (want + 4095) & ~static_cast<size_t>(4095);
+    if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+      (void)hipGetLastError();
+      ctx->scratch = nullptr;
+      return set_error(FGX_EOOM, "scratch allocation of %zu bytes failed", want);
+    }
+    ctx->scratch_bytes = want;
+  }
+  *out = ctx->scratch;
+  return FGX_OK;
+}
+
+int check_compressed(const fgx_compressed* A, const char* name) {
+  if (!A) return set_error(FGX_EINVAL, "%s: NULL descriptor", name);
+  if (A->n_outer < 0 || A->n_inner < 0 || A->nnz < 0)
+    return set_error(FGX_EINVAL, "%s: negative size", name);
+  if (A->index_base != 0 && A->index_base != 1)
+    return set_error(FGX_EINVAL, "%s: index_base must be 0 or 1", name);
+  if (!A->ptr) return set_error(FGX_EINVAL, "%s: NULL ptr", name);
+  if (A->nnz > 0 && (!A->idx || !A->val)) return set_error(FGX_EINVAL, "%s: NULL idx/val", name);
+  if (A->ptr_type == FGX_I32 && A->nnz > INT32_MAX)
+    return set_error(FGX_EUNSUP, "%s: nnz exceeds Int32 ptr", name);
+  return FGX_OK;
+}
+
+int idx_combo(const fgx_compressed* A, IdxCombo* out) {
+  if (A->ptr_type == FGX_I32 && A->idx_type == FGX_I32) *out = IDX_32_32;
+  else if (A->ptr_type == FGX_I64 && A->idx_type == FGX_I64) *out = IDX_64_64;
+  else if (A->ptr_type == FGX_I64 && A->idx_type == FGX_I32) *out = IDX_64_32;
+  else
+    return set_error(FGX_EUNSUP, "unsupported ptr/idx types (%d, %d)", A->ptr_type, A->idx_type);
+  return FGX_OK;
+}
+
+int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess)
+    return set_error(FGX_EHIP, "%s launch failed: %s", what, hipGetErrorString(e));
+  return FGX_OK;
+}
+
+// ---- conversion kernel ----------------------------------------------------
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return static_cast<uint16_t>((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+
+template <typename S, typename D>
+__device__ __forceinline__ D conv(S s);
+
+template <int ST, int DT>
+__global__ void convert_kernel(const void* __restrict__ src, void* __restrict__ dst, int64_t n) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) {
+    float f;
+    double d = 0.0;
+    if constexpr (ST == FGX_F32) f = static_cast<const float*>(src)[i];
+    else if constexpr (ST == FGX_F64) { d = static_cast<const double*>(src)[i]; f = static_cast<float>(d); }
+    else if constexpr (ST == FGX_BF16) f = to_f32(static_cast<const bf16_t*>(src)[i]);
+    else f = to_f32(static_cast<const f16_t*>(src)[i]);
+    if constexpr (DT == FGX_F32) static_cast<float*>(dst)[i] = f;
+    else if constexpr (DT == FGX_F64) static_cast<double*>(dst)[i] = (ST == FGX_F64) ? d : static_cast<double>(f);
+    else if constexpr (DT == FGX_BF16) static_cast<uint16_t*>(dst)[i] = f32_to_bf16_rne(f);
+    else {
+      _Float16 h = static_cast<_Float16>(f);
+      uint16_t b;
+      __builtin_memcpy(&b, &h, 2);
+      static_cast<uint16_t*>(dst)[i] = b;
+    }
+  }
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" {
+
+int fgx_version(void) { return FGX_ABI_VERSION; }
+
+const char* fgx_last_error(void) { return g_last_error; }
+
+int fgx_ctx_create(int device, fgx_ctx** out) {
+  clear_error();
+  if (!out) return set_error(FGX_EINVAL, "fgx_ctx_create: NULL out");
+  int n = 0;
+  FGX_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return set_error(FGX_EINVAL, "fgx_ctx_create: device %d of %d", device, n);
+  FGX_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  FGX_HIP(hipGetDeviceProperties(&prop, device));
+  fgx_ctx* c = new fgx_ctx();
+  c->device = device;
+  c->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_error(FGX_EHIP, "fgx_ctx_create: stream creation failed");
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->host_i64), 64 * sizeof(int64_t)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return set_error(FGX_EHIP, "fgx_ctx_create: pinned host buffer allocation failed");
+  }
+  *out = c;
+  return FGX_OK;
+}
+
+int fgx_ctx_destroy(fgx_ctx* ctx) {
+  if (!ctx) return FGX_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->host_i64) (void)hipHostFree(ctx->host_i64);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return FGX_OK;
+}
+
+void* fgx_ctx_stream(fgx_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+int fgx_sync(fgx_ctx* ctx, void* stream) {
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_sync: NULL ctx");
+  FGX_HIP(hipSetDevice(ctx->device));
+  FGX_HIP(hipStreamSynchronize(pick_stream(ctx, stream)));
+  return FGX_OK;
+}
+
+int fgx_plan_cache_clear(fgx_ctx* ctx) {
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_plan_cache_clear: NULL ctx");
+  return FGX_OK;
+}
+
+int fgx_convert(fgx_ctx* ctx, const void* src, int src_type, void* dst, int dst_type, int64_t n,
+                void* stream) {
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_convert: NULL ctx");
+  if (n < 0) return set_error(FGX_EINVAL, "fgx_convert: negative n");
+  if (n == 0) return FGX_OK;
+  if (!src || !dst) return set_error(FGX_EINVAL, "fgx_convert: NULL buffer");
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  dim3 g(static_cast<unsigned>(blocks)), b(256);
+#define FGX_CONV(ST, DT)                                                       \
+  if (src_type == ST && dst_type == DT) {                                      \
+    hipLaunchKernelGGL((convert_kernel<ST, DT>), g, b, 0, s, src, dst, n);     \
+    return launch_check("fgx_convert");                                        \
+  }
+  FGX_CONV(FGX_F32, FGX_BF16)
+  FGX_CONV(FGX_F32, FGX_F16)
+  FGX_CONV(FGX_BF16, FGX_F32)
+  FGX_CONV(FGX_F16, FGX_F32)
+  FGX_CONV(FGX_F32, FGX_F64)
+  FGX_CONV(FGX_F64, FGX_F32)
+#undef FGX_CONV
+  return set_error(FGX_EUNSUP, "fgx_convert: %d -> %d unsupported", src_type, dst_type);
+}
+
+}  // extern "C"

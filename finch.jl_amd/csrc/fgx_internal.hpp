@@ -1,0 +1,86 @@
+// fgx_internal.hpp — host-side internals of libfinchgpu: the context,
+// error reporting, scratch arena and operand validation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+
+#include "../../include/finchgpu.h"
+
+struct fgx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  // Growable device scratch arena (carries, counters, bins).  One arena per
+  // context; calls on one context are serialised by `mu`.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Small pinned host buffer for data-dependent sizes (symbolic nnz).
+  int64_t* host_i64 = nullptr;
+  std::mutex mu;
+};
+
+namespace fgx {
+
+// Records the calling thread's last error; returns `code`.
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+
+#define FGX_HIP(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return ::fgx::set_error(FGX_EHIP, "%s failed: %s (%s:%d)", #expr,                \
+                              hipGetErrorString(e_), __FILE__, __LINE__);              \
+  } while (0)
+
+#define FGX_TRY(expr)                 \
+  do {                                \
+    int s_ = (expr);                  \
+    if (s_ != FGX_OK) return s_;      \
+  } while (0)
+
+// Scratch of at least `bytes` (grows, never shrinks).  Caller holds ctx->mu.
+int scratch(fgx_ctx* ctx, size_t bytes, void** out);
+
+inline hipStream_t pick_stream(fgx_ctx* ctx, void* s) {
+  return s ? static_cast<hipStream_t>(s) : ctx->stream;
+}
+
+inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Checks the descriptor (non-null buffers, supported type combination).
+int check_compressed(const fgx_compressed* A, const char* name);
+
+// Index-type combinations the kernels are instantiated for.
+enum IdxCombo { IDX_32_32 = 0, IDX_64_64 = 1, IDX_64_32 = 2 };
+int idx_combo(const fgx_compressed* A, IdxCombo* out);
+
+inline int elem_size(int dtype) {
+  switch (dtype) {
+    case FGX_F64: case FGX_I64: return 8;
+    case FGX_F32: case FGX_I32: return 4;
+    case FGX_BF16: case FGX_F16: return 2;
+    default: return 0;
+  }
+}
+
+// Dense element type paired with a value type: F64 for F64, F32 otherwise.
+inline int dense_type_of(int val_type) { return val_type == FGX_F64 ? FGX_F64 : FGX_F32; }
+
+int launch_check(const char* what);
+
+}  // namespace fgx
+
+// Dispatch helpers: call F.template operator()<TP, TI>() for the operand's
+// index combination.
+#define FGX_DISPATCH_IDX(combo, ...)                                   \
+  switch (combo) {                                                     \
+    case ::fgx::IDX_32_32: { using TP = int32_t; using TI = int32_t; __VA_ARGS__; break; } \
+    case ::fgx::IDX_64_64: { using TP = int64_t; using TI = int64_t; __VA_ARGS__; break; } \
+    case ::fgx::IDX_64_32: { using TP = int64_t; using TI = int32_t; __VA_ARGS__; break; } \
+  }

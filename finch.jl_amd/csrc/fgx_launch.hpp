@@ -1,0 +1,42 @@
+// fgx_launch.hpp — launch helpers shared by the CSR tile kernels: the
+// carry fix-up kernel and the value-type dispatch.
+#pragma once
+#include "fgx_csr_tiles.hpp"
+#include "fgx_internal.hpp"
+
+namespace fgx {
+
+template <typename TA>
+__global__ __launch_bounds__(256) void fixup_kernel(const int64_t* __restrict__ carry_row,
+                                                    const TA* __restrict__ carry_val,
+                                                    int64_t ntiles, int width, TA* __restrict__ out,
+                                                    int64_t ld, int64_t kofs, int64_t k) {
+  // One thread per (tile, column).
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t t = gid / width;
+  const int col = static_cast<int>(gid % width);
+  if (t >= ntiles) return;
+  fixup_run<TA>(carry_row, carry_val, ntiles, t, width, col, out, ld, kofs, k);
+}
+
+template <typename TA>
+static inline int launch_fixup(hipStream_t s, const int64_t* crow, const TA* cval, int64_t ntiles,
+                        int width, TA* out, int64_t ld, int64_t kofs, int64_t k) {
+  const int64_t threads = ntiles * width;
+  const int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL((fixup_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
+                     crow, cval, ntiles, width, out, ld, kofs, k);
+  return launch_check("fixup_kernel");
+}
+
+}  // namespace fgx
+
+#define FGX_DISPATCH_VAL(vt, ...)                                              \
+  switch (vt) {                                                                \
+    case FGX_F64: { using TV = double; __VA_ARGS__; break; }                   \
+    case FGX_F32: { using TV = float; __VA_ARGS__; break; }                    \
+    case FGX_BF16: { using TV = bf16_t; __VA_ARGS__; break; }                  \
+    case FGX_F16: { using TV = f16_t; __VA_ARGS__; break; }                    \
+    default: return set_error(FGX_EUNSUP, "unsupported value type %d", vt);    \
+  }
+

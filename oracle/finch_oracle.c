@@ -1,0 +1,364 @@
+/*
+ * finch_oracle.c — CPU oracle: a C restatement of the loop nests Finch.jl
+ * generates for the sparse kernels this backend replaces.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for parity tests and the
+ * timed CPU baseline ("port") in bench.py; the product path (libfinchgpu.so)
+ * never links, loads or calls it.
+ *
+ * Every function follows the generated Julia code it cites (paths relative to
+ * the Finch.jl v1.2.12 tree), in the same accumulation order:
+ *   - 1-based (or 0-based, `base`) ptr/idx; fiber p occupies
+ *     q in [ptr[p]-base, ptr[p+1]-base);
+ *   - `y[i] = x*a + y[i]` products are rounded before the add: built with
+ *     -ffp-contract=off, which reproduces the FP64 golden
+ *     test/reference64/typical/typical_spmv_csc.txt:92 bit for bit;
+ *   - every coiterated coordinate is stored even when its value equals the
+ *     fill (src/tensors/scalars.jl:209-218).
+ * Indices are taken as int64 (Finch's default `Int`); callers widen int32
+ * buffers before calling.
+ *
+ * Pinning (tests/test_oracle.py): each kernel is checked against the
+ * reference's golden vectors where one exists, against scipy/numpy or a
+ * brute-force dense loop on small inputs, and against closed forms.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef int64_t i64;
+
+/* Julia's fld for non-negative operands. */
+static inline i64 fld(i64 a, i64 b) { return a / b; }
+
+/* ---------------------------------------------------------------------------
+ * CSC SpMV:  y .= 0; for j = _, i = _; y[i] += A[i, j] * x[j]
+ * test/reference64/typical/typical_spmv_csc.txt:26-83 — outer loop over the
+ * Dense level (columns j ascending), inner walk of the SparseList fiber,
+ * `y[i] = x[j] * A[i,j] + y[i]` (line 62).
+ * ------------------------------------------------------------------------- */
+#define DEF_SPMV_CSC(T, SFX)                                                       \
+  void or_spmv_csc_##SFX(i64 n_inner, i64 n_outer, const i64* ptr, const i64* idx, \
+                         const T* val, int base, const T* x, T* y) {              \
+    for (i64 i = 0; i < n_inner; i++) y[i] = (T)0;                                 \
+    for (i64 j = 0; j < n_outer; j++) {                                            \
+      T xj = x[j];                                                                 \
+      for (i64 q = ptr[j] - base; q < ptr[j + 1] - base; q++) {                    \
+        i64 i = idx[q] - base;                                                     \
+        y[i] = xj * val[q] + y[i];                                                 \
+      }                                                                            \
+    }                                                                              \
+  }
+DEF_SPMV_CSC(double, f64)
+DEF_SPMV_CSC(float, f32)
+
+/* ---------------------------------------------------------------------------
+ * CSR SpMV over swizzle(A, 2, 1):  y .= 0; for i = parallel(_), j = _;
+ * y[i] += A[j, i] * x[j]
+ * test/reference64/parallel/parallel_spmv.txt:36-76 — thread tid (1-based)
+ * of n takes rows fld(m*(tid-1), n)+1 .. fld(m*tid, n) (static schedule,
+ * src/architecture.jl:689-719); per row, `y[i] = A[q]*x[idx[q]] + y[i]` with
+ * q ascending (line 58).  nthreads <= 1 is the serial program
+ * (benchmark/benchmarks.jl:365-376).
+ * ------------------------------------------------------------------------- */
+#define DEF_SPMV_CSR(T, SFX)                                                          \
+  static void spmv_csr_rows_##SFX(i64 r0, i64 r1, const i64* ptr, const i64* idx,     \
+                                  const T* val, int base, const T* x, T* y) {         \
+    for (i64 i = r0; i < r1; i++) {                                                   \
+      T acc = (T)0;                                                                   \
+      for (i64 q = ptr[i] - base; q < ptr[i + 1] - base; q++)                         \
+        acc = val[q] * x[idx[q] - base] + acc;                                        \
+      y[i] = acc;                                                                     \
+    }                                                                                 \
+  }                                                                                   \
+  void or_spmv_csr_##SFX(i64 n_outer, i64 n_inner, const i64* ptr, const i64* idx,    \
+                         const T* val, int base, const T* x, T* y, int nthreads) {    \
+    (void)n_inner;                                                                    \
+    if (nthreads <= 1) {                                                              \
+      spmv_csr_rows_##SFX(0, n_outer, ptr, idx, val, base, x, y);                     \
+      return;                                                                         \
+    }                                                                                 \
+    _Pragma("omp parallel for schedule(static, 1) num_threads(nthreads)")            \
+    for (int tid = 1; tid <= nthreads; tid++) {                                       \
+      i64 r0 = fld(n_outer * (tid - 1), nthreads);                                    \
+      i64 r1 = fld(n_outer * tid, nthreads);                                          \
+      spmv_csr_rows_##SFX(r0, r1, ptr, idx, val, base, x, y);                         \
+    }                                                                                 \
+  }
+DEF_SPMV_CSR(double, f64)
+DEF_SPMV_CSR(float, f32)
+
+/* ---------------------------------------------------------------------------
+ * CSR SpMM:  @einsum C[i,k] += A[i,j]*B[j,k]  (src/interface/einsum.jl:190),
+ * with A read row-wise.  Rows are split across threads like parallel_spmv.txt
+ * (test/reference64/parallel/parallel_spmms_no_atomics_1.txt: the parallel
+ * loop is over the output rows, no atomics).  For each (i, k) the sum runs
+ * over the stored j of row i in ascending order: C[i,k] = A[q]*B[j,k] + C[i,k].
+ * B is n_inner x k and C is n_outer x k, both row-major.
+ * ------------------------------------------------------------------------- */
+#define DEF_SPMM_CSR(T, SFX)                                                           \
+  static void spmm_csr_rows_##SFX(i64 r0, i64 r1, i64 k, const i64* ptr, const i64* idx,\
+                                  const T* val, int base, const T* B, T* C) {          \
+    for (i64 i = r0; i < r1; i++) {                                                    \
+      T* c = C + i * k;                                                                \
+      for (i64 kk = 0; kk < k; kk++) c[kk] = (T)0;                                     \
+      for (i64 q = ptr[i] - base; q < ptr[i + 1] - base; q++) {                        \
+        const T a = val[q];                                                            \
+        const T* b = B + (idx[q] - base) * k;                                          \
+        for (i64 kk = 0; kk < k; kk++) c[kk] = a * b[kk] + c[kk];                      \
+      }                                                                                \
+    }                                                                                  \
+  }                                                                                    \
+  void or_spmm_csr_##SFX(i64 n_outer, i64 n_inner, i64 k, const i64* ptr, const i64* idx,\
+                         const T* val, int base, const T* B, T* C, int nthreads) {     \
+    (void)n_inner;                                                                     \
+    if (nthreads <= 1) {                                                               \
+      spmm_csr_rows_##SFX(0, n_outer, k, ptr, idx, val, base, B, C);                   \
+      return;                                                                          \
+    }                                                                                  \
+    _Pragma("omp parallel for schedule(static, 1) num_threads(nthreads)")             \
+    for (int tid = 1; tid <= nthreads; tid++) {                                        \
+      i64 r0 = fld(n_outer * (tid - 1), nthreads);                                     \
+      i64 r1 = fld(n_outer * tid, nthreads);                                           \
+      spmm_csr_rows_##SFX(r0, r1, k, ptr, idx, val, base, B, C);                       \
+    }                                                                                  \
+  }
+DEF_SPMM_CSR(double, f64)
+DEF_SPMM_CSR(float, f32)
+
+/* ---------------------------------------------------------------------------
+ * SDDMM:  C[i,j] = A[i,j] * sum_k X[i,k]*Y[j,k] on pattern(A)
+ * The fused plan of compute(lazy(A) .* (lazy(X)*lazy(Yt)))
+ * (benchmark/benchmarks.jl:141-168) is the @finch nest
+ *   for i, j (stored in A), k: C[i,j] += A[i,j] * X[i,k] * Y[j,k].
+ * There is no golden for it; this restatement sums the dot over k ascending
+ * and scales once (the order the survey of the fused plan leaves open), and
+ * is pinned by the closed form A o (X Y^T) in FP64.  X is n_outer x k and Y
+ * n_inner x k, row-major.  Serial: Finch's SparseList output is not
+ * concurrent (src/tensors/levels/sparse_list_levels.jl:185-188).
+ * ------------------------------------------------------------------------- */
+#define DEF_SDDMM(T, SFX)                                                               \
+  void or_sddmm_csr_##SFX(i64 n_outer, i64 n_inner, i64 k, const i64* ptr, const i64* idx,\
+                          const T* val, int base, const T* X, const T* Y, T* out) {     \
+    (void)n_inner;                                                                      \
+    for (i64 i = 0; i < n_outer; i++) {                                                 \
+      const T* xr = X + i * k;                                                          \
+      for (i64 q = ptr[i] - base; q < ptr[i + 1] - base; q++) {                         \
+        const T* yr = Y + (idx[q] - base) * k;                                          \
+        T acc = (T)0;                                                                   \
+        for (i64 kk = 0; kk < k; kk++) acc = xr[kk] * yr[kk] + acc;                     \
+        out[q] = val[q] * acc;                                                          \
+      }                                                                                 \
+    }                                                                                   \
+  }
+DEF_SDDMM(double, f64)
+DEF_SDDMM(float, f32)
+
+/* ---------------------------------------------------------------------------
+ * Element-wise coiteration, fiber by fiber:
+ *   C .= 0; for p = _, i = _; C[i, p] = A[i, p] op B[i, p]
+ * test/reference64/typical/typical_merge_twofinger.txt:45-181 (two-finger
+ * merge of two SparseList fibers, appending to the output SparseList with
+ * `z_lvl_qos += 1`, then the ptr prefix sum of lines 182-186).
+ *   op 0 (MUL): 0 annihilates `*` (src/symbolic/symbolic.jl:147), so only
+ *               coordinates stored in both survive: value a*b;
+ *   op 1 (ADD): union; both -> b + a (golden line 63 writes y + x), one-sided
+ *               -> the stored value (a + 0 == a);
+ *   op 2 (MAX): union; both -> max(a, b), one-sided -> max(v, 0) (fill 0).
+ * Returns nnz(C); cptr gets n_outer+1 entries in `base`; cidx/cval need room
+ * for nnz(A)+nnz(B).
+ * ------------------------------------------------------------------------- */
+#define DEF_EWISE(T, SFX)                                                                 \
+  static inline T ew_max_##SFX(T a, T b) { return a < b ? b : a; }                        \
+  i64 or_ewise_##SFX(int op, i64 n_outer, const i64* aptr, const i64* aidx, const T* aval,\
+                     const i64* bptr, const i64* bidx, const T* bval, int base, i64* cptr,\
+                     i64* cidx, T* cval) {                                                \
+    i64 qos = 0;                                                                          \
+    cptr[0] = base;                                                                       \
+    for (i64 p = 0; p < n_outer; p++) {                                                   \
+      i64 qa = aptr[p] - base, qa_stop = aptr[p + 1] - base;                              \
+      i64 qb = bptr[p] - base, qb_stop = bptr[p + 1] - base;                              \
+      while (qa < qa_stop && qb < qb_stop) {                                              \
+        i64 ia = aidx[qa], ib = bidx[qb];                                                 \
+        i64 i = ia < ib ? ia : ib;                                                        \
+        if (ia == i && ib == i) {                                                         \
+          T a = aval[qa], b = bval[qb];                                                   \
+          cidx[qos] = i;                                                                  \
+          cval[qos] = op == 0 ? a * b : (op == 1 ? b + a : ew_max_##SFX(a, b));           \
+          qos++; qa++; qb++;                                                              \
+        } else if (ia == i) {                                                             \
+          if (op != 0) {                                                                  \
+            cidx[qos] = i;                                                                \
+            cval[qos] = op == 1 ? aval[qa] : ew_max_##SFX(aval[qa], (T)0);                \
+            qos++;                                                                        \
+          }                                                                               \
+          qa++;                                                                           \
+        } else {                                                                          \
+          if (op != 0) {                                                                  \
+            cidx[qos] = i;                                                                \
+            cval[qos] = op == 1 ? bval[qb] : ew_max_##SFX((T)0, bval[qb]);                \
+            qos++;                                                                        \
+          }                                                                               \
+          qb++;                                                                           \
+        }                                                                                 \
+      }                                                                                   \
+      if (op != 0) {                                                                      \
+        for (; qa < qa_stop; qa++, qos++) {                                               \
+          cidx[qos] = aidx[qa];                                                           \
+          cval[qos] = op == 1 ? aval[qa] : ew_max_##SFX(aval[qa], (T)0);                  \
+        }                                                                                 \
+        for (; qb < qb_stop; qb++, qos++) {                                               \
+          cidx[qos] = bidx[qb];                                                           \
+          cval[qos] = op == 1 ? bval[qb] : ew_max_##SFX((T)0, bval[qb]);                  \
+        }                                                                                 \
+      }                                                                                   \
+      cptr[p + 1] = qos + base;                                                           \
+    }                                                                                     \
+    return qos;                                                                           \
+  }
+DEF_EWISE(double, f64)
+DEF_EWISE(float, f32)
+DEF_EWISE(int32_t, i32)
+
+/* ---------------------------------------------------------------------------
+ * Gustavson SpGEMM, one output fiber at a time:
+ *   C .= 0; for j = _; w .= 0; for k = _, i = _; w[i] += A[i,k]*B[k,j];
+ *                      for i = _; C[i,j] = w[i]
+ * docs/examples/spgemm.jl:59-76; generated code
+ * test/reference64/kernels/gustavsons.jl:52-182: a SparseByteMap workspace
+ * (value vector w, touched flags tbl, list srt of touched coordinates), the
+ * update `w[i] = A[i,k]*A[k,j] + w[i]` (line 141) for the stored k of the
+ * fiber in ascending order and the stored i of fiber k, `sort!(srt)` (line
+ * 182), then a copy of the touched entries into the output SparseList.  Here
+ * fibers are CSR rows (the same nest over swizzled operands): for row i of A,
+ * for stored k (ascending), for stored j of row k of B.
+ * Semiring 0: plus-times (w starts at 0; `w = a*b + w`).
+ * Semiring 1: min-plus (w starts at +Inf; `w = min(w, a + b)`), the
+ *             tropical product with fill Inf (src/symbolic/symbolic.jl:146).
+ * Int32 arithmetic wraps, as Julia's Int32 does.
+ * ------------------------------------------------------------------------- */
+static int cmp_i64(const void* a, const void* b) {
+  i64 x = *(const i64*)a, y = *(const i64*)b;
+  return (x > y) - (x < y);
+}
+
+/* Symbolic pass: structural nnz of each output row; cptr gets the prefix sum. */
+i64 or_spgemm_symbolic(i64 m, i64 n, const i64* aptr, const i64* aidx, const i64* bptr,
+                       const i64* bidx, int base, i64* cptr) {
+  unsigned char* tbl = (unsigned char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  i64* srt = (i64*)malloc(sizeof(i64) * (size_t)(n > 0 ? n : 1));
+  i64 total = 0;
+  cptr[0] = base;
+  for (i64 i = 0; i < m; i++) {
+    i64 fill = 0;
+    for (i64 qa = aptr[i] - base; qa < aptr[i + 1] - base; qa++) {
+      i64 k = aidx[qa] - base;
+      for (i64 qb = bptr[k] - base; qb < bptr[k + 1] - base; qb++) {
+        i64 j = bidx[qb] - base;
+        if (!tbl[j]) { tbl[j] = 1; srt[fill++] = j; }
+      }
+    }
+    for (i64 t = 0; t < fill; t++) tbl[srt[t]] = 0;
+    total += fill;
+    cptr[i + 1] = total + base;
+  }
+  free(tbl);
+  free(srt);
+  return total;
+}
+
+#define DEF_SPGEMM(T, SFX, INF)                                                          \
+  void or_spgemm_##SFX(int semiring, i64 m, i64 n, const i64* aptr, const i64* aidx,     \
+                       const T* aval, const i64* bptr, const i64* bidx, const T* bval,   \
+                       int base, const i64* cptr, i64* cidx, T* cval) {                  \
+    T* w = (T*)malloc(sizeof(T) * (size_t)(n > 0 ? n : 1));                              \
+    unsigned char* tbl = (unsigned char*)calloc((size_t)(n > 0 ? n : 1), 1);             \
+    i64* srt = (i64*)malloc(sizeof(i64) * (size_t)(n > 0 ? n : 1));                      \
+    const T init = semiring == 0 ? (T)0 : (INF);                                         \
+    for (i64 j = 0; j < n; j++) w[j] = init;                                             \
+    for (i64 i = 0; i < m; i++) {                                                        \
+      i64 fill = 0;                                                                      \
+      for (i64 qa = aptr[i] - base; qa < aptr[i + 1] - base; qa++) {                     \
+        i64 k = aidx[qa] - base;                                                         \
+        T a = aval[qa];                                                                  \
+        for (i64 qb = bptr[k] - base; qb < bptr[k + 1] - base; qb++) {                   \
+          i64 j = bidx[qb] - base;                                                       \
+          if (semiring == 0) {                                                           \
+            w[j] = SPG_MUL_##SFX(a, bval[qb]) + w[j];                                    \
+          } else {                                                                       \
+            T s = a + bval[qb];                                                          \
+            w[j] = s < w[j] ? s : w[j];                                                  \
+          }                                                                              \
+          if (!tbl[j]) { tbl[j] = 1; srt[fill++] = j; }                                  \
+        }                                                                                \
+      }                                                                                  \
+      qsort(srt, (size_t)fill, sizeof(i64), cmp_i64);                                    \
+      i64 q = cptr[i] - base;                                                            \
+      for (i64 t = 0; t < fill; t++, q++) {                                              \
+        i64 j = srt[t];                                                                  \
+        cidx[q] = j + base;                                                              \
+        cval[q] = w[j];                                                                  \
+        w[j] = init;                                                                     \
+        tbl[j] = 0;                                                                      \
+      }                                                                                  \
+    }                                                                                    \
+    free(w);                                                                             \
+    free(tbl);                                                                           \
+    free(srt);                                                                           \
+  }
+/* Int32 products and sums wrap modulo 2^32 like Julia's Int32. */
+#define SPG_MUL_i32(a, b) ((int32_t)((uint32_t)(a) * (uint32_t)(b)))
+#define SPG_MUL_f32(a, b) ((a) * (b))
+#define SPG_MUL_f64(a, b) ((a) * (b))
+static inline int32_t wrap_add_i32(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+DEF_SPGEMM(float, f32, INFINITY)
+DEF_SPGEMM(double, f64, INFINITY)
+
+/* Int32 plus-times with explicit wrapping adds (signed overflow is undefined
+ * in C, defined modulo 2^32 in Julia). */
+void or_spgemm_i32(i64 m, i64 n, const i64* aptr, const i64* aidx, const int32_t* aval,
+                   const i64* bptr, const i64* bidx, const int32_t* bval, int base,
+                   const i64* cptr, i64* cidx, int32_t* cval) {
+  int32_t* w = (int32_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+  unsigned char* tbl = (unsigned char*)calloc((size_t)(n > 0 ? n : 1), 1);
+  i64* srt = (i64*)malloc(sizeof(i64) * (size_t)(n > 0 ? n : 1));
+  for (i64 i = 0; i < m; i++) {
+    i64 fill = 0;
+    for (i64 qa = aptr[i] - base; qa < aptr[i + 1] - base; qa++) {
+      i64 k = aidx[qa] - base;
+      int32_t a = aval[qa];
+      for (i64 qb = bptr[k] - base; qb < bptr[k + 1] - base; qb++) {
+        i64 j = bidx[qb] - base;
+        w[j] = wrap_add_i32(SPG_MUL_i32(a, bval[qb]), w[j]);
+        if (!tbl[j]) { tbl[j] = 1; srt[fill++] = j; }
+      }
+    }
+    qsort(srt, (size_t)fill, sizeof(i64), cmp_i64);
+    i64 q = cptr[i] - base;
+    for (i64 t = 0; t < fill; t++, q++) {
+      i64 j = srt[t];
+      cidx[q] = j + base;
+      cval[q] = w[j];
+      w[j] = 0;
+      tbl[j] = 0;
+    }
+  }
+  free(w);
+  free(tbl);
+  free(srt);
+}
+
+/* Number of host threads OpenMP would use (for reporting). */
+int or_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

@@ -1,0 +1,252 @@
+"""GPU parity: SpMV (CSR/CSC), SpMM and SDDMM through the C ABI against the oracle.
+
+Tolerances (stated by the backend's contract): FP64 results relative error
+<= 1e-12 against the FP64 oracle; FP32 results normwise error <= 1e-5 against
+an FP64 oracle run on the same (FP32-rounded) inputs; integer-exact goldens
+bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from conftest import dense_to_csc
+from finchgpu import ops, synth
+
+pytestmark = pytest.mark.gpu
+
+FP64_REL = 1e-12
+FP32_NORMWISE = 1e-5
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return np.linalg.norm(got - ref) / (den if den > 0 else 1.0)
+
+
+def up(a, dev, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(dev)
+
+
+def operand(ptr, idx, val, n_outer, n_inner, dev, itype=torch.int64, ptype=None, base=1):
+    ptype = ptype or itype
+    return ops.Compressed(up(np.asarray(ptr) - (1 - base), dev, ptype),
+                          up(np.asarray(idx) - (1 - base), dev, itype), up(val, dev),
+                          n_outer, n_inner, base)
+
+
+IDX = [(torch.int32, torch.int32), (torch.int64, torch.int64), (torch.int64, torch.int32)]
+
+
+# ---- goldens -------------------------------------------------------------------
+
+def test_spmv_csc_golden(dev, goldens):
+    g = goldens["spmv_csc"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    A = operand(ptr, idx, val, 3, 4, dev)
+    y = ops.spmv_csc(A, up(np.array(g["x"]), dev)).cpu().numpy()
+    # typical_spmv_csc.txt:92; each y[i] here is summed by one lane in column order.
+    assert y.tolist() == g["y"]
+
+
+def test_spmv_csr_golden_transpose(dev, goldens):
+    g = goldens["spmv_transpose"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    A = operand(ptr, idx, val, 2, 2, dev)
+    assert ops.spmv_csr(A, up(np.array(g["x"]), dev)).cpu().tolist() == g["y"]
+
+
+def test_zero_based_buffers(dev, goldens):
+    g = goldens["zero_based"]
+    A = ops.Compressed(up(np.array(g["ptr"]), dev), up(np.array(g["idx"]), dev),
+                       up(np.array(g["val"]), dev), g["n"], g["m"], base=0)
+    dense = np.array(g["A_dense"])
+    for j in range(g["n"]):
+        e = up(np.eye(g["n"])[j], dev)
+        assert ops.spmv_csc(A, e).cpu().tolist() == dense[:, j].tolist()
+
+
+# ---- config 1: 10k x 10k FP64, density 1e-3, CSC and swizzled CSR ----------------
+
+@pytest.mark.parametrize("itype,ptype", IDX)
+@pytest.mark.parametrize("base", [1, 0])
+def test_config1_fp64(dev, itype, ptype, base):
+    n = 10_000
+    ptr, idx, val = synth.fsprand(n, n, 1e-3, seed=0)
+    rng = np.random.default_rng(0)
+    x = 2.0 * rng.random(n) - 1.0
+    A = operand(ptr, idx, val, n, n, dev, itype, ptype, base)
+    # CSC: y = A x (typical_spmv_csc.txt order).
+    y = ops.spmv_csc(A, up(x, dev)).cpu().numpy()
+    ref = oracle.spmv_csc(n, ptr, idx, val, x)
+    assert rel_err(y, ref) <= FP64_REL
+    np.testing.assert_allclose(y, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # CSR = swizzle(A, 2, 1) over the same buffers: y = A^T x.
+    yt = ops.spmv_csr(A, up(x, dev)).cpu().numpy()
+    reft = oracle.spmv_csr(ptr, idx, val, x)
+    assert rel_err(yt, reft) <= FP64_REL
+    np.testing.assert_allclose(yt, reft, rtol=1e-12, atol=1e-12 * np.abs(reft).max())
+
+
+# ---- edge shapes ---------------------------------------------------------------------
+
+def _edge_cases():
+    rng = np.random.default_rng(11)
+    cases = {}
+    cases["empty"] = (np.ones(6, np.int64), np.zeros(0, np.int64), np.zeros(0), 5, 7)
+    # One 100k-long row between empty rows: spans many tiles (carry fix-up).
+    L = 100_000
+    ptr = np.array([1, 1, 1 + L, 1 + L, 1 + L], np.int64)
+    idx = np.sort(rng.choice(200_000, L, replace=False)) + 1
+    cases["long_row"] = (ptr, idx, rng.standard_normal(L), 4, 200_000)
+    # 30k empty rows, then a few dense ones, then 20k trailing empty rows:
+    # windows larger than the LDS cache take the global path.
+    m = 50_003
+    lens = np.zeros(m, np.int64)
+    lens[30_000:30_003] = [5, 700, 1]
+    ptr = np.concatenate([[0], np.cumsum(lens)]) + 1
+    idx = np.concatenate([np.sort(rng.choice(1000, l, replace=False)) for l in lens if l]) + 1
+    cases["sparse_rows"] = (ptr, idx, rng.standard_normal(idx.size), m, 1000)
+    # Every row length from 0 to 300 (runs and tiles end at every offset).
+    lens = np.arange(301)
+    rng.shuffle(lens)
+    ptr = np.concatenate([[0], np.cumsum(lens)]) + 1
+    idx = np.concatenate([np.sort(rng.choice(400, l, replace=False)) for l in lens if l]) + 1
+    cases["ragged"] = (ptr, idx, rng.standard_normal(idx.size), lens.size, 400)
+    cases["single"] = (np.array([1, 2]), np.array([3]), np.array([2.5]), 1, 3)
+    return cases
+
+
+EDGE = _edge_cases()
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spmv_edges(dev, name, dtype):
+    ptr, idx, val, m, n = EDGE[name]
+    val = val.astype(dtype)
+    x = np.random.default_rng(1).standard_normal(n).astype(dtype)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    y = ops.spmv_csr(A, up(x, dev)).cpu().numpy()
+    ref64 = oracle.spmv_csr(ptr, idx, val.astype(np.float64), x.astype(np.float64))
+    tol = FP64_REL if dtype == np.float64 else FP32_NORMWISE
+    assert y.shape == (m,)
+    assert rel_err(y, ref64) <= tol
+    assert np.all(y[np.diff(ptr) == 0] == 0)
+    # CSC reading of the same buffers.
+    xc = np.random.default_rng(2).standard_normal(m).astype(dtype)
+    yc = ops.spmv_csc(A, up(xc, dev)).cpu().numpy()
+    refc = oracle.spmv_csc(n, ptr, idx, val.astype(np.float64), xc.astype(np.float64))
+    assert rel_err(yc, refc) <= tol
+
+
+def test_spmv_deterministic(dev):
+    ptr, idx, val, m, n = EDGE["long_row"]
+    A = operand(ptr, idx, val.astype(np.float32), m, n, dev, torch.int32)
+    x = up(np.random.default_rng(3).standard_normal(n).astype(np.float32), dev)
+    y0 = ops.spmv_csr(A, x)
+    for _ in range(3):
+        assert torch.equal(ops.spmv_csr(A, x), y0)
+
+
+# ---- SpMM -------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("k", [1, 3, 8, 16, 32, 64, 100, 128, 300])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmm_widths(dev, k, dtype):
+    ptr, idx, val, m, n = EDGE["ragged"]
+    val = val.astype(dtype)
+    B = synth.dense(n, k, seed=k, dtype=dtype)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    C = ops.spmm_csr(A, up(B, dev)).cpu().numpy()
+    ref = oracle.spmm_csr(ptr, idx, val.astype(np.float64), B.astype(np.float64))
+    tol = FP64_REL if dtype == np.float64 else FP32_NORMWISE
+    assert rel_err(C, ref) <= tol
+
+
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_spmm_layouts_and_long_rows(dev, layout):
+    ptr, idx, val, m, n = EDGE["long_row"]
+    val = val.astype(np.float32)
+    B = synth.dense(n, 64, seed=5)
+    A = operand(ptr, idx, val, m, n, dev, torch.int64)
+    Bd = up(B, dev) if layout == "row" else up(np.asfortranarray(B).T, dev).t()
+    C = ops.spmm_csr(A, Bd).cpu().numpy()
+    ref = oracle.spmm_csr(ptr, idx, val.astype(np.float64), B.astype(np.float64))
+    assert rel_err(C, ref) <= FP32_NORMWISE
+
+
+# ---- SDDMM ------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("k", [4, 32, 64, 7])
+@pytest.mark.parametrize("name", ["ragged", "long_row", "sparse_rows"])
+def test_sddmm(dev, k, name):
+    ptr, idx, val, m, n = EDGE[name]
+    val = val.astype(np.float32)
+    X = synth.dense(m, k, seed=1)
+    Y = synth.dense(n, k, seed=2)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    out = ops.sddmm_csr(A, up(X, dev), up(Y, dev)).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, val.astype(np.float64), X.astype(np.float64),
+                           Y.astype(np.float64))
+    assert rel_err(out, ref) <= FP32_NORMWISE
+
+
+def test_sddmm_fp64(dev):
+    ptr, idx, val, m, n = EDGE["ragged"]
+    X = synth.dense(m, 16, seed=1, dtype=np.float64)
+    Y = synth.dense(n, 16, seed=2, dtype=np.float64)
+    A = operand(ptr, idx, val, m, n, dev, torch.int64)
+    out = ops.sddmm_csr(A, up(X, dev), up(Y, dev)).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, val, X, Y)
+    assert rel_err(out, ref) <= FP64_REL
+
+
+# ---- config 2 at full size (1M x 1M, ~16M nnz, Zipf(1.5) rows) ----------------------------
+
+@pytest.fixture(scope="module")
+def config2():
+    ptr, idx, val = synth.powerlaw_csr(1_000_000, 1_000_000, 16_000_000, seed=0)
+    return ptr, idx, val
+
+
+def test_config2_spmv_spmm_sddmm(dev, config2):
+    ptr, idx, val = config2
+    m = n = 1_000_000
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    nth = max(1, min(16, oracle.max_threads()))
+    x = synth.dense(n, 1, seed=1)[:, 0].copy()
+    y = ops.spmv_csr(A, up(x, dev)).cpu().numpy()
+    ref = oracle.spmv_csr(ptr, idx, val.astype(np.float64), x.astype(np.float64), nthreads=nth)
+    assert rel_err(y, ref) <= FP32_NORMWISE
+    B = synth.dense(n, 64, seed=2)
+    C = ops.spmm_csr(A, up(B, dev)).cpu().numpy()
+    refC = oracle.spmm_csr(ptr, idx, val.astype(np.float64), B.astype(np.float64), nthreads=nth)
+    assert rel_err(C, refC) <= FP32_NORMWISE
+    del refC, C
+    X = synth.dense(m, 64, seed=3)
+    out = ops.sddmm_csr(A, up(X, dev), up(B, dev)).cpu().numpy()
+    refS = oracle.sddmm_csr(ptr, idx, val.astype(np.float64), X.astype(np.float64),
+                            B.astype(np.float64))
+    assert rel_err(out, refS) <= FP32_NORMWISE
+
+
+def test_bf16_values_fp32_accumulate(dev, config2):
+    ptr, idx, val = config2
+    n = 1_000_000
+    A = operand(ptr, idx, val, n, n, dev, torch.int32)
+    Ab = A.to_values(torch.bfloat16)
+    B = up(synth.dense(n, 64, seed=2), dev)
+    C32 = ops.spmm_csr(A, B).cpu().numpy()
+    Cb = ops.spmm_csr(Ab, B).cpu().numpy()
+    # Storage rounding of values only: normwise error vs FP32 values <= 1e-2.
+    assert rel_err(Cb, C32) <= 1e-2
+    # Against the oracle on the bf16-rounded values it is an FP32 sum again.
+    vb = Ab.val.float().cpu().numpy().astype(np.float64)
+    refb = oracle.spmv_csr(ptr, idx, vb, B[:, 0].cpu().numpy().astype(np.float64), nthreads=8)
+    assert rel_err(Cb[:, 0], refb) <= FP32_NORMWISE

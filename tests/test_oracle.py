@@ -1,0 +1,220 @@
+"""Pins the CPU oracle (oracle/finch_oracle.c) before it is trusted as the checker.
+
+Each oracle kernel is checked against Finch's own golden vectors where the
+reference holds one, and otherwise against independent computations: scipy
+sparse products, brute-force dense loops and closed forms.  The checks are
+chosen so that plausible mistakes (a transposed operand, a dropped term, a
+wrong fill, a reordered sum, a lost index base) fail at least one of them.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from conftest import dense_to_csc, read_mtx
+
+
+# ---- SpMV -------------------------------------------------------------------
+
+def test_spmv_csc_golden_bit_exact(goldens):
+    g = goldens["spmv_csc"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    y = oracle.spmv_csc(len(g["A_dense"]), ptr, idx, val, np.array(g["x"]))
+    # typical_spmv_csc.txt:92 — bit-exact, including 15.400000000000002
+    # (x*a rounded before the add; an FMA would give a different last bit).
+    assert y.tolist() == g["y"]
+
+
+def test_spmv_row_dot_on_csc_buffers_is_transpose(goldens):
+    # parallel_spmv_atomics.txt:117 — reading CSC buffers fibre-wise with a
+    # dot product gives A^T x = [4, 6]; the scatter form gives A x = [3, 7].
+    g = goldens["spmv_transpose"]
+    A = np.array(g["A_dense"])
+    ptr, idx, val = dense_to_csc(A)
+    assert oracle.spmv_csr(ptr, idx, val, np.array(g["x"])).tolist() == g["y"]
+    assert oracle.spmv_csc(2, ptr, idx, val, np.array(g["x"])).tolist() == (A @ g["x"]).tolist()
+    assert oracle.spmv_csc(2, ptr, idx, val, np.array(g["x"])).tolist() != g["y"]
+
+
+def test_zero_based_buffers(goldens):
+    # issue_tests.jl:388-404 — PlusOneVector views of 0-based buffers.
+    g = goldens["zero_based"]
+    A = np.array(g["A_dense"])
+    ptr, idx, val = np.array(g["ptr"]), np.array(g["idx"]), np.array(g["val"])
+    for j in range(g["n"]):
+        e = np.eye(g["n"])[j]
+        np.testing.assert_array_equal(oracle.spmv_csc(g["m"], ptr, idx, val, e, base=0), A[:, j])
+    ptr1, idx1, val1 = dense_to_csc(A)
+    np.testing.assert_array_equal(ptr + 1, ptr1)
+    np.testing.assert_array_equal(idx + 1, idx1)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-13), (np.float32, 1e-5)])
+@pytest.mark.parametrize("nthreads", [1, 3])
+def test_spmv_csr_vs_scipy(dtype, tol, nthreads):
+    rng = np.random.default_rng(1)
+    S = sp.random(300, 200, density=0.05, format="csr", random_state=rng, dtype=np.float64)
+    S.data = S.data.astype(dtype)
+    x = rng.standard_normal(200).astype(dtype)
+    y = oracle.spmv_csr(S.indptr + 1, S.indices + 1, S.data, x, nthreads=nthreads)
+    ref = S.astype(np.float64) @ x.astype(np.float64)
+    assert np.linalg.norm(y - ref) <= tol * np.linalg.norm(ref)
+    # Threaded static blocks give the same bits as serial (same per-row order).
+    assert np.array_equal(y, oracle.spmv_csr(S.indptr + 1, S.indices + 1, S.data, x))
+
+
+def test_spmv_csc_vs_scipy_and_empty_columns():
+    rng = np.random.default_rng(2)
+    S = sp.random(150, 90, density=0.03, format="csc", random_state=rng)
+    S[:, 5] = 0
+    S.eliminate_zeros()
+    x = rng.standard_normal(90)
+    y = oracle.spmv_csc(150, S.indptr + 1, S.indices + 1, S.data, x)
+    np.testing.assert_allclose(y, S @ x, rtol=1e-13, atol=1e-13)
+
+
+# ---- SpMM / SDDMM -------------------------------------------------------------
+
+@pytest.mark.parametrize("nthreads", [1, 4])
+def test_spmm_vs_scipy(nthreads):
+    rng = np.random.default_rng(3)
+    S = sp.random(120, 80, density=0.1, format="csr", random_state=rng)
+    B = rng.standard_normal((80, 7))
+    C = oracle.spmm_csr(S.indptr + 1, S.indices + 1, S.data, B, nthreads=nthreads)
+    np.testing.assert_allclose(C, S @ B, rtol=1e-12, atol=1e-12)
+    # Column c of SpMM equals SpMV with B[:, c] (linearity check on the oracle itself).
+    for c in range(7):
+        np.testing.assert_array_equal(C[:, c], oracle.spmv_csr(S.indptr + 1, S.indices + 1,
+                                                               S.data, B[:, c].copy()))
+
+
+def test_sddmm_closed_form():
+    rng = np.random.default_rng(4)
+    S = sp.random(60, 50, density=0.1, format="csr", random_state=rng)
+    X = rng.standard_normal((60, 9))
+    Y = rng.standard_normal((50, 9))
+    out = oracle.sddmm_csr(S.indptr + 1, S.indices + 1, S.data, X, Y)
+    dense = (X @ Y.T) * S.toarray()
+    np.testing.assert_allclose(out, dense[S.nonzero()], rtol=1e-12, atol=1e-12)
+    # Dropping the A scale, or swapping the roles of X and Y, must fail.
+    assert not np.allclose(out, (X @ Y.T)[S.nonzero()])
+    assert not np.allclose(out, (S.toarray() * (X @ X[:50].T))[S.nonzero()])
+
+
+# ---- coiteration ---------------------------------------------------------------
+
+def test_merge_goldens(goldens):
+    g = goldens["merge_add"]
+    x, y = g["x"], g["y"]
+    cptr, cidx, cval = oracle.ewise("add", x["ptr"], x["idx"], np.array(x["val"]),
+                                    y["ptr"], y["idx"], np.array(y["val"]))
+    for kind, z in g["z"].items():
+        assert cptr.tolist() == z["ptr"], kind
+        assert cidx.tolist() == z["idx"], kind
+        assert cval.tolist() == z["val"], kind
+
+
+def _set_algebra(op, S, T):
+    """Expected coordinate set and values from dense algebra on stored patterns."""
+    A, B = S.toarray(), T.toarray()
+    sa, sb = S.toarray() != 0, T.toarray() != 0
+    mask = (sa & sb) if op == "mul" else (sa | sb)
+    val = {"mul": A * B, "add": A + B, "max": np.maximum(A, B)}[op]
+    return mask, val
+
+
+@pytest.mark.parametrize("op", ["mul", "add", "max"])
+def test_ewise_set_algebra(op):
+    rng = np.random.default_rng(5)
+    S = sp.random(40, 30, density=0.2, format="csr", random_state=rng)
+    T = sp.random(40, 30, density=0.3, format="csr", random_state=rng)
+    S.data -= 0.5
+    T.data -= 0.5
+    cptr, cidx, cval = oracle.ewise(op, S.indptr + 1, S.indices + 1, S.data,
+                                    T.indptr + 1, T.indices + 1, T.data)
+    mask, val = _set_algebra(op, S, T)
+    rows, cols = np.nonzero(mask)
+    assert cptr[-1] - 1 == mask.sum()
+    np.testing.assert_array_equal(np.diff(cptr), mask.sum(axis=1))
+    np.testing.assert_array_equal(cidx - 1, cols)
+    np.testing.assert_array_equal(cval, val[rows, cols])
+
+
+def test_ewise_max_stores_zero_from_negative_one_sided():
+    # max.(A, B) over the union: a one-sided -2.0 becomes max(-2, 0) = 0, stored.
+    cptr, cidx, cval = oracle.ewise("max", [1, 2], [3], np.array([-2.0]), [1, 1], [],
+                                    np.array([]))
+    assert cptr.tolist() == [1, 2] and cidx.tolist() == [3] and cval.tolist() == [0.0]
+
+
+# ---- SpGEMM --------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", ["west0132.mtx", "lpi_itest6.mtx", "lp_blend.mtx", "Trec4.mtx"])
+def test_gustavson_matches_scipy_on_finch_fixtures(name):
+    # kernel_tests.jl:89-130 checks B == A*A on these matrices (square ones);
+    # here A*A^T covers the rectangular ones too.
+    A = read_mtx(name)
+    if A.dtype.kind == "i":
+        A = A.astype(np.int32)  # integer fixture: exact Int32 plus-times
+    Bm = A if A.shape[0] == A.shape[1] else A.T.tocsr()
+    A.sort_indices()
+    Bm.sort_indices()
+    cptr, cidx, cval = oracle.spgemm("plus_times", A.shape[0], Bm.shape[1], A.indptr + 1,
+                                     A.indices + 1, A.data, Bm.indptr + 1, Bm.indices + 1,
+                                     Bm.data)
+    ref = (A @ Bm).tocsr()
+    ref.sort_indices()
+    # Structure: every touched coordinate is stored, even where the sum cancels.
+    pa, pb = A.copy(), Bm.copy()
+    pa.data[:] = 1
+    pb.data[:] = 1
+    touched = (pa.astype(np.int64) @ pb.astype(np.int64)).tocsr()
+    touched.sort_indices()
+    np.testing.assert_array_equal(cptr - 1, touched.indptr)
+    np.testing.assert_array_equal(cidx - 1, touched.indices)
+    dense = np.zeros(ref.shape)
+    rows = np.repeat(np.arange(ref.shape[0]), np.diff(cptr))
+    dense[rows, cidx - 1] = cval
+    np.testing.assert_allclose(dense, ref.toarray(), rtol=1e-12, atol=1e-12)
+
+
+def test_gustavson_int32_exact_and_wrapping():
+    rng = np.random.default_rng(6)
+    S = sp.random(50, 40, density=0.1, format="csr", random_state=rng)
+    T = sp.random(40, 60, density=0.1, format="csr", random_state=rng)
+    S.data = rng.integers(1, 10, S.nnz).astype(np.int32)
+    T.data = rng.integers(1, 10, T.nnz).astype(np.int32)
+    cptr, cidx, cval = oracle.spgemm("plus_times", 50, 60, S.indptr + 1, S.indices + 1, S.data,
+                                     T.indptr + 1, T.indices + 1, T.data)
+    ref = (S.astype(np.int64) @ T.astype(np.int64)).toarray()
+    rows = np.repeat(np.arange(50), np.diff(cptr))
+    assert np.array_equal(cval.astype(np.int64), ref[rows, cidx - 1])
+    # Int32 overflow wraps as in Julia: 4*2^30 + 4*2^30 = 2^33 == 0 (mod 2^32).
+    _, _, v = oracle.spgemm("plus_times", 1, 1, [1, 3], [1, 2], np.array([4, 4], np.int32),
+                            [1, 2, 3], [1, 1], np.array([2 ** 30, 2 ** 30], np.int32))
+    assert v.tolist() == [0]
+
+
+def test_min_plus_brute_force():
+    # interface_tests.jl:70-84 checks the max-plus product by brute force; the
+    # min-plus product with fill Inf is checked the same way.
+    rng = np.random.default_rng(7)
+    m, l, n = 12, 9, 11
+    S = sp.random(m, l, density=0.3, format="csr", random_state=rng, dtype=np.float32)
+    T = sp.random(l, n, density=0.3, format="csr", random_state=rng, dtype=np.float32)
+    S.data = rng.standard_normal(S.nnz).astype(np.float32)
+    T.data = rng.standard_normal(T.nnz).astype(np.float32)
+    cptr, cidx, cval = oracle.spgemm("min_plus", m, n, S.indptr + 1, S.indices + 1, S.data,
+                                     T.indptr + 1, T.indices + 1, T.data)
+    A = np.full((m, l), np.inf, np.float32)
+    A[S.nonzero()] = S.data
+    B = np.full((l, n), np.inf, np.float32)
+    B[T.nonzero()] = T.data
+    R = np.full((m, n), np.inf, np.float32)
+    for i in range(m):
+        for j in range(n):
+            for k in range(l):
+                R[i, j] = min(R[i, j], np.float32(A[i, k] + B[k, j]))
+    rows = np.repeat(np.arange(m), np.diff(cptr))
+    assert np.array_equal(cval, R[rows, cidx - 1])           # bit-exact
+    assert np.isfinite(R).sum() == cptr[-1] - 1             # pattern = finite entries
