@@ -1,0 +1,577 @@
+// fgx_batch.hip — batched persistent launches (fgx_batch_*) and the device
+// side shared with the serving loop.
+#include "fgx_batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace fgx {
+
+typedef float bf32x4 __attribute__((ext_vector_type(4)));
+
+// ---- ROWS units ----------------------------------------------------------------------
+// SpMV: kSpmvRowGroup lanes per row, strided over the row, then a shuffle tree.
+template <typename TP, typename TI, typename TV>
+__device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_t r1) {
+  constexpr int G = kSpmvRowGroup;
+  const int lane = lane_id();
+  const int g = lane / G, gl = lane % G;
+  const TP* ptr = static_cast<const TP*>(d.ptr);
+  const TI* idx = static_cast<const TI*>(d.idx);
+  const TV* val = static_cast<const TV*>(d.val);
+  const int64_t base = d.base;
+  for (int64_t r = r0 + g; r < r1; r += kWave / G) {
+    const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
+    float acc = 0.f;
+    for (int64_t q = q0 + gl; q < q1; q += G)
+      acc += widen<float>(ld_stream(val + q)) * d.B[static_cast<int64_t>(ld_stream(idx + q)) - base];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (gl == 0) d.C[r] = acc;
+  }
+}
+
+// SpMM: G lanes per row, each lane a 16-byte slice of the dense width; four
+// stored positions in flight per lane.
+template <int G, typename TP, typename TI, typename TV>
+__device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_t r1) {
+  const int lane = lane_id();
+  const int g = lane / G, gl = lane % G;
+  const TP* ptr = static_cast<const TP*>(d.ptr);
+  const TI* idx = static_cast<const TI*>(d.idx);
+  const TV* val = static_cast<const TV*>(d.val);
+  const int64_t base = d.base;
+  const int64_t col0 = static_cast<int64_t>(gl) * 4;
+  const int vc = static_cast<int>(col0 < d.k ? (d.k - col0 < 4 ? d.k - col0 : 4) : 0);
+  for (int64_t r = r0 + g; r < r1; r += kWave / G) {
+    const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    int64_t q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      float b[4][4];
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t j = static_cast<int64_t>(ld_stream(idx + q + u)) - base;
+        v[u] = widen<float>(ld_stream(val + q + u));
+        if (vc > 0) ld_dense<4, float>(d.B + j * d.ldb + col0, vc, b[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) acc[c] += v[u] * b[u][c];
+    }
+    for (; q < q1; q++) {
+      const int64_t j = static_cast<int64_t>(ld_stream(idx + q)) - base;
+      const float v = widen<float>(ld_stream(val + q));
+      float b[4] = {0.f, 0.f, 0.f, 0.f};
+      if (vc > 0) ld_dense<4, float>(d.B + j * d.ldb + col0, vc, b);
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[c] += v * b[c];
+    }
+    if (vc > 0) st_dense<4, float>(d.C + r * d.ldc + col0, vc, acc);
+  }
+}
+
+// ---- SDDMM units: 64 positions per sub-tile, one per lane ----------------------------------
+template <typename TP, typename TI, typename TV>
+__device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int64_t st1,
+                                           int64_t* lds) {
+  const TP* ptr = static_cast<const TP*>(d.ptr);
+  const TI* idx = static_cast<const TI*>(d.idx);
+  const TV* val = static_cast<const TV*>(d.val);
+  const int64_t base = d.base;
+  const int lane = lane_id();
+  for (int64_t st = st0; st < st1; st++) {
+    const int64_t q0 = st * kWave;
+    const int64_t qend = (q0 + kWave < d.nnz) ? q0 + kWave : d.nnz;
+    int64_t r_lo, r_hi;
+    tile_fibers(ptr, base, d.m, q0, qend, &r_lo, &r_hi);
+    const int64_t r_last = r_hi < d.m ? r_hi : d.m - 1;
+    FiberWindow<TP> win = load_window(ptr, base, r_lo, r_last - r_lo + 1, lds, kWinCap);
+    const int64_t q = q0 + lane;
+    const bool in = q < qend;
+    int64_t i = 0, j = 0;
+    float a = 0.f;
+    if (in) {
+      j = static_cast<int64_t>(ld_stream(idx + q)) - base;
+      a = widen<float>(ld_stream(val + q));
+      i = win.find(q);
+    }
+    const float* xr = d.B + i * d.ldb;
+    const float* yr = d.B2 + j * d.ldc;
+    float dot = 0.f;
+    if (d.mfma) {
+      bf32x4 c = {0.f, 0.f, 0.f, 0.f};
+      for (int64_t kk = 0; kk < d.k; kk += 4) {
+        float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), yb = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (in) {
+          xa = *reinterpret_cast<const float4*>(xr + kk);
+          yb = *reinterpret_cast<const float4*>(yr + kk);
+        }
+        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.x, yb.x, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.y, yb.y, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.z, yb.z, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.w, yb.w, c, 0, 0, 0);
+      }
+      const int dd = lane & 3;
+      dot = dd == 0 ? c[0] : (dd == 1 ? c[1] : (dd == 2 ? c[2] : c[3]));
+    } else if (in) {
+      for (int64_t kk = 0; kk < d.k; kk++) dot += xr[kk] * yr[kk];
+    }
+    if (in) d.C[q] = a * dot;
+  }
+}
+
+// ---- TILES units ------------------------------------------------------------------------------
+template <int G, typename TP, typename TI, typename TV>
+__device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64_t t1,
+                                           int64_t* lds, int64_t* carry_row, float* carry_val) {
+  constexpr int U = G < 8 ? G : 8;
+  for (int64_t t = t0; t < t1; t++)
+    spmm_tile<TP, TI, TV, float, G, 4, kTileE, U>(
+        static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+        static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t, lds,
+        carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4));
+}
+
+template <typename TP, typename TI, typename TV>
+__device__ __forceinline__ void tiles_spmv(const BatchDesc& d, int64_t t0, int64_t t1,
+                                           int64_t* lds, int64_t* carry_row, float* carry_val) {
+  for (int64_t t = t0; t < t1; t++)
+    spmv_tile<TP, TI, TV, float, 8>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+                                    static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.B, d.C,
+                                    t, d.vec != 0, lds, carry_row + d.carry_tile0 + t,
+                                    carry_val + d.carry_val0 + t);
+}
+
+// Blocks are dispatched round-robin over the XCDs; renumber them so that
+// consecutive logical blocks (and so consecutive work units of one item)
+// share an XCD and its L2.
+__device__ __forceinline__ int64_t xcd_block() {
+  constexpr int kXcd = 8;
+  const int64_t nb = gridDim.x;
+  const int64_t b = blockIdx.x;
+  if (nb % kXcd != 0) return b;
+  return (b % kXcd) * (nb / kXcd) + b / kXcd;
+}
+
+template <typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(kBatchThreads) void batch_kernel(
+    const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t n_work,
+    int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
+  __shared__ int64_t lds[kWavesPerBlock][kWinCap];
+  const int w = threadIdx.x / kWave;
+  const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t u = xcd_block() * kWavesPerBlock + w; u < n_work; u += nw) {
+    const BatchWork wk = work[u];
+    const BatchDesc d = desc[wk.item];
+    if (wk.kind == BK_ROWS) {
+      if (d.k == 1) {
+        rows_spmv<TP, TI, TV>(d, wk.begin, wk.end);
+      } else {
+        switch (group_for_k(d.k)) {
+          case 2: rows_spmm<2, TP, TI, TV>(d, wk.begin, wk.end); break;
+          case 4: rows_spmm<4, TP, TI, TV>(d, wk.begin, wk.end); break;
+          case 8: rows_spmm<8, TP, TI, TV>(d, wk.begin, wk.end); break;
+          case 16: rows_spmm<16, TP, TI, TV>(d, wk.begin, wk.end); break;
+          case 32: rows_spmm<32, TP, TI, TV>(d, wk.begin, wk.end); break;
+          default: rows_spmm<64, TP, TI, TV>(d, wk.begin, wk.end); break;
+        }
+      }
+    } else if (wk.kind == BK_TILES) {
+      if (d.k == 1) {
+        tiles_spmv<TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val);
+      } else {
+        switch (group_for_k(d.k)) {
+          case 2: tiles_spmm<2, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+          case 4: tiles_spmm<4, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+          case 8: tiles_spmm<8, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+          case 16: tiles_spmm<16, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+          case 32: tiles_spmm<32, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+          default: tiles_spmm<64, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
+        }
+      }
+    } else {
+      sddmm_unit<TP, TI, TV>(d, wk.begin, wk.end, lds[w]);
+    }
+  }
+}
+
+// Carries of TILES units: one wave per unit, lanes over (tile, column).
+__global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
+    const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
+    const int64_t* __restrict__ carry_row, const float* __restrict__ carry_val) {
+  const int w = threadIdx.x / kWave;
+  const int lane = lane_id();
+  const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
+  for (int64_t u = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; u < n_fix; u += nw) {
+    const BatchWork wk = fix[u];
+    const BatchDesc d = desc[wk.item];
+    const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
+    const int64_t W = tile_width_for_k(d.k);
+    const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
+    const int64_t ld = d.k == 1 ? 1 : d.ldc;
+    const int64_t cells = (wk.end - wk.begin) * width;
+    for (int64_t c = lane; c < cells; c += kWave) {
+      const int64_t t = wk.begin + c / width;
+      const int col = static_cast<int>(c % width);
+      fixup_run<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width, col,
+                       d.C, ld, 0, d.k);
+    }
+  }
+}
+
+// Per-item row-length statistics for the autotuner: [0] max row length.
+template <typename TP>
+__global__ __launch_bounds__(256) void row_stats_kernel(const BatchDesc* __restrict__ desc,
+                                                        int64_t* __restrict__ out) {
+  __shared__ int64_t red[4];
+  const BatchDesc d = desc[blockIdx.x];
+  const TP* ptr = static_cast<const TP*>(d.ptr);
+  int64_t mx = 0;
+  for (int64_t r = threadIdx.x; r < d.m; r += blockDim.x) {
+    const int64_t len = pos_at(ptr, d.base, r + 1) - pos_at(ptr, d.base, r);
+    mx = len > mx ? len : mx;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int64_t y = __shfl_xor(mx, o, 64);
+    mx = y > mx ? y : mx;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t m2 = red[0];
+    for (int i = 1; i < 4; i++) m2 = red[i] > m2 ? red[i] : m2;
+    out[blockIdx.x] = m2;
+  }
+}
+
+// ---- host ----------------------------------------------------------------------------------
+template <typename TP, typename TI, typename TV>
+static void launch_batch_typed(const fgx_batch_plan* p, hipStream_t s) {
+  hipLaunchKernelGGL((batch_kernel<TP, TI, TV>), dim3(static_cast<unsigned>(p->grid)),
+                     dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, p->n_work, p->d_carry_row,
+                     p->d_carry_val);
+}
+
+static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
+  if (p->n_work > 0) {
+#define FGX_BL(TV_) FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo), launch_batch_typed<TP, TI, TV_>(p, s))
+    switch (p->val_type) {
+      case FGX_F32: FGX_BL(float); break;
+      case FGX_BF16: FGX_BL(bf16_t); break;
+      case FGX_F16: FGX_BL(f16_t); break;
+      default: return set_error(FGX_EUNSUP, "batch: value type %d", p->val_type);
+    }
+#undef FGX_BL
+    FGX_TRY(launch_check("batch_kernel"));
+  }
+  if (p->n_fix > 0) {
+    const int64_t g = std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+    hipLaunchKernelGGL(batch_fixup_kernel, dim3(static_cast<unsigned>(g)), dim3(kBatchThreads), 0,
+                       s, p->d_desc, p->d_fix, p->n_fix, p->d_carry_row, p->d_carry_val);
+    FGX_TRY(launch_check("batch_fixup_kernel"));
+  }
+  return FGX_OK;
+}
+
+int batch_launch(fgx_batch_plan* p, hipStream_t s) {
+  if (p->use_graph && p->exec) {
+    FGX_HIP(hipGraphLaunch(p->exec, s));
+    return FGX_OK;
+  }
+  return launch_raw(p, s);
+}
+
+static void free_plan_buffers(fgx_batch_plan* p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
+  if (p->d_desc) (void)hipFree(p->d_desc);
+  if (p->d_work) (void)hipFree(p->d_work);
+  if (p->d_fix) (void)hipFree(p->d_fix);
+  if (p->d_carry_row) (void)hipFree(p->d_carry_row);
+  if (p->d_carry_val) (void)hipFree(p->d_carry_val);
+  p->exec = nullptr;
+  p->graph = nullptr;
+  p->capture_stream = nullptr;
+  p->d_desc = nullptr;
+  p->d_work = nullptr;
+  p->d_fix = nullptr;
+  p->d_carry_row = nullptr;
+  p->d_carry_val = nullptr;
+}
+
+template <typename T>
+static int dev_alloc_copy(T** out, const T* src, int64_t n, hipStream_t s) {
+  const size_t bytes = static_cast<size_t>(std::max<int64_t>(n, 1)) * sizeof(T);
+  if (hipMalloc(reinterpret_cast<void**>(out), bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *out = nullptr;
+    return set_error(FGX_EOOM, "batch: device allocation of %zu bytes failed", bytes);
+  }
+  if (n > 0 && src) FGX_HIP(hipMemcpyAsync(*out, src, static_cast<size_t>(n) * sizeof(T),
+                                           hipMemcpyHostToDevice, s));
+  return FGX_OK;
+}
+
+int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const BatchWork* work,
+                 int64_t n_work, int64_t carry_tiles, int64_t carry_vals, int num_cus,
+                 hipStream_t s) {
+  std::vector<BatchWork> fix;
+  for (int64_t u = 0; u < n_work; u++)
+    if (work[u].kind == BK_TILES) fix.push_back(work[u]);
+  FGX_TRY(dev_alloc_copy(&p->d_desc, desc, n_desc, s));
+  FGX_TRY(dev_alloc_copy(&p->d_work, work, n_work, s));
+  FGX_TRY(dev_alloc_copy(&p->d_fix, fix.data(), static_cast<int64_t>(fix.size()), s));
+  FGX_TRY(dev_alloc_copy<int64_t>(&p->d_carry_row, nullptr, carry_tiles, s));
+  FGX_TRY(dev_alloc_copy<float>(&p->d_carry_val, nullptr, carry_vals, s));
+  FGX_HIP(hipStreamSynchronize(s));
+  p->n_work = n_work;
+  p->n_fix = static_cast<int64_t>(fix.size());
+  // A persistent grid: a few blocks per CU, a multiple of the 8 XCDs.
+  const int64_t want = (n_work + kWavesPerBlock - 1) / kWavesPerBlock;
+  int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(num_cus) * 8);
+  if (grid >= 8) grid -= grid % 8;
+  p->grid = std::max<int64_t>(grid, 1);
+  return FGX_OK;
+}
+
+// The autotuner's choice for one item.
+struct TunePick {
+  int kind;          // BK_ROWS or BK_TILES
+  int64_t unit;      // rows (ROWS) or tiles (TILES) per work unit
+};
+
+// Picks row-split vs nnz-split and the work-unit size from the item's shape
+// and row-length statistics.  Row-split keeps one lane group per row with no
+// carries and wins while rows are even; once the longest row is many times
+// the mean (power-law rows), one row would serialise a group for too long
+// and the nnz-balanced tiles take over.  Units aim at ~256 KB of traffic so
+// that thousands of them spread over the device.
+static TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
+                              int flags) {
+  const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
+  bool tiles = max_row > 256 && static_cast<double>(max_row) > 16.0 * std::max(mean, 1.0);
+  if (flags & FGX_BATCH_FORCE_ROWSPLIT) tiles = false;
+  if (flags & FGX_BATCH_FORCE_NNZSPLIT) tiles = true;
+  const double target = 256.0 * 1024.0;
+  TunePick pk;
+  if (tiles) {
+    const double tile_bytes =
+        static_cast<double>(tile_width_for_k(k)) * (4.0 + vbytes + 4.0 * static_cast<double>(k));
+    pk.kind = BK_TILES;
+    pk.unit = std::max<int64_t>(1, static_cast<int64_t>(target / tile_bytes));
+  } else {
+    const double row_bytes = mean * (4.0 + vbytes + 4.0 * static_cast<double>(k)) + 4.0 * k + 4.0;
+    const int64_t per_iter = k == 1 ? kWave / kSpmvRowGroup : kWave / group_for_k(k);
+    int64_t rows = static_cast<int64_t>(target / std::max(row_bytes, 1.0));
+    rows = std::max<int64_t>(per_iter, (rows / per_iter) * per_iter);
+    pk.kind = BK_ROWS;
+    pk.unit = rows;
+  }
+  return pk;
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" {
+
+int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, int flags,
+                          fgx_batch_plan** out) {
+  clear_error();
+  if (!ctx || !out) return set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL argument");
+  if (count < 0 || (count > 0 && !items)) return set_error(FGX_EINVAL, "fgx_batch_plan_create: bad items");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  auto* p = new fgx_batch_plan();
+  p->device = ctx->device;
+  p->count = count;
+  std::vector<BatchDesc> desc(static_cast<size_t>(count));
+  IdxCombo combo = IDX_32_32;
+  for (int64_t i = 0; i < count; i++) {
+    const fgx_batch_item& it = items[i];
+    int rc = check_compressed(&it.A, "fgx_batch_plan_create");
+    IdxCombo ic;
+    if (rc == FGX_OK) rc = idx_combo(&it.A, &ic);
+    if (rc == FGX_OK && i == 0) {
+      combo = ic;
+      p->val_type = it.A.val_type;
+    }
+    if (rc == FGX_OK && (ic != combo || it.A.val_type != p->val_type))
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: items must share index and value types");
+    if (rc == FGX_OK && it.A.val_type != FGX_F32 && it.A.val_type != FGX_BF16 &&
+        it.A.val_type != FGX_F16)
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: values must be F32, BF16 or F16");
+    if (rc == FGX_OK && (it.n_rhs < 1 || it.n_rhs > 256 || (it.n_rhs > 1 && it.n_rhs % 4 != 0)))
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: n_rhs must be 1 or a multiple of 4 up to 256");
+    if (rc == FGX_OK && (!it.C || (it.A.nnz > 0 && !it.B)))
+      rc = set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL dense operand");
+    if (rc == FGX_OK && it.n_rhs > 1 && (!aligned16(it.B) || !aligned16(it.C)))
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned");
+    if (rc != FGX_OK) {
+      delete p;
+      return rc;
+    }
+    BatchDesc& d = desc[static_cast<size_t>(i)];
+    d = BatchDesc{};
+    d.ptr = it.A.ptr;
+    d.idx = it.A.idx;
+    d.val = it.A.val;
+    d.B = static_cast<const float*>(it.B);
+    d.B2 = nullptr;
+    d.C = static_cast<float*>(it.C);
+    d.m = it.A.n_outer;
+    d.n = it.A.n_inner;
+    d.nnz = it.A.nnz;
+    d.k = it.n_rhs;
+    d.ldb = it.n_rhs;
+    d.ldc = it.n_rhs;
+    d.base = it.A.index_base;
+    d.op = BO_SPMM;
+    d.vec = aligned16(it.A.val) && aligned16(it.A.idx);
+    d.mfma = 0;
+  }
+  p->combo = combo;
+  // Row-length statistics (one block per item).
+  std::vector<int64_t> maxrow(static_cast<size_t>(count), 0);
+  if (count > 0) {
+    BatchDesc* dd = nullptr;
+    int64_t* dstat = nullptr;
+    int rc = dev_alloc_copy(&dd, desc.data(), count, s);
+    if (rc == FGX_OK) rc = dev_alloc_copy<int64_t>(&dstat, nullptr, count, s);
+    if (rc == FGX_OK) {
+      for (int64_t c0 = 0; c0 < count; c0 += 65535) {
+        const int64_t nb = std::min<int64_t>(65535, count - c0);
+        if (combo == IDX_32_32)
+          hipLaunchKernelGGL((row_stats_kernel<int32_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                             0, s, dd + c0, dstat + c0);
+        else
+          hipLaunchKernelGGL((row_stats_kernel<int64_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                             0, s, dd + c0, dstat + c0);
+      }
+      rc = launch_check("row_stats_kernel");
+    }
+    if (rc == FGX_OK && (hipMemcpyAsync(maxrow.data(), dstat, count * sizeof(int64_t),
+                                        hipMemcpyDeviceToHost, s) != hipSuccess ||
+                         hipStreamSynchronize(s) != hipSuccess))
+      rc = set_error(FGX_EHIP, "fgx_batch_plan_create: statistics copy failed");
+    if (dd) (void)hipFree(dd);
+    if (dstat) (void)hipFree(dstat);
+    if (rc != FGX_OK) {
+      delete p;
+      return rc;
+    }
+  }
+  // Work units.
+  const int vbytes = elem_size(p->val_type);
+  const int ibytes = elem_size(items && count ? items[0].A.idx_type : FGX_I32);
+  std::vector<BatchWork> work;
+  int64_t carry_tiles = 0, carry_vals = 0;
+  for (int64_t i = 0; i < count; i++) {
+    BatchDesc& d = desc[static_cast<size_t>(i)];
+    p->total_nnz += d.nnz;
+    p->alg_bytes += d.nnz * (vbytes + ibytes) + (d.m + 1) * elem_size(items[i].A.ptr_type) +
+                    d.n * d.k * 4 + d.m * d.k * 4;
+    if (d.m == 0) continue;
+    const TunePick pk = autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags);
+    if (pk.kind == BK_TILES) {
+      const int64_t W = tile_width_for_k(d.k);
+      const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
+      const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
+      d.carry_tile0 = carry_tiles;
+      d.carry_val0 = carry_vals;
+      carry_tiles += nt;
+      carry_vals += nt * width;
+      for (int64_t t = 0; t < nt; t += pk.unit)
+        work.push_back(BatchWork{static_cast<int32_t>(i), BK_TILES, t, std::min(nt, t + pk.unit)});
+      p->n_tiles++;
+    } else {
+      for (int64_t r = 0; r < d.m; r += pk.unit)
+        work.push_back(BatchWork{static_cast<int32_t>(i), BK_ROWS, r, std::min(d.m, r + pk.unit)});
+      p->n_rows++;
+    }
+  }
+  int rc = batch_upload(p, desc.data(), count, work.data(), static_cast<int64_t>(work.size()),
+                        carry_tiles, carry_vals, ctx->num_cus, s);
+  if (rc != FGX_OK) {
+    free_plan_buffers(p);
+    delete p;
+    return rc;
+  }
+  if (!(flags & FGX_BATCH_NO_GRAPH)) {
+    // Capture the persistent launch (and its fix-up) once; replays reuse it.
+    rc = FGX_OK;
+    if (hipStreamCreateWithFlags(&p->capture_stream, hipStreamNonBlocking) != hipSuccess)
+      rc = set_error(FGX_EHIP, "batch: capture stream");
+    if (rc == FGX_OK && hipStreamBeginCapture(p->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      rc = set_error(FGX_EHIP, "batch: begin capture");
+    if (rc == FGX_OK) {
+      int lrc = launch_raw(p, p->capture_stream);
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(p->capture_stream, &g) != hipSuccess || lrc != FGX_OK)
+        rc = set_error(FGX_EHIP, "batch: end capture");
+      p->graph = g;
+    }
+    if (rc == FGX_OK && hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = set_error(FGX_EHIP, "batch: graph instantiate");
+    if (rc != FGX_OK) {
+      free_plan_buffers(p);
+      delete p;
+      return rc;
+    }
+    p->use_graph = true;
+  }
+  *out = p;
+  return FGX_OK;
+}
+
+int fgx_batch_plan_run(fgx_ctx* ctx, fgx_batch_plan* plan, void* stream) {
+  clear_error();
+  if (!ctx || !plan) return set_error(FGX_EINVAL, "fgx_batch_plan_run: NULL argument");
+  FGX_HIP(hipSetDevice(ctx->device));
+  return batch_launch(plan, pick_stream(ctx, stream));
+}
+
+int fgx_batch_plan_stats(const fgx_batch_plan* plan, int64_t* st) {
+  if (!plan || !st) return set_error(FGX_EINVAL, "fgx_batch_plan_stats: NULL argument");
+  st[0] = plan->n_work;
+  st[1] = plan->n_rows;
+  st[2] = plan->n_tiles;
+  st[3] = plan->grid;
+  st[4] = plan->total_nnz;
+  st[5] = plan->alg_bytes;
+  return FGX_OK;
+}
+
+int fgx_batch_plan_destroy(fgx_batch_plan* plan) {
+  if (!plan) return FGX_OK;
+  (void)hipSetDevice(plan->device);
+  (void)hipDeviceSynchronize();
+  free_plan_buffers(plan);
+  delete plan;
+  return FGX_OK;
+}
+
+int fgx_batch_run_loop(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, void* stream) {
+  clear_error();
+  if (!ctx || (count > 0 && !items)) return set_error(FGX_EINVAL, "fgx_batch_run_loop: NULL argument");
+  for (int64_t i = 0; i < count; i++) {
+    const fgx_batch_item& it = items[i];
+    int rc = it.n_rhs == 1
+                 ? fgx_spmv_csr(ctx, &it.A, it.B, it.C, stream)
+                 : fgx_spmm_csr(ctx, &it.A, it.n_rhs, it.B, it.n_rhs, FGX_ROW_MAJOR, it.C, it.n_rhs,
+                                FGX_ROW_MAJOR, stream);
+    if (rc != FGX_OK) return rc;
+  }
+  return FGX_OK;
+}
+
+}  // extern "C"

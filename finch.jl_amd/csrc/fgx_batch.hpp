@@ -1,0 +1,97 @@
+// fgx_batch.hpp — the batched persistent form: one launch runs the SpMV,
+// SpMM and SDDMM work of many independent operands from a work queue.
+//
+// The host cuts every item into work units of similar cost (bytes moved),
+// in one of three kinds:
+//   ROWS  — rows [begin, end) of a SpMM/SpMV item, one lane group per row
+//           (row-split: cheapest when row lengths are even);
+//   TILES — nnz tiles [begin, end) of a SpMM/SpMV item, merge-path tiles of
+//           fgx_csr_tiles.hpp with carries finished by the fix-up launch
+//           (for skewed rows);
+//   SDDMM — 64-position sub-tiles [begin, end) of an SDDMM item.
+// A grid sized to fill the device walks the queue in a fixed stride, so
+// the launch (and its hipGraph) is the same on every replay and results do
+// not depend on timing.
+#pragma once
+#include "fgx_csr_tiles.hpp"
+#include "fgx_internal.hpp"
+
+namespace fgx {
+
+enum BatchKind : int32_t { BK_ROWS = 0, BK_TILES = 1, BK_SDDMM = 2 };
+enum BatchOp : int32_t { BO_SPMM = 0, BO_SDDMM = 2 };
+
+struct BatchDesc {
+  const void* ptr;
+  const void* idx;
+  const void* val;
+  const float* B;   // SpMM/SpMV: dense input (n x k); SDDMM: X (m x k)
+  const float* B2;  // SDDMM: Y (n x k)
+  float* C;         // SpMM/SpMV: output (m x k); SDDMM: output values (nnz)
+  int64_t m, n, nnz, k, ldb, ldc;
+  int64_t carry_tile0;  // first carry slot of this item (TILES items)
+  int64_t carry_val0;   // first carry value of this item
+  int32_t base;
+  int32_t op;
+  int32_t vec;     // 16-byte aligned ptr/idx/val runs (SpMV tiles)
+  int32_t mfma;    // SDDMM on matrix cores (k % 4 == 0, aligned rows)
+};
+
+struct BatchWork {
+  int32_t item;
+  int32_t kind;
+  int64_t begin;
+  int64_t end;
+};
+
+constexpr int kBatchThreads = 256;
+constexpr int kSpmvRowGroup = 8;   // lanes per row, row-split SpMV
+constexpr int kTileE = 64;         // positions per lane group, TILES kind
+
+// Lanes per row (row-split) or per position (TILES) for a dense width k:
+// 16-byte column slices, rounded up to a power of two.
+__host__ __device__ inline int group_for_k(int64_t k) {
+  const int64_t lanes = (k + 3) / 4;
+  int g = 2;
+  while (g < lanes && g < 64) g <<= 1;
+  return g;
+}
+
+// Positions per TILES tile for a width k.
+__host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
+  if (k == 1) return kWave * 8;
+  return static_cast<int64_t>(kWave / group_for_k(k)) * kTileE;
+}
+
+}  // namespace fgx
+
+// Host side of the plan (fgx_batch.hip).
+struct fgx_batch_plan {
+  int device = 0;
+  int val_type = FGX_F32;
+  int combo = 0;
+  int64_t count = 0;
+  int64_t grid = 0;
+  int64_t n_work = 0, n_rows = 0, n_tiles = 0, n_sddmm = 0;
+  int64_t total_nnz = 0;
+  int64_t alg_bytes = 0;
+  fgx::BatchDesc* d_desc = nullptr;
+  fgx::BatchWork* d_work = nullptr;
+  fgx::BatchWork* d_fix = nullptr;  // TILES work units, for the fix-up launch
+  int64_t n_fix = 0;
+  int64_t* d_carry_row = nullptr;
+  float* d_carry_val = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = false;
+  hipStream_t capture_stream = nullptr;
+};
+
+namespace fgx {
+// Builds (but does not launch) a plan's device tables from descriptors and
+// work units already on the host; used by batch plans and the server.
+int batch_launch(fgx_batch_plan* p, hipStream_t s);
+int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const BatchWork* work,
+                 int64_t n_work, int64_t carry_tiles, int64_t carry_vals, int num_cus,
+                 hipStream_t s);
+}  // namespace fgx

@@ -47,8 +47,12 @@ void clear_error();
 // Scratch of at least `bytes` (grows, never shrinks).  Caller holds ctx->mu.
 int scratch(fgx_ctx* ctx, size_t bytes, void** out);
 
+// The stream a call runs on: the caller's hipStream_t; NULL is the HIP null
+// (default) stream, as in rocBLAS, so calls order with the caller's other
+// default-stream work.
 inline hipStream_t pick_stream(fgx_ctx* ctx, void* s) {
-  return s ? static_cast<hipStream_t>(s) : ctx->stream;
+  (void)ctx;
+  return static_cast<hipStream_t>(s);
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }

@@ -1,0 +1,134 @@
+// fgx_scan.hpp — device-wide exclusive scan of per-fibre counts into a
+// compressed `ptr` array: ptr[0] = base, ptr[i+1] = base + sum(cnt[0..i]).
+// This is the `freeze` step of the SparseList level (the prefix sum of
+// src/tensors/levels/sparse_list_levels.jl:310-327), done in three launches:
+// per-block sums, a one-block scan of those sums, and the final per-block
+// scan that writes ptr.
+#pragma once
+#include "fgx_device.hpp"
+#include "fgx_internal.hpp"
+
+namespace fgx {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+namespace {  // kernels local to each translation unit
+
+__device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t* total) {
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
+  // Inclusive wave scan.
+  int64_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) lds[w] = x;
+  __syncthreads();
+  if (t == 0) {
+    int64_t s = 0;
+    for (int i = 0; i < static_cast<int>(blockDim.x) / 64; i++) {
+      int64_t c = lds[i];
+      lds[i] = s;
+      s += c;
+    }
+    lds[16] = s;
+  }
+  __syncthreads();
+  const int64_t r = lds[w] + x - v;
+  *total = lds[16];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_block_sums(const int64_t* __restrict__ cnt,
+                                                                int64_t n,
+                                                                int64_t* __restrict__ sums) {
+  __shared__ int64_t lds[17];
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  int64_t s = 0;
+  for (int i = 0; i < kScanItems; i++) {
+    const int64_t q = b0 + static_cast<int64_t>(i) * kScanThreads + threadIdx.x;
+    if (q < n) s += cnt[q];
+  }
+  int64_t tot;
+  block_exclusive_scan(s, lds, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kScanThreads) void scan_sums(int64_t* __restrict__ sums, int64_t nb,
+                                                          int64_t* __restrict__ grand) {
+  __shared__ int64_t lds[17];
+  int64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nb; c0 += kScanThreads) {
+    const int64_t q = c0 + threadIdx.x;
+    const int64_t v = q < nb ? sums[q] : 0;
+    int64_t tot;
+    const int64_t ex = block_exclusive_scan(v, lds, &tot);
+    if (q < nb) sums[q] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *grand = carry;
+}
+
+}  // namespace
+
+template <typename TP>
+__global__ __launch_bounds__(kScanThreads) void scan_write_ptr(const int64_t* __restrict__ cnt,
+                                                               int64_t n,
+                                                               const int64_t* __restrict__ sums,
+                                                               int64_t base, TP* __restrict__ ptr) {
+  __shared__ int64_t lds[17];
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  // Each thread owns kScanItems consecutive entries.
+  const int64_t q0 = b0 + static_cast<int64_t>(threadIdx.x) * kScanItems;
+  int64_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    v[i] = (q0 + i < n) ? cnt[q0 + i] : 0;
+    s += v[i];
+  }
+  int64_t tot;
+  int64_t run = block_exclusive_scan(s, lds, &tot) + sums[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) {
+    run += v[i];
+    if (q0 + i < n) ptr[q0 + i + 1] = static_cast<TP>(base + run);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ptr[0] = static_cast<TP>(base);
+}
+
+// Writes ptr (n+1 entries) from cnt (n entries) and returns the total in
+// *total_host (synchronises `s`).  `work` needs scan_work_bytes(n) bytes.
+inline size_t scan_work_bytes(int64_t n) {
+  const int64_t nb = (n + kScanTile - 1) / kScanTile + 1;
+  return static_cast<size_t>(nb + 1) * sizeof(int64_t);
+}
+
+template <typename TP>
+int scan_counts_to_ptr(const int64_t* cnt, int64_t n, int64_t base, TP* ptr, void* work,
+                       int64_t* pinned, int64_t* total_host, hipStream_t s) {
+  const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
+  int64_t* sums = static_cast<int64_t*>(work);
+  int64_t* grand = sums + nb;
+  hipLaunchKernelGGL(scan_block_sums, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s,
+                     cnt, n, sums);
+  FGX_TRY(launch_check("scan_block_sums"));
+  hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums, nb, grand);
+  FGX_TRY(launch_check("scan_sums"));
+  hipLaunchKernelGGL((scan_write_ptr<TP>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0,
+                     s, cnt, n, sums, base, ptr);
+  FGX_TRY(launch_check("scan_write_ptr"));
+  if (total_host) {
+    FGX_HIP(hipMemcpyAsync(pinned, grand, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    FGX_HIP(hipStreamSynchronize(s));
+    *total_host = *pinned;
+  }
+  return FGX_OK;
+}
+
+}  // namespace fgx

@@ -1,0 +1,514 @@
+// fgx_spgemm.hip — Gustavson SpGEMM C = A*B over CSR operands, row by row,
+// as a symbolic/numeric pair (docs/examples/spgemm.jl:59-76; generated code
+// test/reference64/kernels/gustavsons.jl).
+//
+// Each output row C_i = sum over stored k of A_i of A[i,k] * B_k.  The
+// reference accumulates into a SparseByteMap workspace in k order and sorts
+// the touched coordinates at the end.  Here a team of lanes (a wave for rows
+// with at most 256 products, a 256-thread block up to 1536) instead
+//   1. gathers the row's products into LDS, list by list (list t is B_k of the
+//      t-th stored k, already sorted by column);
+//   2. merges the lists pairwise (log2 of their count levels); every product
+//      finds its place in the merged list by a binary search in the partner
+//      list, and ties keep the left list first, so equal columns stay in k
+//      order;
+//   3. folds each run of equal columns in that order, from the semiring's
+//      identity: w = a*b + w (plus-times) or w = min(w, a+b) (min-plus) —
+//      the reference's update sequence, so even FP32/FP64 plus-times sums
+//      round exactly as the oracle's.
+// Rows with more products or lists take a windowed dense accumulator in LDS
+// (exact for Int32 and min-plus; FP plus-times sums there use LDS atomics).
+#include "fgx_scan.hpp"
+
+namespace fgx {
+
+constexpr int kWaveCap = 256;    // products (and lists) per wave-team row
+constexpr int kBlockCap = 1536;  // products (and lists) per block-team row
+constexpr int kWindow = 4096;    // columns per pass of the dense fallback
+
+template <int NT>
+__device__ __forceinline__ void team_sync() {
+  if constexpr (NT == 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// Exclusive scan over a team; `tmp` holds NT/64 + 1 ints of LDS.
+template <int NT>
+__device__ __forceinline__ int team_scan(int v, int tt, int* tmp, int* total) {
+  const int lane = tt & 63;
+  int x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if constexpr (NT == 64) {
+    *total = __shfl(x, 63, 64);
+    return x - v;
+  } else {
+    const int w = tt >> 6;
+    if (lane == 63) tmp[w] = x;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+      const int c = tmp[i];
+      off += i < w ? c : 0;
+      tot += c;
+    }
+    __syncthreads();
+    *total = tot;
+    return off + x - v;
+  }
+}
+
+template <typename TV, int SR>
+__device__ __forceinline__ TV sr_mul(TV a, TV b) {
+  if constexpr (SR == FGX_MIN_PLUS) return a + b;
+  else if constexpr (sizeof(TV) == 4 && TV(0.5) == TV(0))  // Int32: wrap like Julia
+    return static_cast<TV>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b));
+  else return a * b;
+}
+template <typename TV, int SR>
+__device__ __forceinline__ TV sr_add(TV acc, TV p) {
+  if constexpr (SR == FGX_MIN_PLUS) return p < acc ? p : acc;
+  else if constexpr (sizeof(TV) == 4 && TV(0.5) == TV(0))
+    return static_cast<TV>(static_cast<uint32_t>(p) + static_cast<uint32_t>(acc));
+  else return p + acc;
+}
+template <typename TV, int SR>
+__device__ __forceinline__ TV sr_zero() {
+  if constexpr (SR == FGX_MIN_PLUS) return static_cast<TV>(__builtin_huge_val());
+  else return TV(0);
+}
+
+// Last position p in [0, n) with off[p] <= x (the list holding product x).
+__device__ __forceinline__ int list_of(const int* off, int n, int x) {
+  int lo = 0, hi = n;  // off[0] == 0 <= x
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Count of keys[lo, hi) < key (strict) or <= key.
+template <bool LE>
+__device__ __forceinline__ int count_below(const int* keys, int lo, int hi, int key) {
+  int a = lo, b = hi;
+  while (a < b) {
+    const int mid = (a + b) >> 1;
+    const bool go = LE ? keys[mid] <= key : keys[mid] < key;
+    if (go) a = mid + 1; else b = mid;
+  }
+  return a - lo;
+}
+
+// Per-team LDS workspace for rows of up to CAP products and CAP lists.
+template <int CAP, typename TV>
+struct TeamLds {
+  int keys[2][CAP];
+  TV vals[2][CAP];
+  int off[2][CAP + 1];
+  int lk[CAP];      // B row of list t
+  TV la[CAP];       // A value of list t
+  int tmp[8];
+};
+
+// Processes one row with a team of NT lanes.  NUMERIC = false: writes the
+// row's distinct-column count to cnt[row].  NUMERIC = true: writes the row's
+// sorted columns and folded values at cptr[row].
+template <int NT, int CAP, bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+__device__ void team_row(int64_t row, int tt, int64_t base, const TP* __restrict__ aptr,
+                         const TI* __restrict__ aidx, const TV* __restrict__ aval,
+                         const TP* __restrict__ bptr, const TI* __restrict__ bidx,
+                         const TV* __restrict__ bval, int64_t* __restrict__ cnt,
+                         const TP* __restrict__ cptr, TI* __restrict__ cidx,
+                         TV* __restrict__ cval, TeamLds<CAP, TV>& L) {
+  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+  const int nl = static_cast<int>(a1 - a0);
+  // 1. List offsets (prefix sum of the B row lengths).
+  int run = 0;
+  for (int c = 0; c < nl; c += NT) {
+    const int t = c + tt;
+    int len = 0, k = 0;
+    if (t < nl) {
+      k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
+      len = static_cast<int>(pos_at(bptr, base, k + 1) - pos_at(bptr, base, k));
+      L.lk[t] = k;
+      if constexpr (NUMERIC) L.la[t] = aval[a0 + t];
+    }
+    int tot;
+    const int ex = team_scan<NT>(len, tt, L.tmp, &tot);
+    if (t < nl) L.off[0][t] = run + ex;
+    run += tot;
+  }
+  const int np = run;
+  if (tt == 0) L.off[0][nl] = np;
+  team_sync<NT>();
+  // 2. Gather the products.
+  for (int p = tt; p < np; p += NT) {
+    const int t = list_of(L.off[0], nl, p);
+    const int k = L.lk[t];
+    const int64_t q = pos_at(bptr, base, k) + (p - L.off[0][t]);
+    L.keys[0][p] = static_cast<int>(static_cast<int64_t>(bidx[q]) - base);
+    if constexpr (NUMERIC) L.vals[0][p] = sr_mul<TV, SR>(L.la[t], bval[q]);
+  }
+  team_sync<NT>();
+  // 3. Pairwise merges, stable (left list first on ties).
+  int cur = 0, nlist = nl;
+  while (nlist > 1) {
+    const int* off = L.off[cur];
+    for (int p = tt; p < np; p += NT) {
+      const int t = list_of(off, nlist, p);
+      const int key = L.keys[cur][p];
+      int np_ = p;
+      if ((t ^ 1) < nlist) {
+        if ((t & 1) == 0)
+          np_ = p + count_below<false>(L.keys[cur], off[t + 1], off[t + 2], key);
+        else
+          np_ = off[t - 1] + (p - off[t]) +
+                count_below<true>(L.keys[cur], off[t - 1], off[t], key);
+      }
+      L.keys[cur ^ 1][np_] = key;
+      if constexpr (NUMERIC) L.vals[cur ^ 1][np_] = L.vals[cur][p];
+    }
+    const int nn = (nlist + 1) >> 1;
+    for (int s = tt; s <= nn; s += NT) L.off[cur ^ 1][s] = s < nn ? off[2 * s] : np;
+    team_sync<NT>();
+    cur ^= 1;
+    nlist = nn;
+  }
+  // 4. Runs of equal columns.
+  int seen = 0;
+  for (int c = 0; c < np; c += NT) {
+    const int p = c + tt;
+    const bool head = p < np && (p == 0 || L.keys[cur][p] != L.keys[cur][p - 1]);
+    int tot;
+    const int ex = team_scan<NT>(head ? 1 : 0, tt, L.tmp, &tot);
+    if constexpr (NUMERIC) {
+      if (head) {
+        const int key = L.keys[cur][p];
+        TV w = sr_zero<TV, SR>();
+        for (int r = p; r < np && L.keys[cur][r] == key; r++) w = sr_add<TV, SR>(w, L.vals[cur][r]);
+        const int64_t o = pos_at(cptr, base, row) + seen + ex;
+        cidx[o] = static_cast<TI>(key + base);
+        cval[o] = w;
+      }
+    }
+    seen += tot;
+  }
+  if constexpr (!NUMERIC) {
+    if (tt == 0) cnt[row] = seen;
+  }
+  team_sync<NT>();
+}
+
+// Wave teams: 4 rows per 256-thread block.
+template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(256) void spgemm_wave_kernel(
+    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
+    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
+    const TI* __restrict__ bidx, const TV* __restrict__ bval, int64_t* __restrict__ cnt,
+    const TP* __restrict__ cptr, TI* __restrict__ cidx, TV* __restrict__ cval) {
+  __shared__ TeamLds<kWaveCap, TV> lds[4];
+  const int w = threadIdx.x >> 6;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (r >= nrows) return;
+  team_row<64, kWaveCap, NUMERIC, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr,
+                                      bidx, bval, cnt, cptr, cidx, cval, lds[w]);
+}
+
+template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(256) void spgemm_block_kernel(
+    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
+    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
+    const TI* __restrict__ bidx, const TV* __restrict__ bval, int64_t* __restrict__ cnt,
+    const TP* __restrict__ cptr, TI* __restrict__ cidx, TV* __restrict__ cval) {
+  extern __shared__ __align__(16) unsigned char dyn[];
+  TeamLds<kBlockCap, TV>& L = *reinterpret_cast<TeamLds<kBlockCap, TV>*>(dyn);
+  const int64_t r = blockIdx.x;
+  if (r >= nrows) return;
+  team_row<256, kBlockCap, NUMERIC, SR>(rows[r], threadIdx.x, base, aptr, aidx, aval, bptr, bidx,
+                                        bval, cnt, cptr, cidx, cval, L);
+}
+
+// ---- dense-window fallback ---------------------------------------------------------
+template <typename TV, int SR>
+__device__ __forceinline__ void lds_accumulate(TV* p, TV v) {
+  if constexpr (SR == FGX_MIN_PLUS) {
+    if constexpr (sizeof(TV) == 8) {
+      unsigned long long* a = reinterpret_cast<unsigned long long*>(p);
+      unsigned long long old = *a;
+      while (v < __longlong_as_double(static_cast<long long>(old))) {
+        const unsigned long long prev =
+            atomicCAS(a, old, static_cast<unsigned long long>(__double_as_longlong(v)));
+        if (prev == old) break;
+        old = prev;
+      }
+    } else {
+      unsigned int* a = reinterpret_cast<unsigned int*>(p);
+      unsigned int old = *a;
+      while (v < __uint_as_float(old)) {
+        const unsigned int prev = atomicCAS(a, old, __float_as_uint(v));
+        if (prev == old) break;
+        old = prev;
+      }
+    }
+  } else {
+    atomicAdd(p, v);
+  }
+}
+
+template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(256) void spgemm_window_kernel(
+    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, int64_t n_inner,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
+    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
+    int64_t* __restrict__ cnt, const TP* __restrict__ cptr, TI* __restrict__ cidx,
+    TV* __restrict__ cval) {
+  __shared__ TV acc[kWindow];
+  __shared__ int flag[kWindow];
+  __shared__ int tmp[8];
+  const int64_t rr = blockIdx.x;
+  if (rr >= nrows) return;
+  const int64_t row = rows[rr];
+  const int tt = threadIdx.x;
+  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+  int64_t seen = 0;
+  for (int64_t w0 = 0; w0 < n_inner; w0 += kWindow) {
+    const int64_t w1 = w0 + kWindow < n_inner ? w0 + kWindow : n_inner;
+    for (int c = tt; c < kWindow; c += 256) {
+      flag[c] = 0;
+      acc[c] = sr_zero<TV, SR>();
+    }
+    __syncthreads();
+    for (int64_t q = a0 + tt; q < a1; q += 256) {
+      const int64_t k = static_cast<int64_t>(aidx[q]) - base;
+      const TV a = NUMERIC ? aval[q] : TV(0);
+      const int64_t b0 = pos_at(bptr, base, k), b1 = pos_at(bptr, base, k + 1);
+      int64_t lo = b0, hi = b1;
+      while (lo < hi) {
+        const int64_t mid = lo + ((hi - lo) >> 1);
+        if (static_cast<int64_t>(bidx[mid]) - base < w0) lo = mid + 1; else hi = mid;
+      }
+      for (int64_t p = lo; p < b1; p++) {
+        const int64_t j = static_cast<int64_t>(bidx[p]) - base;
+        if (j >= w1) break;
+        flag[j - w0] = 1;
+        if constexpr (NUMERIC) lds_accumulate<TV, SR>(&acc[j - w0], sr_mul<TV, SR>(a, bval[p]));
+      }
+    }
+    __syncthreads();
+    // Ordered emit of the window's touched columns.
+    for (int c = 0; c < kWindow; c += 256) {
+      const int j = c + tt;
+      const int f = (w0 + j < w1) ? flag[j] : 0;
+      int tot;
+      const int ex = team_scan<256>(f, tt, tmp, &tot);
+      if constexpr (NUMERIC) {
+        if (f) {
+          const int64_t o = pos_at(cptr, base, row) + seen + ex;
+          cidx[o] = static_cast<TI>(w0 + j + base);
+          cval[o] = acc[j];
+        }
+      }
+      seen += tot;
+    }
+    __syncthreads();
+  }
+  if constexpr (!NUMERIC) {
+    if (tt == 0) cnt[row] = seen;
+  }
+}
+
+// Row bins: 0 wave team, 1 block team, 2 dense window.
+template <typename TP, typename TI>
+__global__ void spgemm_bin_kernel(int64_t m, int64_t base, const TP* __restrict__ aptr,
+                                  const TI* __restrict__ aidx, const TP* __restrict__ bptr,
+                                  int64_t* __restrict__ bin_count, int64_t* __restrict__ bin_rows) {
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= m) return;
+  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+  int64_t prod = 0;
+  for (int64_t q = a0; q < a1 && prod <= kBlockCap; q++) {
+    const int64_t k = static_cast<int64_t>(aidx[q]) - base;
+    prod += pos_at(bptr, base, k + 1) - pos_at(bptr, base, k);
+  }
+  const int64_t key = prod > a1 - a0 ? prod : a1 - a0;
+  const int bin = key <= kWaveCap ? 0 : (key <= kBlockCap ? 1 : 2);
+  const unsigned long long slot =
+      atomicAdd(reinterpret_cast<unsigned long long*>(bin_count + bin), 1ull);
+  bin_rows[bin * m + static_cast<int64_t>(slot)] = row;
+}
+
+struct Bins {
+  int64_t* rows;  // 3 * m
+  int64_t count[3];
+};
+
+template <typename TP, typename TI>
+static int make_bins(fgx_ctx* ctx, const fgx_compressed* A, const fgx_compressed* B, int64_t* dev,
+                     Bins* bins, hipStream_t s) {
+  const int64_t m = A->n_outer;
+  int64_t* bin_count = dev;
+  bins->rows = dev + 4;
+  FGX_HIP(hipMemsetAsync(bin_count, 0, 4 * sizeof(int64_t), s));
+  if (m > 0) {
+    hipLaunchKernelGGL((spgemm_bin_kernel<TP, TI>), dim3(static_cast<unsigned>((m + 255) / 256)),
+                       dim3(256), 0, s, m, static_cast<int64_t>(A->index_base),
+                       static_cast<const TP*>(A->ptr), static_cast<const TI*>(A->idx),
+                       static_cast<const TP*>(B->ptr), bin_count, bins->rows);
+    FGX_TRY(launch_check("spgemm_bin_kernel"));
+  }
+  FGX_HIP(hipMemcpyAsync(ctx->host_i64, bin_count, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  FGX_HIP(hipStreamSynchronize(s));
+  for (int b = 0; b < 3; b++) bins->count[b] = ctx->host_i64[b];
+  return FGX_OK;
+}
+
+template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+static int run_bins(const fgx_compressed* A, const fgx_compressed* B, const Bins& bins,
+                    int64_t* cnt, const void* cptr, void* cidx, void* cval, hipStream_t s) {
+  const int64_t m = A->n_outer;
+  const int64_t base = A->index_base;
+  auto aptr = static_cast<const TP*>(A->ptr);
+  auto aidx = static_cast<const TI*>(A->idx);
+  auto aval = static_cast<const TV*>(A->val);
+  auto bptr = static_cast<const TP*>(B->ptr);
+  auto bidx = static_cast<const TI*>(B->idx);
+  auto bval = static_cast<const TV*>(B->val);
+  auto cp = static_cast<const TP*>(cptr);
+  auto ci = static_cast<TI*>(cidx);
+  auto cv = static_cast<TV*>(cval);
+  if (bins.count[0] > 0) {
+    hipLaunchKernelGGL((spgemm_wave_kernel<NUMERIC, SR, TP, TI, TV>),
+                       dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
+                       bins.rows, bins.count[0], base, aptr, aidx, aval, bptr, bidx, bval, cnt, cp,
+                       ci, cv);
+    FGX_TRY(launch_check("spgemm_wave_kernel"));
+  }
+  if (bins.count[1] > 0) {
+    const size_t lds = sizeof(TeamLds<kBlockCap, TV>);
+    hipLaunchKernelGGL((spgemm_block_kernel<NUMERIC, SR, TP, TI, TV>),
+                       dim3(static_cast<unsigned>(bins.count[1])), dim3(256), lds, s,
+                       bins.rows + m, bins.count[1], base, aptr, aidx, aval, bptr, bidx, bval, cnt,
+                       cp, ci, cv);
+    FGX_TRY(launch_check("spgemm_block_kernel"));
+  }
+  if (bins.count[2] > 0) {
+    hipLaunchKernelGGL((spgemm_window_kernel<NUMERIC, SR, TP, TI, TV>),
+                       dim3(static_cast<unsigned>(bins.count[2])), dim3(256), 0, s,
+                       bins.rows + 2 * m, bins.count[2], base, B->n_inner, aptr, aidx, aval, bptr,
+                       bidx, bval, cnt, cp, ci, cv);
+    FGX_TRY(launch_check("spgemm_window_kernel"));
+  }
+  return FGX_OK;
+}
+
+static int check_gemm(int semiring, const fgx_compressed* A, const fgx_compressed* B,
+                      const char* what) {
+  FGX_TRY(check_compressed(A, what));
+  FGX_TRY(check_compressed(B, what));
+  if (A->n_inner != B->n_outer)
+    return set_error(FGX_EDIM, "%s: A has %lld columns, B has %lld rows", what,
+                     (long long)A->n_inner, (long long)B->n_outer);
+  if (A->ptr_type != B->ptr_type || A->idx_type != B->idx_type || A->val_type != B->val_type ||
+      A->index_base != B->index_base)
+    return set_error(FGX_EUNSUP, "%s: operands must share ptr/idx/value types and index base", what);
+  if (B->n_inner > INT32_MAX || A->n_inner > INT32_MAX)
+    return set_error(FGX_EUNSUP, "%s: dimensions beyond Int32 range", what);
+  if (semiring == FGX_PLUS_TIMES) {
+    if (A->val_type != FGX_I32 && A->val_type != FGX_F32 && A->val_type != FGX_F64)
+      return set_error(FGX_EUNSUP, "%s: plus-times value type %d", what, A->val_type);
+  } else if (semiring == FGX_MIN_PLUS) {
+    if (A->val_type != FGX_F32 && A->val_type != FGX_F64)
+      return set_error(FGX_EUNSUP, "%s: min-plus needs F32 or F64 values", what);
+  } else {
+    return set_error(FGX_EINVAL, "%s: bad semiring %d", what, semiring);
+  }
+  return FGX_OK;
+}
+
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" {
+
+int fgx_spgemm_symbolic(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
+                        const fgx_compressed* B, void* c_ptr, int64_t* c_nnz, void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_spgemm_symbolic: NULL ctx");
+  FGX_TRY(check_gemm(semiring, A, B, "fgx_spgemm_symbolic"));
+  if (!c_ptr || !c_nnz) return set_error(FGX_EINVAL, "fgx_spgemm_symbolic: NULL output");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  const int64_t m = A->n_outer;
+  void* scr;
+  const size_t need = static_cast<size_t>(4 + 3 * m + m + 1) * sizeof(int64_t) + scan_work_bytes(m);
+  FGX_TRY(scratch(ctx, need, &scr));
+  int64_t* binmem = static_cast<int64_t*>(scr);
+  int64_t* cnt = binmem + 4 + 3 * m;
+  void* work = cnt + m + 1;
+  int64_t total = 0;
+  FGX_DISPATCH_IDX(ic, {
+    Bins bins;
+    FGX_TRY((make_bins<TP, TI>(ctx, A, B, binmem, &bins, s)));
+    // Structure does not depend on values: count with the FP32 instantiation.
+    FGX_TRY((run_bins<false, FGX_PLUS_TIMES, TP, TI, float>(A, B, bins, cnt, nullptr, nullptr,
+                                                            nullptr, s)));
+    FGX_TRY(scan_counts_to_ptr<TP>(cnt, m, A->index_base, static_cast<TP*>(c_ptr), work,
+                                   ctx->host_i64, &total, s));
+  });
+  *c_nnz = total;
+  return FGX_OK;
+}
+
+int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
+                       const fgx_compressed* B, const void* c_ptr, void* c_idx, void* c_val,
+                       void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_spgemm_numeric: NULL ctx");
+  FGX_TRY(check_gemm(semiring, A, B, "fgx_spgemm_numeric"));
+  if (!c_ptr) return set_error(FGX_EINVAL, "fgx_spgemm_numeric: NULL c_ptr");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  const int64_t m = A->n_outer;
+  void* scr;
+  FGX_TRY(scratch(ctx, static_cast<size_t>(4 + 3 * m) * sizeof(int64_t), &scr));
+  int64_t* binmem = static_cast<int64_t*>(scr);
+#define FGX_GEMM_NUM(SR_, TV_)                                                            \
+  FGX_DISPATCH_IDX(ic, {                                                                  \
+    Bins bins;                                                                            \
+    FGX_TRY((make_bins<TP, TI>(ctx, A, B, binmem, &bins, s)));                            \
+    FGX_TRY((run_bins<true, SR_, TP, TI, TV_>(A, B, bins, nullptr, c_ptr, c_idx, c_val, s))); \
+  })
+  if (semiring == FGX_PLUS_TIMES) {
+    switch (A->val_type) {
+      case FGX_I32: FGX_GEMM_NUM(FGX_PLUS_TIMES, int32_t); break;
+      case FGX_F32: FGX_GEMM_NUM(FGX_PLUS_TIMES, float); break;
+      case FGX_F64: FGX_GEMM_NUM(FGX_PLUS_TIMES, double); break;
+    }
+  } else {
+    switch (A->val_type) {
+      case FGX_F32: FGX_GEMM_NUM(FGX_MIN_PLUS, float); break;
+      case FGX_F64: FGX_GEMM_NUM(FGX_MIN_PLUS, double); break;
+    }
+  }
+#undef FGX_GEMM_NUM
+  return FGX_OK;
+}
+
+}  // extern "C"

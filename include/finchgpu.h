@@ -26,9 +26,10 @@
  *   otherwise; `ld` is the element stride between consecutive rows (row-major)
  *   or columns (column-major).  Finch's own `Dense(Dense(Element))` is
  *   column-major; `swizzle(Bt, 2, 1)` of the transpose gives row-major.
- * - `stream` is a hipStream_t; NULL means the context's stream.  Calls are
- *   stream-ordered; entry points that must return a data-dependent size
- *   (`*_symbolic`) synchronise that stream before returning.
+ * - `stream` is a hipStream_t; NULL means the HIP null (default) stream, as in
+ *   rocBLAS.  Calls are stream-ordered; entry points that must return a
+ *   data-dependent size (`*_symbolic`) synchronise that stream before
+ *   returning.
  * - Return value: 0 (FGX_OK) or an fgx_status code; fgx_last_error() gives the
  *   message of the calling thread's last failure.
  */
@@ -106,9 +107,10 @@ int fgx_version(void);
  * a `GPU <: AbstractDevice` instance (src/architecture.jl:6-64,170-173). */
 int fgx_ctx_create(int device, fgx_ctx** out);
 int fgx_ctx_destroy(fgx_ctx* ctx);
-/* The context's default stream (a hipStream_t). */
+/* A non-blocking stream owned by the context (a hipStream_t) for callers that
+ * want one; it is not used unless passed explicitly. */
 void* fgx_ctx_stream(fgx_ctx* ctx);
-/* Wait for all work the context queued on `stream` (NULL: its own stream). */
+/* Wait for all work queued on `stream` (NULL: the null stream). */
 int fgx_sync(fgx_ctx* ctx, void* stream);
 /* Message of the calling thread's last failing call ("" if none). */
 const char* fgx_last_error(void);

@@ -1,0 +1,185 @@
+"""GPU parity: element-wise coiteration (.*, +, max.) and Gustavson SpGEMM
+(plus-times Int32/FP32/FP64, min-plus FP32) against the oracle.
+
+Output structure (ptr, idx) and integer / min-plus values must match the
+oracle bit for bit; FP plus-times values within FP64 1e-12 / FP32 1e-5.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from conftest import read_mtx
+from finchgpu import ops, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def up(a, dev, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    return (t.to(dtype) if dtype is not None else t).to(dev)
+
+
+def operand(ptr, idx, val, n_outer, n_inner, dev, itype=torch.int32, ptype=None, base=1):
+    ptype = ptype or itype
+    sh = 1 - base
+    return ops.Compressed(up(np.asarray(ptr) - sh, dev, ptype), up(np.asarray(idx) - sh, dev, itype),
+                          up(val, dev), n_outer, n_inner, base)
+
+
+def host(C: ops.Compressed):
+    return C.ptr.cpu().numpy(), C.idx.cpu().numpy(), C.val.cpu().numpy()
+
+
+# ---- coiteration ---------------------------------------------------------------------
+
+def test_merge_golden(dev, goldens):
+    g = goldens["merge_add"]
+    x, y = g["x"], g["y"]
+    X = operand(x["ptr"], x["idx"], np.array(x["val"]), 1, g["shape"], dev, torch.int64)
+    Y = operand(y["ptr"], y["idx"], np.array(y["val"]), 1, g["shape"], dev, torch.int64)
+    cptr, cidx, cval = host(ops.ewise("add", X, Y))
+    z = g["z"]["twofinger"]
+    assert cptr.tolist() == z["ptr"] and cidx.tolist() == z["idx"] and cval.tolist() == z["val"]
+
+
+def _pair(seed, m, n, la, lb, dtype):
+    rng = np.random.default_rng(seed)
+    a = synth.poisson_csr(m, n, la, rng=rng, dtype=dtype)
+    b = synth.poisson_csr(m, n, lb, rng=rng, dtype=dtype)
+    return a, b
+
+
+@pytest.mark.parametrize("op", ["mul", "add", "max"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64, np.int32])
+@pytest.mark.parametrize("shape", [(1000, 500, 3.0, 9.0), (300, 20000, 60.0, 150.0)])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_ewise_bit_exact(dev, op, dtype, shape, itype):
+    m, n, la, lb = shape
+    (ap, ai, av), (bp, bi, bv) = _pair(7, m, n, la, lb, np.float64)
+    if dtype == np.int32:
+        av = np.rint(av * 10).astype(np.int32)
+        bv = np.rint(bv * 10).astype(np.int32)
+    av, bv = av.astype(dtype), bv.astype(dtype)
+    A = operand(ap, ai, av, m, n, dev, itype)
+    B = operand(bp, bi, bv, m, n, dev, itype)
+    cptr, cidx, cval = host(ops.ewise(op, A, B))
+    rptr, ridx, rval = oracle.ewise(op, ap, ai, av, bp, bi, bv)
+    assert np.array_equal(cptr, rptr)
+    assert np.array_equal(cidx, ridx)
+    assert np.array_equal(cval, rval)      # same operations, same operands: bit-exact
+
+
+def test_ewise_edges(dev):
+    # Empty fibres on either side, one empty operand, identical patterns.
+    m, n = 50, 40
+    (ap, ai, av), _ = _pair(3, m, n, 2.0, 2.0, np.float32)
+    ep = np.ones(m + 1, np.int64)
+    E = operand(ep, np.zeros(0, np.int64), np.zeros(0, np.float32), m, n, dev)
+    A = operand(ap, ai, av, m, n, dev)
+    for op in ("mul", "add", "max"):
+        for (X, xs), (Y, ys) in [((A, (ap, ai, av)), (E, (ep, [], np.zeros(0, np.float32)))),
+                                 ((E, (ep, [], np.zeros(0, np.float32))), (A, (ap, ai, av))),
+                                 ((A, (ap, ai, av)), (A, (ap, ai, av)))]:
+            cptr, cidx, cval = host(ops.ewise(op, X, Y))
+            rptr, ridx, rval = oracle.ewise(op, *xs, *ys)
+            assert np.array_equal(cptr, rptr) and np.array_equal(cidx, ridx)
+            assert np.array_equal(cval, rval)
+
+
+def test_ewise_config3_size(dev):
+    # 200k x 200k, Poisson(8) and Poisson(24) rows, FP32 values.
+    m = n = 200_000
+    (ap, ai, av), (bp, bi, bv) = _pair(0, m, n, 8.0, 24.0, np.float32)
+    A = operand(ap, ai, av, m, n, dev)
+    B = operand(bp, bi, bv, m, n, dev)
+    for op in ("mul", "add", "max"):
+        cptr, cidx, cval = host(ops.ewise(op, A, B))
+        rptr, ridx, rval = oracle.ewise(op, ap, ai, av, bp, bi, bv)
+        assert np.array_equal(cptr, rptr) and np.array_equal(cidx, ridx)
+        assert np.array_equal(cval, rval)
+
+
+# ---- SpGEMM ------------------------------------------------------------------------------
+
+def _gemm_check(dev, semiring, ap, ai, av, m, l, bp, bi, bv, n, itype=torch.int32, exact=True,
+                tol=0.0):
+    A = operand(ap, ai, av, m, l, dev, itype)
+    B = operand(bp, bi, bv, l, n, dev, itype)
+    cptr, cidx, cval = host(ops.spgemm(semiring, A, B))
+    rptr, ridx, rval = oracle.spgemm(semiring, m, n, ap, ai, av, bp, bi, bv)
+    assert np.array_equal(cptr, rptr)
+    assert np.array_equal(cidx, ridx)
+    if exact:
+        assert np.array_equal(cval, rval)
+    else:
+        err = np.linalg.norm(cval.astype(np.float64) - rval) / max(np.linalg.norm(rval), 1e-300)
+        assert err <= tol
+
+
+@pytest.mark.parametrize("name", ["west0132.mtx", "lpi_itest6.mtx", "lp_blend.mtx", "Trec4.mtx"])
+def test_spgemm_finch_fixtures(dev, name):
+    # kernel_tests.jl:89-130: Gustavson B == A*A on Finch's Matrix Market data.
+    S = read_mtx(name)
+    T = S if S.shape[0] == S.shape[1] else S.T.tocsr()
+    if S.dtype.kind == "i":
+        S, T = S.astype(np.int32), T.astype(np.int32)
+    S.sort_indices()
+    T.sort_indices()
+    # Same k order as the reference's update sequence: FP64 sums bit-exact.
+    _gemm_check(dev, "plus_times", S.indptr + 1, S.indices + 1, S.data, S.shape[0], S.shape[1],
+                T.indptr + 1, T.indices + 1, T.data, T.shape[1], itype=torch.int64)
+
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_spgemm_config3_int32_and_min_plus(dev, itype):
+    # Config 3: 200k x 200k, Poisson(8) x Poisson(24) rows; Int32 U{1..9}
+    # plus-times and FP32 min-plus, both bit-exact.
+    m = 200_000
+    rng = np.random.default_rng(0)
+    ap, ai, av = synth.poisson_csr(m, m, 8.0, rng=rng, dtype=np.int32, values="int19")
+    bp, bi, bv = synth.poisson_csr(m, m, 24.0, rng=rng, dtype=np.int32, values="int19")
+    _gemm_check(dev, "plus_times", ap, ai, av, m, m, bp, bi, bv, m, itype)
+    af = rng.standard_normal(ai.size).astype(np.float32)
+    bf = rng.standard_normal(bi.size).astype(np.float32)
+    _gemm_check(dev, "min_plus", ap, ai, af, m, m, bp, bi, bf, m, itype)
+
+
+def _skewed(seed, dtype, values):
+    """Rows whose product counts land in every bin: wave (<=256), block
+    (<=1536) and the dense-window path (longer)."""
+    rng = np.random.default_rng(seed)
+    l, n = 3000, 20_000
+    lens_a = np.array([0, 1, 5, 20, 40, 150, 700, 2500, 3, 0])
+    m = lens_a.size
+    ap = np.concatenate([[0], np.cumsum(lens_a)]) + 1
+    ai = np.concatenate([np.sort(rng.choice(l, k, replace=False)) for k in lens_a if k]) + 1
+    bp, bi, bv = synth.poisson_csr(l, n, 6.0, rng=rng, dtype=dtype, values=values)
+    av = (rng.integers(1, 10, ai.size) if values == "int19" else rng.standard_normal(ai.size))
+    return ap, ai, av.astype(dtype), m, l, bp, bi, bv, n
+
+
+def test_spgemm_all_bins_exact(dev):
+    ap, ai, av, m, l, bp, bi, bv, n = _skewed(1, np.int32, "int19")
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n)
+    ap, ai, av, m, l, bp, bi, bv, n = _skewed(2, np.float32, "randn")
+    _gemm_check(dev, "min_plus", ap, ai, av, m, l, bp, bi, bv, n)
+    ap, ai, av, m, l, bp, bi, bv, n = _skewed(3, np.float64, "randn")
+    _gemm_check(dev, "min_plus", ap, ai, av, m, l, bp, bi, bv, n)
+
+
+def test_spgemm_fp_plus_times_all_bins(dev):
+    # Wave/block bins fold in k order (bit-exact); the dense-window bin sums
+    # with LDS atomics, so FP sums there are held to the tolerance.
+    ap, ai, av, m, l, bp, bi, bv, n = _skewed(4, np.float32, "randn")
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n, exact=False, tol=1e-5)
+    ap, ai, av, m, l, bp, bi, bv, n = _skewed(5, np.float64, "randn")
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n, exact=False, tol=1e-12)
+
+
+def test_spgemm_empty(dev):
+    ep = np.ones(6, np.int64)
+    e = np.zeros(0, np.int64)
+    _gemm_check(dev, "plus_times", ep, e, np.zeros(0, np.float32), 5, 4, np.ones(5, np.int64), e,
+                np.zeros(0, np.float32), 7)

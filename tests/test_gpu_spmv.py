@@ -40,7 +40,7 @@ def operand(ptr, idx, val, n_outer, n_inner, dev, itype=torch.int64, ptype=None,
                           n_outer, n_inner, base)
 
 
-IDX = [(torch.int32, torch.int32), (torch.int64, torch.int64), (torch.int64, torch.int32)]
+IDX = [(torch.int32, torch.int32), (torch.int64, torch.int64), (torch.int32, torch.int64)]
 
 
 # ---- goldens -------------------------------------------------------------------
