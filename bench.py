@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU time budget of the baseline leg")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first copy B (256 MB) once: a kernel of known traffic that calibrates "
+                         "rocprofv3 byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; the sparse kernels are "
@@ -77,6 +80,12 @@ def main():
     C = torch.empty((M, K), dtype=torch.float32, device=dev)
     y = torch.empty(M, dtype=torch.float32, device=dev)
     out = torch.empty(nnz, dtype=torch.float32, device=dev)
+
+    if args.copy_probe:
+        probe = torch.empty_like(B)
+        probe.copy_(B)
+        torch.cuda.synchronize()
+        del probe
 
     def run_timed(fn, steps, warmup):
         for _ in range(warmup):

@@ -1,14 +1,13 @@
 // fgx_batch.hip — batched persistent launches (fgx_batch_*) and the device
 // side shared with the serving loop.
 #include "fgx_batch.hpp"
+#include "fgx_sddmm_core.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 namespace fgx {
-
-typedef float bf32x4 __attribute__((ext_vector_type(4)));
 
 // ---- ROWS units ----------------------------------------------------------------------
 // SpMV: kSpmvRowGroup lanes per row, strided over the row, then a shuffle tree.
@@ -75,21 +74,25 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
 }
 
 // ---- SDDMM units: 64 positions per sub-tile, one per lane ----------------------------------
+// `lds` holds the wave's fibre window (kSdWin entries) followed by the X and
+// Y staging tiles of the matrix-core path.
 template <typename TP, typename TI, typename TV>
 __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int64_t st1,
-                                           int64_t* lds) {
+                                           unsigned char* lds) {
   const TP* ptr = static_cast<const TP*>(d.ptr);
   const TI* idx = static_cast<const TI*>(d.idx);
   const TV* val = static_cast<const TV*>(d.val);
   const int64_t base = d.base;
   const int lane = lane_id();
+  int64_t* win_lds = reinterpret_cast<int64_t*>(lds);
+  float(*xs)[kSdPitch] = reinterpret_cast<float(*)[kSdPitch]>(lds + kSdWin * sizeof(int64_t));
+  float(*ys)[kSdPitch] = xs + kWave;
   for (int64_t st = st0; st < st1; st++) {
     const int64_t q0 = st * kWave;
     const int64_t qend = (q0 + kWave < d.nnz) ? q0 + kWave : d.nnz;
-    int64_t r_lo, r_hi;
-    tile_fibers(ptr, base, d.m, q0, qend, &r_lo, &r_hi);
-    const int64_t r_last = r_hi < d.m ? r_hi : d.m - 1;
-    FiberWindow<TP> win = load_window(ptr, base, r_lo, r_last - r_lo + 1, lds, kWinCap);
+    int64_t r_lo, r_hi, pos_lo;
+    FiberWindow<TP> win = tile_setup(ptr, base, d.m, q0, qend, d.table, st, win_lds, &r_lo, &r_hi,
+                                     &pos_lo, kSdWin);
     const int64_t q = q0 + lane;
     const bool in = q < qend;
     int64_t i = 0, j = 0;
@@ -99,25 +102,14 @@ __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int6
       a = widen<float>(ld_stream(val + q));
       i = win.find(q);
     }
-    const float* xr = d.B + i * d.ldb;
-    const float* yr = d.B2 + j * d.ldc;
     float dot = 0.f;
     if (d.mfma) {
-      bf32x4 c = {0.f, 0.f, 0.f, 0.f};
-      for (int64_t kk = 0; kk < d.k; kk += 4) {
-        float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), yb = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (in) {
-          xa = *reinterpret_cast<const float4*>(xr + kk);
-          yb = *reinterpret_cast<const float4*>(yr + kk);
-        }
-        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.x, yb.x, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.y, yb.y, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.z, yb.z, c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.w, yb.w, c, 0, 0, 0);
-      }
-      const int dd = lane & 3;
-      dot = dd == 0 ? c[0] : (dd == 1 ? c[1] : (dd == 2 ? c[2] : c[3]));
+      f32x4 c = {0.f, 0.f, 0.f, 0.f};
+      sddmm_sub_mfma(d.B, d.ldb, d.B2, d.ldc, d.k, in, i, j, xs, ys, c);
+      dot = mfma_diag(c);
     } else if (in) {
+      const float* xr = d.B + i * d.ldb;
+      const float* yr = d.B2 + j * d.ldc;
       for (int64_t kk = 0; kk < d.k; kk++) dot += xr[kk] * yr[kk];
     }
     if (in) d.C[q] = a * dot;
@@ -132,7 +124,8 @@ __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64
   for (int64_t t = t0; t < t1; t++)
     spmm_tile<TP, TI, TV, float, G, 4, kTileE, U>(
         static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
-        static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t, lds,
+        static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t,
+        d.table, lds,
         carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4));
 }
 
@@ -142,7 +135,7 @@ __device__ __forceinline__ void tiles_spmv(const BatchDesc& d, int64_t t0, int64
   for (int64_t t = t0; t < t1; t++)
     spmv_tile<TP, TI, TV, float, 8>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
                                     static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.B, d.C,
-                                    t, d.vec != 0, lds, carry_row + d.carry_tile0 + t,
+                                    t, d.vec != 0, d.table, lds, carry_row + d.carry_tile0 + t,
                                     carry_val + d.carry_val0 + t);
 }
 
@@ -157,11 +150,20 @@ __device__ __forceinline__ int64_t xcd_block() {
   return (b % kXcd) * (nb / kXcd) + b / kXcd;
 }
 
-template <typename TP, typename TI, typename TV>
+// Per-wave LDS of the batch kernel: the fibre window of ROWS/TILES units, or
+// (with SDDMM units) the SDDMM window plus X/Y staging tiles.
+constexpr int kLdsPlain = kWinCap * static_cast<int>(sizeof(int64_t));
+constexpr int kLdsSddmm = kSdWin * static_cast<int>(sizeof(int64_t)) + 2 * kWave * kSdPitch * 4;
+
+template <typename TP, typename TI, typename TV, bool SDDMM>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t n_work,
     int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
-  __shared__ int64_t lds[kWavesPerBlock][kWinCap];
+  constexpr int kWaveLds = SDDMM ? (kLdsSddmm > kLdsPlain ? kLdsSddmm : kLdsPlain) : kLdsPlain;
+  __shared__ __align__(16) unsigned char smem[kWavesPerBlock][kWaveLds];
+  int64_t* lds[kWavesPerBlock];
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; i++) lds[i] = reinterpret_cast<int64_t*>(smem[i]);
   const int w = threadIdx.x / kWave;
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = xcd_block() * kWavesPerBlock + w; u < n_work; u += nw) {
@@ -193,18 +195,17 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
           default: tiles_spmm<64, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
         }
       }
-    } else {
-      sddmm_unit<TP, TI, TV>(d, wk.begin, wk.end, lds[w]);
+    } else if constexpr (SDDMM) {
+      sddmm_unit<TP, TI, TV>(d, wk.begin, wk.end, smem[w]);
     }
   }
 }
 
-// Carries of TILES units: one wave per unit, lanes over (tile, column).
+// Carries of TILES units: one wave per unit, over the unit's tiles in order.
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
     const int64_t* __restrict__ carry_row, const float* __restrict__ carry_val) {
   const int w = threadIdx.x / kWave;
-  const int lane = lane_id();
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; u < n_fix; u += nw) {
     const BatchWork wk = fix[u];
@@ -213,13 +214,9 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const int64_t W = tile_width_for_k(d.k);
     const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
     const int64_t ld = d.k == 1 ? 1 : d.ldc;
-    const int64_t cells = (wk.end - wk.begin) * width;
-    for (int64_t c = lane; c < cells; c += kWave) {
-      const int64_t t = wk.begin + c / width;
-      const int col = static_cast<int>(c % width);
-      fixup_run<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width, col,
-                       d.C, ld, 0, d.k);
-    }
+    for (int64_t t = wk.begin; t < wk.end; t++)
+      fixup_tile<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width,
+                        d.C, ld, 0, d.k);
   }
 }
 
@@ -252,9 +249,14 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const BatchDesc* __restr
 // ---- host ----------------------------------------------------------------------------------
 template <typename TP, typename TI, typename TV>
 static void launch_batch_typed(const fgx_batch_plan* p, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<TP, TI, TV>), dim3(static_cast<unsigned>(p->grid)),
-                     dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, p->n_work, p->d_carry_row,
-                     p->d_carry_val);
+  if (p->has_sddmm)
+    hipLaunchKernelGGL((batch_kernel<TP, TI, TV, true>), dim3(static_cast<unsigned>(p->grid)),
+                       dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, p->n_work,
+                       p->d_carry_row, p->d_carry_val);
+  else
+    hipLaunchKernelGGL((batch_kernel<TP, TI, TV, false>), dim3(static_cast<unsigned>(p->grid)),
+                       dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, p->n_work,
+                       p->d_carry_row, p->d_carry_val);
 }
 
 static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
@@ -340,11 +342,44 @@ int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const
   return FGX_OK;
 }
 
-// The autotuner's choice for one item.
-struct TunePick {
-  int kind;          // BK_ROWS or BK_TILES
-  int64_t unit;      // rows (ROWS) or tiles (TILES) per work unit
-};
+template <typename T>
+static int grow_device(T** buf, int64_t* cap, int64_t need) {
+  if (need <= *cap && *buf) return FGX_OK;
+  int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * *cap, 64));
+  if (*buf) (void)hipFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  if (hipMalloc(reinterpret_cast<void**>(buf), static_cast<size_t>(want) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(FGX_EOOM, "batch: device allocation failed");
+  }
+  *cap = want;
+  return FGX_OK;
+}
+
+int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
+                        const BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
+                        int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
+                        hipStream_t s) {
+  FGX_TRY(grow_device(&p->d_desc, &p->cap_desc, nd));
+  FGX_TRY(grow_device(&p->d_work, &p->cap_work, nw));
+  FGX_TRY(grow_device(&p->d_fix, &p->cap_fix, nf));
+  FGX_TRY(grow_device(&p->d_carry_row, &p->cap_crow, carry_tiles));
+  FGX_TRY(grow_device(&p->d_carry_val, &p->cap_cval, carry_vals));
+  if (nd > 0) FGX_HIP(hipMemcpyAsync(p->d_desc, h_desc, nd * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+  if (nw > 0) FGX_HIP(hipMemcpyAsync(p->d_work, h_work, nw * sizeof(BatchWork), hipMemcpyHostToDevice, s));
+  if (nf > 0) FGX_HIP(hipMemcpyAsync(p->d_fix, h_fix, nf * sizeof(BatchWork), hipMemcpyHostToDevice, s));
+  p->n_work = nw;
+  p->n_fix = nf;
+  p->has_sddmm = has_sddmm;
+  const int64_t want = (nw + kWavesPerBlock - 1) / kWavesPerBlock;
+  int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(num_cus) * 8);
+  if (grid >= 8) grid -= grid % 8;
+  p->grid = std::max<int64_t>(grid, 1);
+  return launch_raw(p, s);
+}
+
+void batch_free(fgx_batch_plan* p) { free_plan_buffers(p); }
 
 // Picks row-split vs nnz-split and the work-unit size from the item's shape
 // and row-length statistics.  Row-split keeps one lane group per row with no
@@ -352,7 +387,43 @@ struct TunePick {
 // the mean (power-law rows), one row would serialise a group for too long
 // and the nnz-balanced tiles take over.  Units aim at ~256 KB of traffic so
 // that thousands of them spread over the device.
-static TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
+int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out,
+                    hipStream_t s) {
+  if (n <= 0) return FGX_OK;
+  std::vector<BatchDesc> desc(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; i++) {
+    BatchDesc& d = desc[static_cast<size_t>(i)];
+    d = BatchDesc{};
+    d.ptr = ops[i].ptr;
+    d.m = ops[i].n_outer;
+    d.base = ops[i].index_base;
+  }
+  BatchDesc* dd = nullptr;
+  int64_t* dstat = nullptr;
+  int rc = dev_alloc_copy(&dd, desc.data(), n, s);
+  if (rc == FGX_OK) rc = dev_alloc_copy<int64_t>(&dstat, nullptr, n, s);
+  if (rc == FGX_OK) {
+    for (int64_t c0 = 0; c0 < n; c0 += 65535) {
+      const int64_t nb = std::min<int64_t>(65535, n - c0);
+      if (combo == IDX_32_32)
+        hipLaunchKernelGGL((row_stats_kernel<int32_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                           0, s, dd + c0, dstat + c0);
+      else
+        hipLaunchKernelGGL((row_stats_kernel<int64_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
+                           0, s, dd + c0, dstat + c0);
+    }
+    rc = launch_check("row_stats_kernel");
+  }
+  if (rc == FGX_OK && (hipMemcpyAsync(out, dstat, n * sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+                           hipSuccess ||
+                       hipStreamSynchronize(s) != hipSuccess))
+    rc = set_error(FGX_EHIP, "row statistics copy failed");
+  if (dd) (void)hipFree(dd);
+  if (dstat) (void)hipFree(dstat);
+  return rc;
+}
+
+TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
                               int flags) {
   const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
   bool tiles = max_row > 256 && static_cast<double>(max_row) > 16.0 * std::max(mean, 1.0);
@@ -438,33 +509,15 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     d.op = BO_SPMM;
     d.vec = aligned16(it.A.val) && aligned16(it.A.idx);
     d.mfma = 0;
+    d.table = nullptr;
   }
   p->combo = combo;
   // Row-length statistics (one block per item).
   std::vector<int64_t> maxrow(static_cast<size_t>(count), 0);
   if (count > 0) {
-    BatchDesc* dd = nullptr;
-    int64_t* dstat = nullptr;
-    int rc = dev_alloc_copy(&dd, desc.data(), count, s);
-    if (rc == FGX_OK) rc = dev_alloc_copy<int64_t>(&dstat, nullptr, count, s);
-    if (rc == FGX_OK) {
-      for (int64_t c0 = 0; c0 < count; c0 += 65535) {
-        const int64_t nb = std::min<int64_t>(65535, count - c0);
-        if (combo == IDX_32_32)
-          hipLaunchKernelGGL((row_stats_kernel<int32_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
-                             0, s, dd + c0, dstat + c0);
-        else
-          hipLaunchKernelGGL((row_stats_kernel<int64_t>), dim3(static_cast<unsigned>(nb)), dim3(256),
-                             0, s, dd + c0, dstat + c0);
-      }
-      rc = launch_check("row_stats_kernel");
-    }
-    if (rc == FGX_OK && (hipMemcpyAsync(maxrow.data(), dstat, count * sizeof(int64_t),
-                                        hipMemcpyDeviceToHost, s) != hipSuccess ||
-                         hipStreamSynchronize(s) != hipSuccess))
-      rc = set_error(FGX_EHIP, "fgx_batch_plan_create: statistics copy failed");
-    if (dd) (void)hipFree(dd);
-    if (dstat) (void)hipFree(dstat);
+    std::vector<fgx_compressed> ops(static_cast<size_t>(count));
+    for (int64_t i = 0; i < count; i++) ops[static_cast<size_t>(i)] = items[i].A;
+    int rc = max_row_lengths(ops.data(), count, combo, maxrow.data(), s);
     if (rc != FGX_OK) {
       delete p;
       return rc;

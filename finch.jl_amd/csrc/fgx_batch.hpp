@@ -29,6 +29,7 @@ struct BatchDesc {
   const float* B2;  // SDDMM: Y (n x k)
   float* C;         // SpMM/SpMV: output (m x k); SDDMM: output values (nnz)
   int64_t m, n, nnz, k, ldb, ldc;
+  const int64_t* table; // tile table hint of TILES items (may be null)
   int64_t carry_tile0;  // first carry slot of this item (TILES items)
   int64_t carry_val0;   // first carry value of this item
   int32_t base;
@@ -84,10 +85,28 @@ struct fgx_batch_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool use_graph = false;
+  bool has_sddmm = false;
+  int64_t cap_desc = 0, cap_work = 0, cap_fix = 0, cap_crow = 0, cap_cval = 0;
   hipStream_t capture_stream = nullptr;
 };
 
 namespace fgx {
+// The autotuner's choice for one item: row-split (ROWS) or nnz-split
+// (TILES), and rows or tiles per work unit.
+struct TunePick {
+  int kind;
+  int64_t unit;
+};
+TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes, int flags);
+// Uploads host descriptor/work arrays into p's (grown) device buffers on
+// stream s and launches; the previous launch on p must have finished.
+int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
+                        const BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
+                        int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
+                        hipStream_t s);
+void batch_free(fgx_batch_plan* p);
+// Maximum row length of each compressed operand (one block per operand).
+int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out, hipStream_t s);
 // Builds (but does not launch) a plan's device tables from descriptors and
 // work units already on the host; used by batch plans and the server.
 int batch_launch(fgx_batch_plan* p, hipStream_t s);

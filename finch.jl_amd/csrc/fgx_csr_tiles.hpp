@@ -40,6 +40,53 @@ __device__ __forceinline__ void tile_fibers(const TP* __restrict__ ptr, int64_t 
   *r_hi = hi;
 }
 
+// Fibres of a tile from a precomputed table (table[t] = the fibre owning
+// position t*W, table[ntiles] = m), verified against ptr before use: the
+// table is only a hint, so a stale one (ptr rewritten in place) costs a
+// search, never a wrong answer.  Loads the LDS window and returns pos(r_lo).
+template <typename TP>
+__device__ __forceinline__ FiberWindow<TP> tile_setup(const TP* __restrict__ ptr, int64_t base,
+                                                      int64_t m, int64_t q0, int64_t qend,
+                                                      const int64_t* __restrict__ table,
+                                                      int64_t tile, int64_t* lds,
+                                                      int64_t* r_lo_out, int64_t* r_hi_out,
+                                                      int64_t* pos_lo_out, int cap = kWinCap) {
+  int64_t r_lo = 0, r_hi = m;
+  bool good = false;
+  FiberWindow<TP> win;
+  if (table) {
+    r_lo = tile == 0 ? 0 : table[tile];
+    r_hi = table[tile + 1];
+    good = r_lo >= 0 && r_lo <= r_hi && r_hi <= m && r_lo < m;
+    if (good) {
+      const int64_t r_last = r_hi < m ? r_hi : m - 1;
+      const int64_t p_lo = pos_at(ptr, base, r_lo);
+      win = load_window(ptr, base, r_lo, r_last - r_lo + 1, lds, cap);
+      if (tile > 0) good = good && p_lo <= q0 && win.end_of(r_lo) > q0;
+      if (r_hi < m) good = good && win.end_of(r_hi) > qend;
+      else good = good && pos_at(ptr, base, m) <= qend;
+      if (r_hi > r_lo) good = good && win.end_of(r_hi - 1) <= qend;
+      *pos_lo_out = p_lo;
+    }
+  }
+  if (!good) {
+    tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
+    const int64_t r_last = r_hi < m ? r_hi : m - 1;
+    *pos_lo_out = pos_at(ptr, base, r_lo);
+    win = load_window(ptr, base, r_lo, r_last - r_lo + 1, lds, cap);
+  }
+  *r_lo_out = r_lo;
+  *r_hi_out = r_hi;
+  return win;
+}
+
+// Carry slot of a tile: -1 (no carry); r (the tile starts a run of tiles
+// carrying into row r); -(r + 2) (the tile continues the run of the tile
+// before it, i.e. the whole tile lies inside row r, which began earlier).
+__device__ __forceinline__ int64_t encode_carry(bool carries, bool continues, int64_t r) {
+  return carries ? (continues ? -(r + 2) : r) : -1;
+}
+
 // ---------------------------------------------------------------------------
 // SpMV tile: lane-level runs of V positions.  Writes y for every row the tile
 // finalises and (carry_row, carry_val) of the row continuing into the next
@@ -49,16 +96,16 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
                                           const TV* __restrict__ val, int64_t base, int64_t m,
                                           int64_t nnz, const TA* __restrict__ x,
                                           TA* __restrict__ y, int64_t tile, bool vec,
-                                          int64_t* lds_win, int64_t* carry_row_out,
-                                          TA* carry_val_out) {
+                                          const int64_t* __restrict__ table, int64_t* lds_win,
+                                          int64_t* carry_row_out, TA* carry_val_out) {
   constexpr int W = kWave * V;
   const int lane = lane_id();
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  int64_t r_lo, r_hi;
-  tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win =
+      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
   const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
-  FiberWindow<TP> win = load_window(ptr, base, r_lo, r_win_last - r_lo + 1, lds_win, kWinCap);
 
   const int64_t a = q0 + static_cast<int64_t>(lane) * V;
   const int64_t b = (a + V < qend) ? a + V : qend;
@@ -137,7 +184,8 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
       const int64_t start_hi = (r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
       carries = start_hi < qend;
     }
-    *carry_row_out = carries ? r_hi : -1;
+    const bool continues = tile > 0 && r_lo == r_hi && pos_lo < q0;
+    *carry_row_out = encode_carry(carries, continues, r_hi);
     *carry_val_out = carries ? v : TA(0);
   }
 }
@@ -190,13 +238,37 @@ __device__ __forceinline__ void st_dense(TA* __restrict__ p, int valid_cols, con
     if (c < valid_cols) p[c] = o[c];
 }
 
+// Output rows are written once and not read again by the kernel: stream
+// them past the caches so they do not evict the gathered dense rows.
+template <int KV, typename TA>
+__device__ __forceinline__ void st_dense_nt(TA* __restrict__ p, int valid_cols, const TA (&o)[KV]) {
+  if constexpr (KV == 4 && sizeof(TA) == 4) {
+    if (valid_cols >= 4) {
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      f4v t = {o[0], o[1], o[2], o[3]};
+      __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+      return;
+    }
+  }
+  if constexpr (KV == 2 && sizeof(TA) == 8) {
+    if (valid_cols >= 2) {
+      typedef double d2v __attribute__((ext_vector_type(2)));
+      d2v t = {o[0], o[1]};
+      __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+      return;
+    }
+  }
+  st_dense<KV, TA>(p, valid_cols, o);
+}
+
 template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
 __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                           const TV* __restrict__ val, int64_t base, int64_t m,
                                           int64_t nnz, int64_t k, int64_t kofs,
                                           const TA* __restrict__ B, int64_t ldb,
                                           TA* __restrict__ C, int64_t ldc, int64_t tile,
-                                          int64_t* lds_win, int64_t* carry_row_out,
+                                          const int64_t* __restrict__ table, int64_t* lds_win,
+                                          int64_t* carry_row_out,
                                           TA* carry_val_out /* G*KV entries */) {
   constexpr int NG = kWave / G;
   constexpr int W = NG * E;
@@ -208,10 +280,10 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
   const int gbase = g * G;
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  int64_t r_lo, r_hi;
-  tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win =
+      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
   const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
-  FiberWindow<TP> win = load_window(ptr, base, r_lo, r_win_last - r_lo + 1, lds_win, kWinCap);
 
   const int64_t col0 = kofs + static_cast<int64_t>(gl) * KV;
   const int valid_cols = static_cast<int>(col0 < k ? (k - col0 < KV ? k - col0 : KV) : 0);
@@ -237,7 +309,7 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
 #pragma unroll
       for (int c = 0; c < KV; c++) first_val[c] = acc[c];
     } else if (valid_cols > 0) {
-      st_dense<KV, TA>(C + row * ldc + col0, valid_cols, acc);
+      st_dense_nt<KV, TA>(C + row * ldc + col0, valid_cols, acc);
     }
 #pragma unroll
     for (int c = 0; c < KV; c++) acc[c] = 0;
@@ -310,35 +382,75 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
     TA o[KV];
 #pragma unroll
     for (int c = 0; c < KV; c++) o[c] = first_val[c] + cin[c];
-    st_dense<KV, TA>(C + first_row * ldc + col0, valid_cols, o);
+    st_dense_nt<KV, TA>(C + first_row * ldc + col0, valid_cols, o);
   }
   bool carries = false;
   if (r_hi < m && qend > q0) {
     const int64_t start_hi = (r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
     carries = start_hi < qend;
   }
-  if (lane == kWave - 1) *carry_row_out = carries ? r_hi : -1;
+  const bool continues = tile > 0 && r_lo == r_hi && pos_lo < q0;
+  if (lane == kWave - 1) *carry_row_out = encode_carry(carries, continues, r_hi);
   if (g == NG - 1 && carries) {
 #pragma unroll
     for (int c = 0; c < KV; c++) carry_val_out[gl * KV + c] = v[c];
   }
 }
 
-// Adds the carries of runs of consecutive tiles to the row they continue:
-// one thread (SpMV) or lane group (SpMM) per tile; only the first tile of a
-// run does the work, summing in tile order.
+// Completes the row that a run of tiles carries into, if tile t starts such
+// a run (carry slot >= 0).  Called by a whole wave.  The run's length is
+// found 64 slots at a time with a ballot; then SpMV (width 1) sums the run's
+// carries with lanes over tiles and a fixed shuffle tree, and SpMM sums them
+// in tile order with lanes over columns.  Deterministic.
 template <typename TA>
-__device__ __forceinline__ void fixup_run(const int64_t* __restrict__ carry_row,
-                                          const TA* __restrict__ carry_val, int64_t ntiles,
-                                          int64_t t, int width, int col, TA* __restrict__ out_row_base,
-                                          int64_t ld, int64_t kofs, int64_t k) {
+__device__ __forceinline__ void fixup_tile(const int64_t* __restrict__ carry_row,
+                                           const TA* __restrict__ carry_val, int64_t ntiles,
+                                           int64_t t, int width, TA* __restrict__ out, int64_t ld,
+                                           int64_t kofs, int64_t k) {
+  const int lane = lane_id();
   const int64_t r = carry_row[t];
   if (r < 0) return;
-  if (t > 0 && carry_row[t - 1] == r) return;
-  if (col >= width || kofs + col >= k) return;
-  TA s = 0;
-  for (int64_t t2 = t; t2 < ntiles && carry_row[t2] == r; t2++) s += carry_val[t2 * width + col];
-  out_row_base[r * ld + kofs + col] += s;
+  const int64_t cont = -(r + 2);
+  int64_t len = 1;
+  for (int64_t c = t + 1;; c += kWave) {
+    const int64_t t2 = c + lane;
+    const bool in = t2 < ntiles && carry_row[t2] == cont;
+    const uint64_t msk = ballot(in);
+    const int run = (~msk == 0) ? kWave : __ffsll(static_cast<unsigned long long>(~msk)) - 1;
+    len += run;
+    if (run < kWave) break;
+  }
+  if (width == 1) {
+    TA s = 0;
+    for (int64_t i = lane; i < len; i += kWave) s += carry_val[t + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+    if (lane == 0 && kofs < k) out[r * ld + kofs] += s;
+  } else {
+    for (int col = lane; col < width; col += kWave) {
+      if (kofs + col >= k) continue;
+      TA s = 0;
+      for (int64_t i = 0; i < len; i++) s += carry_val[(t + i) * width + col];
+      out[r * ld + kofs + col] += s;
+    }
+  }
+}
+
+// Builds the tile table: table[t] = smallest r with pos(r+1) > t*W (m if
+// none), for t in [0, ntiles]; table[ntiles] uses position nnz.
+template <typename TP>
+__global__ void tile_table_kernel(const TP* __restrict__ ptr, int64_t base, int64_t m,
+                                  int64_t nnz, int64_t W, int64_t ntiles,
+                                  int64_t* __restrict__ table) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t > ntiles) return;
+  const int64_t q = t < ntiles ? t * W : nnz;
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (pos_at(ptr, base, mid + 1) <= q) lo = mid + 1; else hi = mid;
+  }
+  table[t] = lo;
 }
 
 }  // namespace fgx

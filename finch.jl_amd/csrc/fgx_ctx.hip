@@ -1,7 +1,7 @@
 // fgx_ctx.hip — context lifecycle, error reporting, scratch arena and the
 // value-precision conversion kernel.
 #include "fgx_internal.hpp"
-#include "fgx_device.hpp"
+#include "fgx_csr_tiles.hpp"
 
 #include <cstring>
 
@@ -40,6 +40,66 @@ int scratch(fgx_ctx* ctx, size_t bytes, void** out) {
     ctx->scratch_bytes = want;
   }
   *out = ctx->scratch;
+  return FGX_OK;
+}
+
+constexpr size_t kMaxTables = 32;
+
+void free_tile_tables(fgx_ctx* ctx) {
+  if (ctx->tables.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& t : ctx->tables) (void)hipFree(t.d_table);
+  ctx->tables.clear();
+}
+
+int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t** out) {
+  const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
+  int64_t* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), (ntiles + 1) * sizeof(int64_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(FGX_EOOM, "tile table allocation failed");
+  }
+  const unsigned blocks = static_cast<unsigned>((ntiles + 1 + 255) / 256);
+  if (A->ptr_type == FGX_I32)
+    hipLaunchKernelGGL((tile_table_kernel<int32_t>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const int32_t*>(A->ptr), static_cast<int64_t>(A->index_base),
+                       A->n_outer, A->nnz, W, ntiles, d);
+  else
+    hipLaunchKernelGGL((tile_table_kernel<int64_t>), dim3(blocks), dim3(256), 0, s,
+                       static_cast<const int64_t*>(A->ptr), static_cast<int64_t>(A->index_base),
+                       A->n_outer, A->nnz, W, ntiles, d);
+  int rc = launch_check("tile_table_kernel");
+  if (rc != FGX_OK) {
+    (void)hipFree(d);
+    return rc;
+  }
+  *out = d;
+  return FGX_OK;
+}
+
+int tile_table(fgx_ctx* ctx, const fgx_compressed* A, int64_t W, hipStream_t s,
+               const int64_t** out) {
+  for (auto& t : ctx->tables) {
+    if (t.ptr == A->ptr && t.n_outer == A->n_outer && t.nnz == A->nnz && t.W == W &&
+        t.ptr_type == A->ptr_type && t.base == A->index_base) {
+      t.stamp = ++ctx->clock;
+      *out = t.d_table;
+      return FGX_OK;
+    }
+  }
+  if (ctx->tables.size() >= kMaxTables) {
+    size_t oldest = 0;
+    for (size_t i = 1; i < ctx->tables.size(); i++)
+      if (ctx->tables[i].stamp < ctx->tables[oldest].stamp) oldest = i;
+    FGX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctx->tables[oldest].d_table);
+    ctx->tables.erase(ctx->tables.begin() + static_cast<long>(oldest));
+  }
+  fgx_tile_table t{A->ptr, A->n_outer, A->nnz, W, A->ptr_type, A->index_base, nullptr,
+                   ++ctx->clock};
+  FGX_TRY(build_tile_table(A, W, s, &t.d_table));
+  ctx->tables.push_back(t);
+  *out = t.d_table;
   return FGX_OK;
 }
 
@@ -146,6 +206,7 @@ int fgx_ctx_destroy(fgx_ctx* ctx) {
   if (!ctx) return FGX_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  free_tile_tables(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->host_i64) (void)hipHostFree(ctx->host_i64);
   (void)hipStreamDestroy(ctx->stream);
@@ -164,6 +225,9 @@ int fgx_sync(fgx_ctx* ctx, void* stream) {
 
 int fgx_plan_cache_clear(fgx_ctx* ctx) {
   if (!ctx) return set_error(FGX_EINVAL, "fgx_plan_cache_clear: NULL ctx");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  free_tile_tables(ctx);
   return FGX_OK;
 }
 

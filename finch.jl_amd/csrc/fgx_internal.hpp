@@ -8,8 +8,20 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/finchgpu.h"
+
+// Cached tile table of one operand (see tile_setup): keyed by the ptr
+// buffer, its shape and the tile width.  Tables are hints verified by every
+// tile, so a ptr rewritten in place is still handled correctly.
+struct fgx_tile_table {
+  const void* ptr;
+  int64_t n_outer, nnz, W;
+  int32_t ptr_type, base;
+  int64_t* d_table;
+  uint64_t stamp;
+};
 
 struct fgx_ctx {
   int device = 0;
@@ -21,6 +33,8 @@ struct fgx_ctx {
   size_t scratch_bytes = 0;
   // Small pinned host buffer for data-dependent sizes (symbolic nnz).
   int64_t* host_i64 = nullptr;
+  std::vector<fgx_tile_table> tables;
+  uint64_t clock = 0;
   std::mutex mu;
 };
 
@@ -56,6 +70,14 @@ inline hipStream_t pick_stream(fgx_ctx* ctx, void* s) {
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+// Tile table (ntiles+1 entries) of A for tiles of W positions, built on
+// first use and cached in the context.  Caller holds ctx->mu.
+int tile_table(fgx_ctx* ctx, const fgx_compressed* A, int64_t W, hipStream_t s,
+               const int64_t** out);
+void free_tile_tables(fgx_ctx* ctx);
+// Builds an uncached tile table; *out is hipMalloc'd and owned by the caller.
+int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t** out);
 
 // Checks the descriptor (non-null buffers, supported type combination).
 int check_compressed(const fgx_compressed* A, const char* name);

@@ -6,24 +6,22 @@
 
 namespace fgx {
 
+// One wave per tile (see fixup_tile); waves of tiles that start no run exit
+// after one load.
 template <typename TA>
 __global__ __launch_bounds__(256) void fixup_kernel(const int64_t* __restrict__ carry_row,
                                                     const TA* __restrict__ carry_val,
                                                     int64_t ntiles, int width, TA* __restrict__ out,
                                                     int64_t ld, int64_t kofs, int64_t k) {
-  // One thread per (tile, column).
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t t = gid / width;
-  const int col = static_cast<int>(gid % width);
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
   if (t >= ntiles) return;
-  fixup_run<TA>(carry_row, carry_val, ntiles, t, width, col, out, ld, kofs, k);
+  fixup_tile<TA>(carry_row, carry_val, ntiles, t, width, out, ld, kofs, k);
 }
 
 template <typename TA>
 static inline int launch_fixup(hipStream_t s, const int64_t* crow, const TA* cval, int64_t ntiles,
-                        int width, TA* out, int64_t ld, int64_t kofs, int64_t k) {
-  const int64_t threads = ntiles * width;
-  const int64_t blocks = (threads + 255) / 256;
+                               int width, TA* out, int64_t ld, int64_t kofs, int64_t k) {
+  const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
   hipLaunchKernelGGL((fixup_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                      crow, cval, ntiles, width, out, ld, kofs, k);
   return launch_check("fixup_kernel");

@@ -1,31 +1,24 @@
 // fgx_sddmm.hip — SDDMM: out[q] = A[q] * X[i,:].Y[j,:] on the pattern of A.
 //
-// Fast path (FP32 dense, row-major, width a multiple of 4): every wave takes
-// 64 stored positions at a time, one per lane, and computes the 64 dot
-// products on the matrix cores with V_MFMA_F32_4X4X1_16B_F32.  Its 16 blocks
-// each form the 4x4 outer product of four lanes' X and Y values; summed over
-// the width, the diagonal of block b holds the dot products of lanes
-// 4b..4b+3 (lane t's result sits in accumulator register t % 4 of lane t).
-// Only the diagonal is used: a sampled product has no reuse across
-// positions, so this is a quarter of the MFMA rate, which still runs ahead
-// of the gathers that bound the kernel.
+// Fast path (FP32 dense, row-major, width a multiple of 4): nnz-balanced wave
+// tiles of 4 x 64 positions, dot products on the matrix cores
+// (fgx_sddmm_core.hpp), X/Y rows staged through LDS.
 //
 // General path (any layout, FP64): VALU dot products, one lane per position.
-#include "fgx_csr_tiles.hpp"
+#include "fgx_sddmm_core.hpp"
 #include "fgx_internal.hpp"
 
 namespace fgx {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kSddmmSub = 4;  // 64-position sub-tiles per wave tile
 
 template <typename TP, typename TI, typename TV>
 __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, int64_t k, const float* __restrict__ X, int64_t ldx,
-    const float* __restrict__ Y, int64_t ldy, float* __restrict__ out, int64_t ntiles) {
-  __shared__ int64_t lds[kWavesPerBlock][kWinCap];
+    const float* __restrict__ Y, int64_t ldy, float* __restrict__ out, int64_t ntiles,
+    const int64_t* __restrict__ table) {
+  __shared__ int64_t lds[kWavesPerBlock][kSdWin];
+  __shared__ __align__(16) float xs[kWavesPerBlock][kWave][kSdPitch];
+  __shared__ __align__(16) float ys[kWavesPerBlock][kWave][kSdPitch];
   const int w = threadIdx.x / kWave;
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
@@ -33,11 +26,9 @@ __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
   const int lane = lane_id();
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  int64_t r_lo, r_hi;
-  tile_fibers(ptr, base, m, q0, qend, &r_lo, &r_hi);
-  const int64_t r_last = r_hi < m ? r_hi : m - 1;
-  FiberWindow<TP> win = load_window(ptr, base, r_lo, r_last - r_lo + 1, lds[w], kWinCap);
-
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win = tile_setup(ptr, base, m, q0, qend, table, tile, lds[w], &r_lo, &r_hi,
+                                   &pos_lo, kSdWin);
 #pragma unroll 1
   for (int sub = 0; sub < kSddmmSub; sub++) {
     const int64_t q = q0 + sub * kWave + lane;
@@ -49,24 +40,10 @@ __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
       a = widen<float>(ld_stream(val + q));
       i = win.find(q);
     }
-    const float* xr = X + i * ldx;
-    const float* yr = Y + j * ldy;
     f32x4 c = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int64_t kk = 0; kk < k; kk += 4) {
-      float4 xa = make_float4(0.f, 0.f, 0.f, 0.f), yb = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (in) {
-        xa = *reinterpret_cast<const float4*>(xr + kk);
-        yb = *reinterpret_cast<const float4*>(yr + kk);
-      }
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.x, yb.x, c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.y, yb.y, c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.z, yb.z, c, 0, 0, 0);
-      c = __builtin_amdgcn_mfma_f32_4x4x1f32(xa.w, yb.w, c, 0, 0, 0);
-    }
-    const int d = lane & 3;
-    const float dot = d == 0 ? c[0] : (d == 1 ? c[1] : (d == 2 ? c[2] : c[3]));
-    if (in) out[q] = a * dot;
+    sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, xs[w], ys[w], c);
+    const float dot = mfma_diag(c);
+    if (in) __builtin_nontemporal_store(a * dot, out + q);
     if (q0 + (sub + 1) * kWave >= qend) break;
   }
 }
@@ -98,9 +75,9 @@ __global__ __launch_bounds__(256) void sddmm_valu_kernel(
 }
 
 template <typename TP, typename TI, typename TV>
-static int sddmm_typed(const fgx_compressed* A, int64_t k, const void* Xv, int64_t ldx,
-                       int x_layout, const void* Yv, int64_t ldy, int y_layout, void* outv,
-                       hipStream_t s) {
+static int sddmm_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const void* Xv,
+                       int64_t ldx, int x_layout, const void* Yv, int64_t ldy, int y_layout,
+                       void* outv, hipStream_t s) {
   using TA = typename acc_of<TV>::type;
   const int64_t m = A->n_outer, nnz = A->nnz;
   if (nnz == 0 || m == 0) return FGX_OK;
@@ -117,11 +94,13 @@ static int sddmm_typed(const fgx_compressed* A, int64_t k, const void* Xv, int64
       const int64_t W = kWave * kSddmmSub;
       const int64_t ntiles = (nnz + W - 1) / W;
       const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+      const int64_t* table = nullptr;
+      FGX_TRY(tile_table(ctx, A, W, s, &table));
       hipLaunchKernelGGL((sddmm_mfma_kernel<TP, TI, TV>), dim3(static_cast<unsigned>(blocks)),
                          dim3(256), 0, s, static_cast<const TP*>(A->ptr),
                          static_cast<const TI*>(A->idx), static_cast<const TV*>(A->val),
                          static_cast<int64_t>(A->index_base), m, nnz, k, X, ldx, Y, ldy, out,
-                         ntiles);
+                         ntiles, table);
       return launch_check("sddmm_mfma_kernel");
     }
   }
@@ -163,10 +142,10 @@ extern "C" int fgx_sddmm_csr(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, c
   hipStream_t s = pick_stream(ctx, stream);
   int rc = FGX_OK;
   switch (A->val_type) {
-    case FGX_F64: { using TV = double; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
-    case FGX_F32: { using TV = float; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
-    case FGX_BF16: { using TV = bf16_t; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
-    case FGX_F16: { using TV = f16_t; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
+    case FGX_F64: { using TV = double; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(ctx, A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
+    case FGX_F32: { using TV = float; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(ctx, A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
+    case FGX_BF16: { using TV = bf16_t; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(ctx, A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
+    case FGX_F16: { using TV = f16_t; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(ctx, A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }
     default: return set_error(FGX_EUNSUP, "fgx_sddmm_csr: unsupported value type %d", A->val_type);
   }
   return rc;

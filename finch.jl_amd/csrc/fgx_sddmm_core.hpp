@@ -1,0 +1,75 @@
+// fgx_sddmm_core.hpp — the matrix-core dot products of SDDMM, shared by the
+// plain SDDMM kernel and the batched/serving kernel.
+//
+// A wave takes 64 stored positions at a time, one per lane, and computes the
+// 64 dot products X[i,:].Y[j,:] with V_MFMA_F32_4X4X1_16B_F32: its 16 blocks
+// each form the 4x4 outer product of four lanes' X and Y values; summed over
+// the width, the diagonal of block b holds the dot products of lanes
+// 4b..4b+3.  Only the diagonal is used (a sampled product has no reuse across
+// positions), a quarter of the MFMA rate, which still runs ahead of the row
+// gathers that bound the kernel.
+#pragma once
+#include "fgx_csr_tiles.hpp"
+
+namespace fgx {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kSddmmSub = 4;     // 64-position sub-tiles per wave tile
+constexpr int kSdChunk = 16;     // dense columns staged per pass
+constexpr int kSdPitch = 20;     // LDS row pitch (floats): conflict-free 16-byte reads
+constexpr int kSdWin = 256;      // fibre ends cached per wave
+
+// One 64-position sub-tile: lane t owns position q0+t.  The X rows (of the
+// positions' rows) and Y rows (of their columns) are staged through LDS 16
+// columns at a time with coalesced 16-byte loads (4 lanes per row), then
+// each lane feeds its own X and Y values to the matrix core.
+__device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int64_t ldx,
+                                               const float* __restrict__ Y, int64_t ldy,
+                                               int64_t k, bool in, int64_t i, int64_t j,
+                                               float (*xs)[kSdPitch], float (*ys)[kSdPitch],
+                                               f32x4& c) {
+  const int lane = lane_id();
+  const int part = lane & 3;
+  for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int t = (lane >> 2) + 16 * s;
+      const int64_t it = shfl(i, t);
+      const int64_t jt = shfl(j, t);
+      const int tin = shfl(static_cast<int>(in), t);
+      const int64_t col = c0 + 4 * part;
+      float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (tin && col < k) {
+        xv = *reinterpret_cast<const float4*>(X + it * ldx + col);
+        yv = *reinterpret_cast<const float4*>(Y + jt * ldy + col);
+      }
+      *reinterpret_cast<float4*>(&xs[t][4 * part]) = xv;
+      *reinterpret_cast<float4*>(&ys[t][4 * part]) = yv;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int kk = 0; kk < kSdChunk; kk += 4) {
+      const float4 a4 = *reinterpret_cast<const float4*>(&xs[lane][kk]);
+      const float4 b4 = *reinterpret_cast<const float4*>(&ys[lane][kk]);
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.x, b4.x, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.y, b4.y, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.z, b4.z, c, 0, 0, 0);
+      c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.w, b4.w, c, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Lane t's dot product: the diagonal entry of its 4x4 block, held in
+// accumulator register t % 4.
+__device__ __forceinline__ float mfma_diag(const f32x4& c) {
+  const int d = lane_id() & 3;
+  return d == 0 ? c[0] : (d == 1 ? c[1] : (d == 2 ? c[2] : c[3]));
+}
+
+}  // namespace fgx

@@ -12,14 +12,14 @@ template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int
 __global__ __launch_bounds__(256) void spmm_csr_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, int64_t k, int64_t kofs, const TA* __restrict__ B,
-    int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, int64_t* __restrict__ carry_row,
-    TA* __restrict__ carry_val) {
+    int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, const int64_t* __restrict__ table,
+    int64_t* __restrict__ carry_row, TA* __restrict__ carry_val) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
   const int w = threadIdx.x / kWave;
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
   spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
-                                         tile, lds[w], carry_row + tile,
+                                         tile, table, lds[w], carry_row + tile,
                                          carry_val + tile * (G * KV));
 }
 
@@ -59,12 +59,14 @@ static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t
   FGX_TRY(scratch(ctx, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA) * width), &scr));
   int64_t* crow = static_cast<int64_t*>(scr);
   TA* cval = reinterpret_cast<TA*>(crow + ntiles);
+  const int64_t* table = nullptr;
+  FGX_TRY(tile_table(ctx, A, W, s, &table));
   const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
   hipLaunchKernelGGL((spmm_csr_kernel<TP, TI, TV, TA, G, KV, E, U>),
                      dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                      static_cast<const TP*>(A->ptr), static_cast<const TI*>(A->idx),
                      static_cast<const TV*>(A->val), static_cast<int64_t>(A->index_base), m,
-                     A->nnz, k, kofs, B, ldb, C, ldc, ntiles, crow, cval);
+                     A->nnz, k, kofs, B, ldb, C, ldc, ntiles, table, crow, cval);
   FGX_TRY(launch_check("spmm_csr_kernel"));
   if (ntiles > 1) FGX_TRY(launch_fixup<TA>(s, crow, cval, ntiles, width, C, ldc, kofs, k));
   return FGX_OK;

@@ -16,12 +16,13 @@ template <typename TP, typename TI, typename TV, typename TA, int V>
 __global__ __launch_bounds__(256) void spmv_csr_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, const TA* __restrict__ x, TA* __restrict__ y,
-    int64_t ntiles, int vec, int64_t* __restrict__ carry_row, TA* __restrict__ carry_val) {
+    int64_t ntiles, int vec, const int64_t* __restrict__ table, int64_t* __restrict__ carry_row,
+    TA* __restrict__ carry_val) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
   const int w = threadIdx.x / kWave;
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
-  spmv_tile<TP, TI, TV, TA, V>(ptr, idx, val, base, m, nnz, x, y, tile, vec != 0, lds[w],
+  spmv_tile<TP, TI, TV, TA, V>(ptr, idx, val, base, m, nnz, x, y, tile, vec != 0, table, lds[w],
                                carry_row + tile, carry_val + tile);
 }
 
@@ -98,12 +99,14 @@ static int spmv_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, 
   int64_t* crow = static_cast<int64_t*>(scr);
   TA* cval = reinterpret_cast<TA*>(crow + ntiles);
   const int vec = aligned16(A->val) && aligned16(A->idx);
+  const int64_t* table = nullptr;
+  FGX_TRY(tile_table(ctx, A, W, s, &table));
   const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
   hipLaunchKernelGGL((spmv_csr_kernel<TP, TI, TV, TA, V>), dim3(static_cast<unsigned>(blocks)),
                      dim3(256), 0, s, static_cast<const TP*>(A->ptr),
                      static_cast<const TI*>(A->idx), static_cast<const TV*>(A->val),
                      static_cast<int64_t>(A->index_base), m, A->nnz, static_cast<const TA*>(x),
-                     static_cast<TA*>(y), ntiles, vec, crow, cval);
+                     static_cast<TA*>(y), ntiles, vec, table, crow, cval);
   FGX_TRY(launch_check("spmv_csr_kernel"));
   if (ntiles > 1) FGX_TRY(launch_fixup<TA>(s, crow, cval, ntiles, 1, static_cast<TA*>(y), 1, 0, 1));
   return FGX_OK;
