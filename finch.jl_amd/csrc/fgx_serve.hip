@@ -1,0 +1,463 @@
+// fgx_serve.hip — the serving loop: SpMV / SpMM / SDDMM requests against
+// device-resident operands, replayed from an open-loop arrival trace.
+//
+// Continuous batching (mode 0).  One scheduler thread (the caller) runs:
+//   * release: requests whose arrival time has passed join a FIFO queue;
+//   * completion: finished launches (hipEventQuery) retire their requests,
+//     giving back their slots, HBM footprint and output space;
+//   * admission and launch: while a stream is idle, requests are admitted in
+//     FIFO order as long as the in-flight slots and the HBM budget allow
+//     (footprint = the bytes of a request's inputs and output), and fused
+//     into one persistent launch of the batch kernel (fgx_batch.hip) on
+//     that stream.  Several streams keep launches overlapping on the
+//     device: requests that arrive while others run go into the next launch.
+// Each request's work units depend only on its operand and width, never on
+// the rest of the batch, so a result is bit-identical alone and in any mix.
+//
+// Plain path (mode 1): each request, in arrival order, is one library call
+// followed by a stream synchronisation — the per-call GPU path.
+#include "fgx_batch.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <thread>
+#include <vector>
+
+namespace fgx {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+constexpr int kServeStreams = 4;
+constexpr double kUnitTarget = 256.0 * 1024.0;  // bytes of traffic per work unit
+
+// First-fit allocator over a device arena, for request outputs the server owns.
+class Arena {
+ public:
+  int init(size_t bytes) {
+    size_ = bytes;
+    if (bytes == 0) return FGX_OK;
+    if (hipMalloc(&base_, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      base_ = nullptr;
+      return set_error(FGX_EOOM, "serve: output arena of %zu bytes", bytes);
+    }
+    free_.push_back({0, bytes});
+    return FGX_OK;
+  }
+  ~Arena() {
+    if (base_) (void)hipFree(base_);
+  }
+  void* alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~static_cast<size_t>(255);
+    for (size_t i = 0; i < free_.size(); i++) {
+      if (free_[i].second >= bytes) {
+        const size_t off = free_[i].first;
+        free_[i].first += bytes;
+        free_[i].second -= bytes;
+        if (free_[i].second == 0) free_.erase(free_.begin() + static_cast<long>(i));
+        return static_cast<char*>(base_) + off;
+      }
+    }
+    return nullptr;
+  }
+  void release(void* p, size_t bytes) {
+    bytes = (bytes + 255) & ~static_cast<size_t>(255);
+    const size_t off = static_cast<size_t>(static_cast<char*>(p) - static_cast<char*>(base_));
+    auto it = std::lower_bound(free_.begin(), free_.end(), std::make_pair(off, size_t(0)));
+    it = free_.insert(it, {off, bytes});
+    // Coalesce with the neighbours.
+    size_t i = static_cast<size_t>(it - free_.begin());
+    if (i + 1 < free_.size() && free_[i].first + free_[i].second == free_[i + 1].first) {
+      free_[i].second += free_[i + 1].second;
+      free_.erase(free_.begin() + static_cast<long>(i) + 1);
+    }
+    if (i > 0 && free_[i - 1].first + free_[i - 1].second == free_[i].first) {
+      free_[i - 1].second += free_[i].second;
+      free_.erase(free_.begin() + static_cast<long>(i));
+    }
+  }
+  size_t size() const { return size_; }
+
+ private:
+  void* base_ = nullptr;
+  size_t size_ = 0;
+  std::vector<std::pair<size_t, size_t>> free_;
+};
+
+struct OperandInfo {
+  int64_t max_row = 0;
+  // Tile tables of large operands, by width class: [0] SpMV, [1] SDDMM, and
+  // per SpMM width the table for tile_width_for_k(k) (looked up below).
+  std::vector<std::pair<int64_t, int64_t*>> tables;  // (W, table)
+  const int64_t* table_for(int64_t W) const {
+    for (auto& t : tables)
+      if (t.first == W) return t.second;
+    return nullptr;
+  }
+};
+
+struct Launch {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  int32_t seq = 0;
+  std::vector<int64_t> reqs;
+  fgx_batch_plan plan;
+  BatchDesc* h_desc = nullptr;
+  BatchWork* h_work = nullptr;
+  BatchWork* h_fix = nullptr;
+  int64_t cap_desc = 0, cap_work = 0, cap_fix = 0;
+};
+
+template <typename T>
+int grow_pinned(T** buf, int64_t* cap, int64_t need) {
+  if (need <= *cap && *buf) return FGX_OK;
+  const int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * *cap, 256));
+  if (*buf) (void)hipHostFree(*buf);
+  *buf = nullptr;
+  *cap = 0;
+  if (hipHostMalloc(reinterpret_cast<void**>(buf), static_cast<size_t>(want) * sizeof(T)) !=
+      hipSuccess)
+    return set_error(FGX_EOOM, "serve: pinned staging allocation failed");
+  *cap = want;
+  return FGX_OK;
+}
+
+struct Sizes {
+  int64_t in = 0, in2 = 0, out = 0;
+  int64_t footprint() const { return in + in2 + out; }
+};
+
+Sizes request_sizes(const fgx_request& r, const fgx_compressed& A) {
+  Sizes z;
+  const int64_t k = r.kind == FGX_REQ_SPMV ? 1 : r.k;
+  if (r.kind == FGX_REQ_SDDMM) {
+    z.in = A.n_outer * k * 4;
+    z.in2 = A.n_inner * k * 4;
+    z.out = A.nnz * 4;
+  } else {
+    z.in = A.n_inner * k * 4;
+    z.out = A.n_outer * k * 4;
+  }
+  return z;
+}
+
+// Algorithmic bytes of one request (its operand read once, dense in/out once).
+double request_cost(const fgx_request& r, const fgx_compressed& A, int vbytes) {
+  const Sizes z = request_sizes(r, A);
+  return static_cast<double>(A.nnz) * (vbytes + 4) + 4.0 * (A.n_outer + 1) +
+         static_cast<double>(z.footprint());
+}
+
+}  // namespace
+
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64_t n_operands,
+                             const fgx_request* reqs, int64_t n_reqs, const fgx_serve_config* cfg,
+                             fgx_request_result* results) {
+  clear_error();
+  if (!ctx || !cfg || (n_reqs > 0 && (!reqs || !results)) || (n_operands > 0 && !operands))
+    return set_error(FGX_EINVAL, "fgx_serve_run: NULL argument");
+  if (n_operands <= 0 && n_reqs > 0) return set_error(FGX_EINVAL, "fgx_serve_run: no operands");
+  if (cfg->mode != 0 && cfg->mode != 1) return set_error(FGX_EINVAL, "fgx_serve_run: bad mode");
+  const int max_slots = std::max(1, cfg->max_slots);
+  const int max_batch = std::max(1, cfg->max_batch);
+  const int64_t budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : INT64_MAX;
+  // Operand checks: one value type and index combination for the whole set.
+  IdxCombo combo = IDX_32_32;
+  for (int64_t i = 0; i < n_operands; i++) {
+    FGX_TRY(check_compressed(&operands[i], "fgx_serve_run"));
+    IdxCombo ic;
+    FGX_TRY(idx_combo(&operands[i], &ic));
+    if (i == 0) combo = ic;
+    if (ic != combo || operands[i].val_type != operands[0].val_type)
+      return set_error(FGX_EUNSUP, "fgx_serve_run: operands must share index and value types");
+    if (operands[i].val_type != FGX_F32 && operands[i].val_type != FGX_BF16 &&
+        operands[i].val_type != FGX_F16)
+      return set_error(FGX_EUNSUP, "fgx_serve_run: values must be F32, BF16 or F16");
+  }
+  for (int64_t r = 0; r < n_reqs; r++) {
+    const fgx_request& q = reqs[r];
+    if (q.operand < 0 || q.operand >= n_operands || q.kind < FGX_REQ_SPMV || q.kind > FGX_REQ_SDDMM)
+      return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: bad operand or kind", (long long)r);
+    const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
+    if (k < 1 || k > 256 || (k > 1 && k % 4 != 0))
+      return set_error(FGX_EUNSUP, "fgx_serve_run: request %lld: width must be 1 or a multiple "
+                                   "of 4 up to 256", (long long)r);
+    if (!q.in || (q.kind == FGX_REQ_SDDMM && !q.in2))
+      return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: NULL input", (long long)r);
+  }
+  FGX_HIP(hipSetDevice(ctx->device));
+  const int vbytes = n_operands > 0 ? elem_size(operands[0].val_type) : 4;
+
+  // ---- set-up (untimed): streams, operand statistics, tile tables, arena ----
+  std::vector<Launch> L(kServeStreams);
+  int rc = FGX_OK;
+  for (auto& l : L) {
+    if (hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&l.done, hipEventDisableTiming) != hipSuccess)
+      rc = set_error(FGX_EHIP, "fgx_serve_run: stream/event creation failed");
+  }
+  std::vector<OperandInfo> info(static_cast<size_t>(n_operands));
+  std::vector<int64_t> maxrow(static_cast<size_t>(n_operands), 0);
+  if (rc == FGX_OK) rc = max_row_lengths(operands, n_operands, combo, maxrow.data(), L[0].stream);
+  // Tile tables for the widths each operand is requested at, where the
+  // autotuner picks nnz-split tiles, and for SDDMM on large operands.
+  if (rc == FGX_OK) {
+    for (int64_t r = 0; r < n_reqs && rc == FGX_OK; r++) {
+      const fgx_request& q = reqs[r];
+      const fgx_compressed& A = operands[q.operand];
+      OperandInfo& oi = info[static_cast<size_t>(q.operand)];
+      oi.max_row = maxrow[static_cast<size_t>(q.operand)];
+      const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
+      int64_t W = 0;
+      if (q.kind == FGX_REQ_SDDMM) {
+        if (A.n_outer >= 65536) W = kWave;
+      } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0).kind == BK_TILES) {
+        W = tile_width_for_k(k);
+      }
+      if (W > 0 && !oi.table_for(W)) {
+        int64_t* t = nullptr;
+        rc = build_tile_table(&A, W, L[0].stream, &t);
+        if (rc == FGX_OK) oi.tables.push_back({W, t});
+      }
+    }
+  }
+  for (int64_t i = 0; i < n_operands; i++) info[static_cast<size_t>(i)].max_row = maxrow[static_cast<size_t>(i)];
+  // Output arena for requests without an `out` buffer.
+  Arena arena;
+  int64_t max_out = 0;
+  bool needs_arena = false;
+  for (int64_t r = 0; r < n_reqs; r++) {
+    if (!reqs[r].out) {
+      needs_arena = true;
+      max_out = std::max(max_out, request_sizes(reqs[r], operands[reqs[r].operand]).out);
+    }
+  }
+  if (rc == FGX_OK && needs_arena) {
+    size_t want = static_cast<size_t>(std::min<int64_t>(budget, int64_t(8) << 30));
+    want = std::max(want, static_cast<size_t>(max_out) + 256);
+    rc = arena.init(want);
+  }
+  if (rc == FGX_OK && hipStreamSynchronize(L[0].stream) != hipSuccess)
+    rc = set_error(FGX_EHIP, "fgx_serve_run: set-up failed");
+
+  std::vector<int64_t> order(static_cast<size_t>(n_reqs));
+  for (int64_t i = 0; i < n_reqs; i++) order[static_cast<size_t>(i)] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int64_t a, int64_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+  std::vector<void*> outp(static_cast<size_t>(n_reqs), nullptr);
+  for (int64_t r = 0; r < n_reqs; r++) {
+    results[r].admit_s = results[r].done_s = -1.0;
+    results[r].status = FGX_OK;
+    results[r].batch = -1;
+  }
+
+  const auto t0 = Clock::now();
+  auto now_s = [&]() { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+
+  if (rc == FGX_OK && cfg->mode == 1) {
+    // ---- plain path: one call per request, synchronised ----
+    hipStream_t s = L[0].stream;
+    int32_t seq = 0;
+    for (size_t oi = 0; oi < order.size() && rc == FGX_OK; oi++) {
+      const int64_t r = order[oi];
+      const fgx_request& q = reqs[r];
+      const fgx_compressed& A = operands[q.operand];
+      while (now_s() < q.arrival_s) std::this_thread::yield();
+      const Sizes z = request_sizes(q, A);
+      if (z.footprint() > budget) {
+        results[r].status = FGX_EADMIT;
+        results[r].admit_s = results[r].done_s = now_s();
+        continue;
+      }
+      void* out = q.out;
+      if (!out) out = arena.alloc(static_cast<size_t>(z.out));
+      results[r].admit_s = now_s();
+      if (q.kind == FGX_REQ_SPMV) rc = fgx_spmv_csr(ctx, &A, q.in, out, s);
+      else if (q.kind == FGX_REQ_SPMM)
+        rc = fgx_spmm_csr(ctx, &A, q.k, q.in, q.k, FGX_ROW_MAJOR, out, q.k, FGX_ROW_MAJOR, s);
+      else
+        rc = fgx_sddmm_csr(ctx, &A, q.k, q.in, q.k, FGX_ROW_MAJOR, q.in2, q.k, FGX_ROW_MAJOR, out, s);
+      if (rc == FGX_OK && hipStreamSynchronize(s) != hipSuccess)
+        rc = set_error(FGX_EHIP, "fgx_serve_run: plain call failed");
+      results[r].done_s = now_s();
+      results[r].batch = seq++;
+      if (!q.out) arena.release(out, static_cast<size_t>(z.out));
+    }
+  } else if (rc == FGX_OK) {
+    // ---- continuous batching ----
+    std::deque<int64_t> pending;
+    size_t next = 0;
+    int64_t finished = 0, inflight_reqs = 0, inflight_bytes = 0;
+    int32_t seq = 0;
+    std::vector<BatchDesc> hdesc;
+    std::vector<BatchWork> hwork, hfix;
+    while (finished < n_reqs && rc == FGX_OK) {
+      bool progress = false;
+      const double tn = now_s();
+      while (next < order.size() && reqs[order[next]].arrival_s <= tn) {
+        pending.push_back(order[next++]);
+        progress = true;
+      }
+      // Completions.
+      for (auto& l : L) {
+        if (!l.busy) continue;
+        const hipError_t e = hipEventQuery(l.done);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) {
+          rc = set_error(FGX_EHIP, "fgx_serve_run: launch failed: %s", hipGetErrorString(e));
+          break;
+        }
+        const double td = now_s();
+        for (int64_t r : l.reqs) {
+          results[r].done_s = td;
+          const Sizes z = request_sizes(reqs[r], operands[reqs[r].operand]);
+          inflight_bytes -= z.footprint();
+          inflight_reqs--;
+          if (!reqs[r].out) arena.release(outp[static_cast<size_t>(r)], static_cast<size_t>(z.out));
+          finished++;
+        }
+        l.reqs.clear();
+        l.busy = false;
+        progress = true;
+      }
+      if (rc != FGX_OK) break;
+      // Admission and launch.
+      for (auto& l : L) {
+        if (l.busy || pending.empty()) continue;
+        hdesc.clear();
+        hwork.clear();
+        hfix.clear();
+        int64_t carry_tiles = 0, carry_vals = 0;
+        bool has_sddmm = false;
+        double batch_cost = 0.0;
+        while (!pending.empty() && static_cast<int>(l.reqs.size()) < max_batch &&
+               inflight_reqs < max_slots) {
+          const int64_t r = pending.front();
+          const fgx_request& q = reqs[r];
+          const fgx_compressed& A = operands[q.operand];
+          const Sizes z = request_sizes(q, A);
+          if (z.footprint() > budget || (!q.out && static_cast<size_t>(z.out) + 256 > arena.size())) {
+            pending.pop_front();
+            results[r].status = FGX_EADMIT;
+            results[r].admit_s = results[r].done_s = now_s();
+            finished++;
+            continue;
+          }
+          if (inflight_bytes + z.footprint() > budget) break;
+          const double cost = request_cost(q, A, vbytes);
+          // A large request goes in a launch of its own, so small requests
+          // never wait behind it in the same launch.
+          if (!l.reqs.empty() && (cost > 32.0e6 || batch_cost > 32.0e6)) break;
+          void* out = q.out;
+          if (!out) {
+            out = arena.alloc(static_cast<size_t>(z.out));
+            if (!out) break;
+          }
+          pending.pop_front();
+          outp[static_cast<size_t>(r)] = out;
+          results[r].admit_s = now_s();
+          results[r].batch = seq;
+          inflight_reqs++;
+          inflight_bytes += z.footprint();
+          batch_cost += cost;
+          l.reqs.push_back(r);
+          // Descriptor and work units of this request.
+          const OperandInfo& oi = info[static_cast<size_t>(q.operand)];
+          const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
+          BatchDesc d{};
+          d.ptr = A.ptr;
+          d.idx = A.idx;
+          d.val = A.val;
+          d.B = static_cast<const float*>(q.in);
+          d.B2 = static_cast<const float*>(q.in2);
+          d.C = static_cast<float*>(out);
+          d.m = A.n_outer;
+          d.n = A.n_inner;
+          d.nnz = A.nnz;
+          d.k = k;
+          d.ldb = k;
+          d.ldc = k;
+          d.base = A.index_base;
+          d.vec = aligned16(A.val) && aligned16(A.idx);
+          const int32_t item = static_cast<int32_t>(hdesc.size());
+          if (q.kind == FGX_REQ_SDDMM) {
+            d.op = BO_SDDMM;
+            d.mfma = (k % 4 == 0) && aligned16(q.in) && aligned16(q.in2);
+            d.table = oi.table_for(kWave);
+            has_sddmm = true;
+            const int64_t nsub = (A.nnz + kWave - 1) / kWave;
+            const int64_t per = std::max<int64_t>(
+                1, static_cast<int64_t>(kUnitTarget / (kWave * (8.0 + 8.0 * static_cast<double>(k)))));
+            for (int64_t t = 0; t < nsub; t += per)
+              hwork.push_back(BatchWork{item, BK_SDDMM, t, std::min(nsub, t + per)});
+          } else {
+            d.op = BO_SPMM;
+            const TunePick pk = autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0);
+            if (pk.kind == BK_TILES) {
+              const int64_t W = tile_width_for_k(k);
+              const int64_t nt = A.nnz > 0 ? (A.nnz + W - 1) / W : 1;
+              const int width = k == 1 ? 1 : group_for_k(k) * 4;
+              d.table = oi.table_for(W);
+              d.carry_tile0 = carry_tiles;
+              d.carry_val0 = carry_vals;
+              carry_tiles += nt;
+              carry_vals += nt * width;
+              for (int64_t t = 0; t < nt; t += pk.unit) {
+                const BatchWork w{item, BK_TILES, t, std::min(nt, t + pk.unit)};
+                hwork.push_back(w);
+                hfix.push_back(w);
+              }
+            } else if (A.n_outer > 0) {
+              for (int64_t r0 = 0; r0 < A.n_outer; r0 += pk.unit)
+                hwork.push_back(BatchWork{item, BK_ROWS, r0, std::min(A.n_outer, r0 + pk.unit)});
+            }
+          }
+          hdesc.push_back(d);
+        }
+        if (l.reqs.empty()) continue;
+        rc = grow_pinned(&l.h_desc, &l.cap_desc, static_cast<int64_t>(hdesc.size()));
+        if (rc == FGX_OK) rc = grow_pinned(&l.h_work, &l.cap_work, static_cast<int64_t>(hwork.size()));
+        if (rc == FGX_OK) rc = grow_pinned(&l.h_fix, &l.cap_fix, static_cast<int64_t>(hfix.size()));
+        if (rc != FGX_OK) break;
+        std::copy(hdesc.begin(), hdesc.end(), l.h_desc);
+        std::copy(hwork.begin(), hwork.end(), l.h_work);
+        std::copy(hfix.begin(), hfix.end(), l.h_fix);
+        l.plan.val_type = operands[0].val_type;
+        l.plan.combo = combo;
+        rc = batch_launch_staged(&l.plan, l.h_desc, static_cast<int64_t>(hdesc.size()), l.h_work,
+                                 static_cast<int64_t>(hwork.size()), l.h_fix,
+                                 static_cast<int64_t>(hfix.size()), carry_tiles, carry_vals,
+                                 has_sddmm, ctx->num_cus, l.stream);
+        if (rc == FGX_OK && hipEventRecord(l.done, l.stream) != hipSuccess)
+          rc = set_error(FGX_EHIP, "fgx_serve_run: event record failed");
+        if (rc != FGX_OK) break;
+        l.busy = true;
+        l.seq = seq++;
+        progress = true;
+      }
+      if (!progress) std::this_thread::yield();
+    }
+  }
+
+  // ---- tear-down ----
+  for (auto& l : L) {
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    batch_free(&l.plan);
+    if (l.h_desc) (void)hipHostFree(l.h_desc);
+    if (l.h_work) (void)hipHostFree(l.h_work);
+    if (l.h_fix) (void)hipHostFree(l.h_fix);
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+  }
+  for (auto& oi : info)
+    for (auto& t : oi.tables) (void)hipFree(t.second);
+  return rc;
+}

@@ -1,0 +1,288 @@
+"""Finch's level formats on the host and on the device.
+
+Mirrors the parts of Finch's tensor interface that the backend's kernels read
+and write:
+
+* ``Element(fill)`` — leaf values (src/tensors/levels/element_levels.jl:20-22);
+* ``SparseList(lvl, shape)`` — 1-based ``ptr``/``idx`` compressed level
+  (src/tensors/levels/sparse_list_levels.jl:37-42);
+* ``Dense(lvl, shape)`` — dense level (src/tensors/levels/dense_levels.jl:26-29);
+* ``Tensor(lvl, arr)`` — a tensor over a level tree, built from a dense array
+  the way ``dropfills!`` does (src/interface/copy.jl:113-116): column-major
+  traversal, fill values dropped, every other value stored;
+* ``swizzle(A, 2, 1)`` — the zero-copy index permutation Finch uses to read
+  column-compressed buffers row-wise (src/tensors/combinators/swizzle.jl:1-64);
+* ``GPU(id)`` and ``transfer(device, tensor)`` — the device and the move of
+  level buffers onto it (src/architecture.jl:163,269-294; per level
+  sparse_list_levels.jl:64-69, dense_levels.jl:47-49, element_levels.jl:46-48).
+
+Buffers are numpy arrays on the host and torch tensors on a HIP device.  The
+formats the kernels take are Dense(Element) (vectors and column-major
+matrices) and Dense(SparseList(Element)) (compressed matrices).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+try:  # torch is only needed for device buffers
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+def _is_device(a) -> bool:
+    return torch is not None and isinstance(a, torch.Tensor) and a.is_cuda
+
+
+class Element:
+    def __init__(self, fill=0.0, dtype=None, val=None):
+        self.fill = fill
+        self.dtype = np.dtype(dtype) if dtype is not None else np.asarray(fill).dtype
+        self.val = val if val is not None else np.zeros(0, self.dtype)
+
+    def __repr__(self):
+        return f"Element{{{self.fill}, {self.dtype}}}({_show(self.val)})"
+
+
+class SparseList:
+    def __init__(self, lvl, shape=None, ptr=None, idx=None, Ti=np.int64):
+        self.lvl = lvl
+        self.shape = shape
+        self.Ti = np.dtype(Ti)
+        self.ptr = ptr if ptr is not None else np.ones(1, self.Ti)
+        self.idx = idx if idx is not None else np.zeros(0, self.Ti)
+
+    def __repr__(self):
+        return f"SparseList{{{self.Ti}}}({self.lvl!r}, {self.shape}, {_show(self.ptr)}, {_show(self.idx)})"
+
+
+class Dense:
+    def __init__(self, lvl, shape=None):
+        self.lvl = lvl
+        self.shape = shape
+
+    def __repr__(self):
+        return f"Dense({self.lvl!r}, {self.shape})"
+
+
+def _show(a):
+    if _is_device(a):
+        return f"<{a.device} {tuple(a.shape)} {a.dtype}>"
+    return np.asarray(a).tolist()
+
+
+class GPU:
+    """A HIP device: Finch's `AbstractDevice` for the backend (one per process)."""
+
+    def __init__(self, index: int = 0):
+        self.index = int(index)
+
+    @property
+    def torch_device(self):
+        return torch.device("cuda", self.index)
+
+    def __repr__(self):
+        return f"GPU({self.index})"
+
+
+class Tensor:
+    """A tensor over a level tree: Dense(Element), SparseList(Element) or
+    Dense(SparseList(Element))."""
+
+    def __init__(self, lvl, arr=None):
+        self.lvl = lvl
+        if arr is not None:
+            self._fill_from(arr)
+
+    # ---- formats --------------------------------------------------------------
+    @property
+    def format(self) -> str:
+        l = self.lvl
+        if isinstance(l, Dense) and isinstance(l.lvl, Element):
+            return "dense"
+        if isinstance(l, Dense) and isinstance(l.lvl, Dense) and isinstance(l.lvl.lvl, Element):
+            return "dense2"
+        if isinstance(l, SparseList) and isinstance(l.lvl, Element):
+            return "sparsevec"
+        if isinstance(l, Dense) and isinstance(l.lvl, SparseList) and isinstance(l.lvl.lvl, Element):
+            return "csc"
+        raise NotImplementedError(f"level tree {l!r} is not a backend format")
+
+    @property
+    def shape(self):
+        f = self.format
+        if f == "dense" or f == "sparsevec":
+            return (self.lvl.shape,)
+        if f == "dense2":
+            return (self.lvl.lvl.shape, self.lvl.shape)
+        return (self.lvl.lvl.shape, self.lvl.shape)
+
+    @property
+    def element(self) -> Element:
+        l = self.lvl
+        while not isinstance(l, Element):
+            l = l.lvl
+        return l
+
+    @property
+    def fill_value(self):
+        return self.element.fill
+
+    @property
+    def on_device(self) -> bool:
+        return _is_device(self.element.val)
+
+    def countstored(self) -> int:
+        f = self.format
+        if f in ("csc",):
+            return int(self.lvl.lvl.idx.numel() if self.on_device else len(self.lvl.lvl.idx))
+        if f == "sparsevec":
+            return int(len(self.lvl.idx))
+        return int(np.prod(self.shape))
+
+    # ---- construction ----------------------------------------------------------
+    def _fill_from(self, arr):
+        f = self.format
+        el = self.element
+        fill = el.fill
+        if hasattr(arr, "tocsc"):  # scipy.sparse (zero-copy in Finch; a copy here)
+            if f != "csc":
+                raise NotImplementedError("scipy input needs Dense(SparseList(Element))")
+            S = arr.tocsc()
+            S.sort_indices()
+            sl = self.lvl.lvl
+            sl.shape, self.lvl.shape = S.shape
+            sl.ptr = S.indptr.astype(sl.Ti) + 1
+            sl.idx = S.indices.astype(sl.Ti) + 1
+            el.val = S.data.astype(el.dtype if el.val.size == 0 else el.val.dtype)
+            return
+        a = np.asarray(arr)
+        dt = np.result_type(a.dtype, np.asarray(fill).dtype) if el.val.size == 0 else el.val.dtype
+        if f == "dense":
+            self.lvl.shape = a.shape[0]
+            el.val = a.astype(dt).copy()
+        elif f == "dense2":
+            self.lvl.lvl.shape, self.lvl.shape = a.shape
+            el.val = np.asfortranarray(a).astype(dt).ravel(order="F")
+        elif f == "sparsevec":
+            keep = a != fill
+            self.lvl.shape = a.shape[0]
+            self.lvl.ptr = np.array([1, keep.sum() + 1], self.lvl.Ti)
+            self.lvl.idx = (np.nonzero(keep)[0] + 1).astype(self.lvl.Ti)
+            el.val = a[keep].astype(dt)
+        else:  # csc: column-major traversal of a 2-D array
+            m, n = a.shape
+            sl = self.lvl.lvl
+            sl.shape, self.lvl.shape = m, n
+            keep = a != fill
+            counts = keep.sum(axis=0)
+            sl.ptr = np.concatenate([[1], 1 + np.cumsum(counts)]).astype(sl.Ti)
+            rows, cols = np.nonzero(keep.T)           # column-major order
+            sl.idx = (cols + 1).astype(sl.Ti)
+            el.val = a.T[rows, cols].astype(dt)
+        el.dtype = np.dtype(dt) if not _is_device(el.val) else el.dtype
+
+    @classmethod
+    def from_buffers(cls, ptr, idx, val, m, n, fill=0.0, base=1):
+        """Dense(SparseList(Element)) over existing buffers (1-based, or 0-based
+        with base=0 as a PlusOneVector view)."""
+        t = cls(Dense(SparseList(Element(fill, np.asarray(val).dtype if not _is_device(val) else None),
+                                 m), n))
+        sl = t.lvl.lvl
+        if base == 0:
+            ptr = ptr + 1
+            idx = idx + 1
+        sl.ptr, sl.idx = ptr, idx
+        t.element.val = val
+        return t
+
+    # ---- views -------------------------------------------------------------------
+    def todense(self) -> np.ndarray:
+        """Host dense copy (for inspection and tests)."""
+        f = self.format
+        el = self.element
+        host = lambda a: a.cpu().numpy() if _is_device(a) else np.asarray(a)
+        val = host(el.val)
+        if f == "dense":
+            return val.copy()
+        if f == "dense2":
+            return val.reshape(self.shape, order="F").copy()
+        if f == "sparsevec":
+            out = np.full(self.shape, el.fill, dtype=val.dtype)
+            out[host(self.lvl.idx) - 1] = val
+            return out
+        m, n = self.shape
+        sl = self.lvl.lvl
+        ptr, idx = host(sl.ptr).astype(np.int64), host(sl.idx).astype(np.int64)
+        out = np.full((m, n), el.fill, dtype=val.dtype)
+        cols = np.repeat(np.arange(n), np.diff(ptr))
+        out[idx - 1, cols] = val
+        return out
+
+    def __repr__(self):
+        return f"Tensor({self.lvl!r})"
+
+
+class SwizzleArray:
+    """swizzle(A, dims...): A read with permuted indices (no data moves)."""
+
+    def __init__(self, body: Tensor, dims):
+        self.body = body
+        self.dims = tuple(dims)
+
+    @property
+    def shape(self):
+        s = self.body.shape
+        return tuple(s[d - 1] for d in self.dims)
+
+    def todense(self):
+        return np.transpose(self.body.todense(), [d - 1 for d in self.dims])
+
+    def __repr__(self):
+        return f"swizzle({self.body!r}, {', '.join(map(str, self.dims))})"
+
+
+def swizzle(A, *dims) -> SwizzleArray:
+    if isinstance(A, SwizzleArray):  # compose permutations
+        dims = tuple(A.dims[d - 1] for d in dims)
+        A = A.body
+    if sorted(dims) != list(range(1, len(dims) + 1)):
+        raise ValueError(f"{dims} is not a permutation")
+    return SwizzleArray(A, dims)
+
+
+def transfer(device: GPU, t):
+    """Moves every level buffer of `t` to `device` (torch tensors)."""
+    if isinstance(t, SwizzleArray):
+        return SwizzleArray(transfer(device, t.body), t.dims)
+    dev = device.torch_device
+    mv = lambda a: a.to(dev) if _is_device(a) else torch.as_tensor(np.ascontiguousarray(a)).to(dev)
+    out = Tensor(_copy_levels(t.lvl))
+    l = out.lvl
+    while True:
+        if isinstance(l, SparseList):
+            l.ptr, l.idx = mv(l.ptr), mv(l.idx)
+        if isinstance(l, Element):
+            l.val = mv(l.val)
+            break
+        l = l.lvl
+    return out
+
+
+def _copy_levels(l):
+    if isinstance(l, Element):
+        return Element(l.fill, l.dtype, l.val)
+    if isinstance(l, SparseList):
+        return SparseList(_copy_levels(l.lvl), l.shape, l.ptr, l.idx, l.Ti)
+    return Dense(_copy_levels(l.lvl), l.shape)
+
+
+def fsprand(m, n, p, seed=0, dtype=np.float64, Ti=np.int64):
+    """A Dense(SparseList(Element(0))) m x n tensor with density p (the
+    distribution of Finch's fsprand; see finchgpu.synth)."""
+    from .synth import fsprand as _fsprand
+    ptr, idx, val = _fsprand(m, n, p, seed=seed, dtype=dtype)
+    t = Tensor(Dense(SparseList(Element(dtype(0)), m, Ti=Ti), n))
+    t.lvl.lvl.ptr, t.lvl.lvl.idx = ptr.astype(Ti), idx.astype(Ti)
+    t.element.val = val
+    return t

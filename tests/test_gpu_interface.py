@@ -1,0 +1,124 @@
+"""GPU tests of the Finch interface mirror: programs written as in Finch's own
+tests, run through einsum / lazy / compute, checked against the oracle and
+Finch's goldens."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+import oracle
+from conftest import read_mtx
+from finchgpu import interface as fi
+from finchgpu.tensors import GPU, Dense, Element, SparseList, Tensor, swizzle, transfer
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return np.linalg.norm(got - ref) / max(np.linalg.norm(ref), 1e-300)
+
+
+def test_spmv_csc_golden(dev, goldens):
+    # test/suites/typical_tests.jl:47-73 with typical_spmv_csc.txt:92.
+    g = goldens["spmv_csc"]
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), np.array(g["A_dense"])))
+    x = transfer(GPU(0), Tensor(Dense(Element(0.0)), np.array(g["x"])))
+    y = fi.einsum("y[i] += A[i, j] * x[j]", A=A, x=x)
+    assert y.todense().tolist() == g["y"]
+
+
+def test_spmv_swizzled_csr(dev, goldens):
+    g = goldens["spmv_transpose"]
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), np.array(g["A_dense"])))
+    x = transfer(GPU(0), Tensor(Dense(Element(0.0)), np.array(g["x"])))
+    # swizzle(A, 2, 1)[i, j] = A[j, i]: row-wise read of the column buffers.
+    y = fi.einsum("y[i] += S[i, j] * x[j]", S=swizzle(A, 2, 1), x=x)
+    assert y.todense().tolist() == g["y"]
+
+
+def _rand(m, n, density, seed, dtype=np.float64):
+    rng = np.random.default_rng(seed)
+    S = sp.random(m, n, density=density, format="csc", random_state=rng, dtype=np.float64)
+    S.data = rng.standard_normal(S.nnz)
+    return S.astype(dtype)
+
+
+def test_spmm_with_finch_column_major_dense(dev):
+    # @einsum C[i,k] += A[i,j]*B[j,k] (interface_tests.jl:22-24) with A read
+    # row-wise (swizzle of the transposed storage) and B in Finch's
+    # column-major Dense(Dense(Element)).
+    S = _rand(300, 200, 0.05, 1)
+    At = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S.T.tocsc()))  # stores A^T
+    Bn = np.random.default_rng(2).standard_normal((200, 7))
+    B = transfer(GPU(0), Tensor(Dense(Dense(Element(0.0))), Bn))
+    C = fi.einsum("C[i, k] += A[i, j] * B[j, k]", A=swizzle(At, 2, 1), B=B)
+    assert C.shape == (300, 7)
+    assert rel_err(C.todense(), S @ Bn) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["west0132.mtx", "lp_blend.mtx"])
+def test_gustavson_column_compressed(dev, name):
+    # kernel_tests.jl:89-130: B == A*A (or A*A^T) on Finch's fixtures, with the
+    # operands in Finch's default column-compressed format.
+    S = read_mtx(name).tocsc()
+    T = S if S.shape[0] == S.shape[1] else S.T.tocsc()
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T))
+    C = fi.einsum("C[i, k] += A[i, j] * B[j, k]", A=A, B=B)
+    ref = (S @ T).toarray()
+    assert rel_err(C.todense(), ref) <= 1e-12
+    # Same update order as the reference's column-wise Gustavson: bit-exact
+    # against the oracle run column by column (rows of T^T x rows of S^T).
+    Tt, St = T.T.tocsr(), S.T.tocsr()
+    Tt.sort_indices()
+    St.sort_indices()
+    cptr, cidx, cval = oracle.spgemm("plus_times", Tt.shape[0], St.shape[1], Tt.indptr + 1,
+                                     Tt.indices + 1, Tt.data, St.indptr + 1, St.indices + 1,
+                                     St.data)
+    sl = C.lvl.lvl
+    assert np.array_equal(sl.ptr.cpu().numpy(), cptr)
+    assert np.array_equal(sl.idx.cpu().numpy(), cidx)
+    assert np.array_equal(C.element.val.cpu().numpy(), cval)
+
+
+def test_min_plus_einsum(dev):
+    # interface_tests.jl:70-84 (max-plus with fill -Inf) in its min-plus form.
+    rng = np.random.default_rng(3)
+    An = np.where(rng.random((9, 8)) < 0.4, rng.standard_normal((9, 8)), np.inf)
+    Bn = np.where(rng.random((8, 7)) < 0.4, rng.standard_normal((8, 7)), np.inf)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.inf))), An))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.inf))), Bn))
+    C = fi.einsum("C[i, k] <<min>>= A[i, j] + B[j, k]", A=A, B=B)
+    ref = np.full((9, 7), np.inf)
+    for i in range(9):
+        for k in range(7):
+            for j in range(8):
+                ref[i, k] = min(ref[i, k], An[i, j] + Bn[j, k])
+    assert np.array_equal(C.todense(), ref)
+
+
+@pytest.mark.parametrize("prog,fn", [("C[i, j] = A[i, j] * B[i, j]", lambda a, b: a * b),
+                                     ("C[i, j] = A[i, j] + B[i, j]", lambda a, b: a + b),
+                                     ("C[i, j] = max(A[i, j], B[i, j])", np.maximum)])
+def test_elementwise(dev, prog, fn):
+    S, T = _rand(60, 50, 0.1, 4), _rand(60, 50, 0.2, 5)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T))
+    C = fi.einsum(prog, A=A, B=B)
+    assert np.array_equal(C.todense(), fn(S.toarray(), T.toarray()))
+
+
+def test_fused_sddmm_compute(dev):
+    # compute(lazy(M) .* (lazy(A) * lazy(B))) (benchmark/benchmarks.jl:141-168).
+    S = _rand(80, 70, 0.1, 6, np.float32)
+    Mt = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.float32(0)))), S.T.tocsc()))
+    rng = np.random.default_rng(7)
+    Xn = rng.standard_normal((80, 16)).astype(np.float32)
+    Yn = rng.standard_normal((16, 70)).astype(np.float32)
+    X = transfer(GPU(0), Tensor(Dense(Dense(Element(np.float32(0)))), Xn))
+    Y = transfer(GPU(0), Tensor(Dense(Dense(Element(np.float32(0)))), Yn))
+    C = fi.compute(fi.lazy(swizzle(Mt, 2, 1)) * (fi.lazy(X) @ fi.lazy(Y)))
+    ref = S.toarray().astype(np.float64) * (Xn.astype(np.float64) @ Yn.astype(np.float64))
+    assert rel_err(C.todense(), ref) <= 1e-5
