@@ -13,7 +13,12 @@ in HBM before the timed region; `roofline` compares it with the 8 TB/s HBM
 peak.  `cpu_baseline` times the oracle's threaded CSR SpMM (the C restatement
 of Finch's row-parallel loop) on this host.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The line also carries the batched form (4,096 uneven matrices: one persistent
+hipGraph launch vs a per-matrix loop, BF16 vs FP32 values) and the serving
+loop (req/s within a p99 target, batched FP32 / BF16 vs the plain per-call path).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+                       [--no-serve]
 """
 from __future__ import annotations
 
@@ -51,14 +56,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="approximate CPU time budget of the baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-form section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--serve-slo-ms", type=float, default=5.0,
+                    help="p99 latency target at which served req/s are compared")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy B (256 MB) once: a kernel of known traffic that calibrates "
                          "rocprofv3 byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark runs on one GPU only; the sparse kernels are "
-                                   "measured at their full single-GPU shape (see DESIGN.md for "
-                                   "how rows would shard across GPUs)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the sparse kernels are "
+                                   "measured at their full single-GPU shape; a multi-GPU run "
+                                   "would shard rows by nnz and is not part of this build",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
@@ -148,6 +159,18 @@ def main():
                          f"0..{rows} of A ({q1} nnz) x the full 1M x 64 B; median of {reps} "
                          f"reps, {tmed * 1e3:.1f} ms each"}
 
+    batched = None
+    if not args.no_batch:
+        del C, X, out
+        torch.cuda.empty_cache()
+        batched = bench_batched(dev, args.batch_count)
+        print("# batched done", file=sys.stderr, flush=True)
+    served = None
+    if not args.no_serve:
+        torch.cuda.empty_cache()
+        served = bench_serving(dev, args.serve_slo_ms, [A])
+        print("# serving done", file=sys.stderr, flush=True)
+
     line = {
         "metric": "SpMM/SpMV HBM GB/s (% of peak); served req/s at p99, batched vs plain",
         "value": round(value, 2),
@@ -161,7 +184,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded numpy PCG64; Zipf(1.5) rows, uniform columns, N(0,1))",
-        "config": {"workload": "config2 SpMM C=A*B: A 1Mx1M CSR FP32 Int32 idx, "
+        "config": {"workload": "SpMM C=A*B: A 1Mx1M CSR FP32 Int32 idx, "
                                f"nnz={nnz}, Zipf(1.5) rows; B 1Mx64 FP32 row-major",
                    "m": M, "n": N, "nnz": nnz, "k": K, "index": "int32"},
         "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
@@ -178,8 +201,133 @@ def main():
                   "gflops": round(2 * nnz * K / (sddmm_ms * 1e-3) / 1e9, 1),
                   "pct_of_hbm_peak": round(100 * gbs(by_sddmm, sddmm_ms) / HBM_PEAK_GBS, 2)},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": served,
     }
     print(json.dumps(line), flush=True)
+
+
+def _events_ms(fn, reps, warmup=1):
+    """Mean event time of fn() on the current stream (ms)."""
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        fn()
+    e.record()
+    torch.cuda.synchronize()
+    return s.elapsed_time(e) / reps
+
+
+def bench_batched(dev, count):
+    """Batched form: `count` independent square CSR matrices (rows log-uniform in
+    [256, 65536], Poisson nnz/row with mean log-uniform in [4, 64], n_rhs in
+    {1, 8, 32, 128}): one autotuned persistent hipGraph launch vs a per-matrix
+    loop, and BF16-stored values (FP32 accumulate) vs FP32."""
+    import torch
+
+    from finchgpu import batch
+
+    t0 = time.time()
+    items = batch.synth_batch(count, seed=0, device=dev)
+    nnz = sum(it.A.nnz for it in items)
+    alg = sum(algorithmic_bytes(it.A.n_outer, it.A.n_inner, it.A.nnz, it.n_rhs, "spmm")
+              for it in items)
+    print(f"# batch generated: {count} matrices, nnz={nnz} in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    plan = batch.BatchPlan(items)
+    plan_ms = _events_ms(plan.run, 5)
+    ref = [it.C.clone() for it in items]
+    loop_ms = _events_ms(lambda: batch.run_loop(items), 3)
+    stats = plan.stats()
+    plan.close()
+    items16 = [batch.BatchItem(it.A.to_values(torch.bfloat16), it.B) for it in items]
+    alg16 = alg - sum(it.A.nnz for it in items) * 2
+    plan16 = batch.BatchPlan(items16)
+    plan16_ms = _events_ms(plan16.run, 5)
+    plan16.close()
+    num = sum(float(torch.linalg.vector_norm((b.C - r).double()) ** 2) for b, r in zip(items16, ref))
+    den = sum(float(torch.linalg.vector_norm(r.double()) ** 2) for r in ref)
+    gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
+    out = {"workload": f"{count} CSR matrices, nnz={nnz}, FP32 values, Int32 idx",
+           "plan_graph_ms": round(plan_ms, 3), "plan_graph_gbs": gbs(alg, plan_ms),
+           "loop_ms": round(loop_ms, 3), "loop_gbs": gbs(alg, loop_ms),
+           "speedup_vs_loop": round(loop_ms / plan_ms, 2),
+           "bf16_plan_ms": round(plan16_ms, 3), "bf16_plan_gbs": gbs(alg16, plan16_ms),
+           "bf16_normwise_err_vs_fp32": float(f"{(num / max(den, 1e-300)) ** 0.5:.3e}"),
+           "algorithmic_bytes": alg, "plan_stats": stats}
+    del items, items16, ref
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_serving(dev, slo_ms, large):
+    """Serving form: open-loop Poisson arrivals, 60 % SpMV / 30 % SpMM (k 16 or
+    64) / 10 % SDDMM (k 32) over 256 operands of the batched size distribution
+    plus four 1M-row Zipf(1.5) matrices (the timed matrix and three row
+    permutations of it), HBM admission budget 200 GB.  For each path, the
+    offered rate is raised until p99 latency exceeds the target; the largest
+    achieved req/s within it is reported."""
+    import numpy as np
+    import torch
+
+    from finchgpu import ops, serve
+
+    def permuted(A, seed):
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed)
+        ptr = A.ptr.to(torch.int64) - A.base
+        lens = ptr[1:] - ptr[:-1]
+        perm = torch.randperm(A.n_outer, device=dev, generator=g)
+        nl = lens[perm]
+        nptr = torch.zeros(A.n_outer + 1, dtype=torch.int64, device=dev)
+        nptr[1:] = torch.cumsum(nl, 0)
+        row_of = torch.repeat_interleave(torch.arange(A.n_outer, device=dev), nl)
+        src = ptr[perm][row_of] + (torch.arange(A.nnz, device=dev) - nptr[row_of])
+        return ops.Compressed((nptr + A.base).to(A.ptr.dtype), A.idx[src].contiguous(),
+                              A.val[src].contiguous(), A.n_outer, A.n_inner, A.base)
+
+    small = serve.serving_workload(dev, n_small=256, n_large=0, seed=0)
+    big = [large[0]] + [permuted(large[0], 100 + j) for j in range(3)]
+    operands32 = small + big
+    operands16 = [A.to_values(torch.bfloat16) for A in operands32]
+    inputs = serve.Inputs(operands32, seed=1)
+    qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 50_000, 100_000, 200_000, 500_000]
+
+    def sweep(ops_, mode):
+        best, points = None, []
+        for q in qps_grid:
+            n = int(min(20_000, max(2_000, q * 0.25)))
+            reqs = serve.make_trace(operands32, inputs, qps=q, n=n, seed=7)
+            s = serve.serve(ops_, reqs, mode).summary()
+            points.append({"offered_qps": q, "req_per_s": round(s["req_per_s"], 1),
+                           "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+                           "launches": s["launches"]})
+            if s["p99_ms"] <= slo_ms and s["rejected"] == 0:
+                if best is None or s["req_per_s"] > best["req_per_s"]:
+                    best = points[-1]
+            else:
+                break
+        return best, points
+
+    res = {}
+    for name, ops_, mode in (("plain_fp32", operands32, "plain"),
+                             ("batched_fp32", operands32, "batched"),
+                             ("batched_bf16", operands16, "batched")):
+        best, points = sweep(ops_, mode)
+        res[name] = {"best": best, "sweep": points}
+        print(f"# serving {name}: {best}", file=sys.stderr, flush=True)
+    base = (res["plain_fp32"]["best"] or {}).get("req_per_s")
+    for name in ("batched_fp32", "batched_bf16"):
+        b = (res[name]["best"] or {}).get("req_per_s")
+        res[name]["x_plain"] = round(b / base, 2) if (b and base) else None
+    res["p99_target_ms"] = slo_ms
+    res["operands"] = "256 batched-distribution CSR + 4 x 1M-row Zipf(1.5), FP32 and BF16 values"
+    return res
 
 
 if __name__ == "__main__":

@@ -35,9 +35,9 @@ CXXFLAGS = [
 ]
 
 
-def _compile(src: str, verbose: bool) -> str:
-    obj = os.path.join(OBJ_DIR, os.path.basename(src) + ".o")
-    cmd = [HIPCC, *CXXFLAGS, "-c", src, "-o", obj]
+def _compile(src: str, verbose: bool, obj_dir: str, defines) -> str:
+    obj = os.path.join(obj_dir, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *CXXFLAGS, *[f"-D{d}" for d in defines], "-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -49,23 +49,31 @@ def _compile(src: str, verbose: bool) -> str:
     return obj
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> str:
+def build(jobs: int | None = None, verbose: bool = False, out: str | None = None,
+          defines=()) -> str:
+    """Compiles every translation unit and links the library.
+
+    `out` and `defines` build a variant elsewhere (tile-shape sweeps, e.g.
+    defines=("FGX_SPMV_RUN=16",)); the default builds the in-tree library.
+    """
     if not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    os.makedirs(OBJ_DIR, exist_ok=True)
+    lib = LIB if out is None else os.path.join(out, "libfinchgpu.so")
+    obj_dir = OBJ_DIR if out is None else os.path.join(out, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
     jobs = jobs or min(8, os.cpu_count() or 4, len(srcs))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: _compile(s, verbose, obj_dir, defines), srcs))
+    tmp = lib + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 def clean() -> None:
@@ -79,7 +87,10 @@ if __name__ == "__main__":
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--clean", action="store_true")
+    ap.add_argument("--out", default=None, help="build a variant library into this directory")
+    ap.add_argument("-D", dest="defines", action="append", default=[],
+                    help="preprocessor definition for a variant build (NAME=VALUE)")
     a = ap.parse_args()
     if a.clean:
         clean()
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, a.out, a.defines))

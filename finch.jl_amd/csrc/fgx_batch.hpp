@@ -47,7 +47,6 @@ struct BatchWork {
 
 constexpr int kBatchThreads = 256;
 constexpr int kSpmvRowGroup = 8;   // lanes per row, row-split SpMV
-constexpr int kTileE = 64;         // positions per lane group, TILES kind
 
 // Lanes per row (row-split) or per position (TILES) for a dense width k:
 // 16-byte column slices, rounded up to a power of two.
@@ -58,10 +57,14 @@ __host__ __device__ inline int group_for_k(int64_t k) {
   return g;
 }
 
+// Positions per lane group of a TILES tile, for G lanes per position.
+__host__ __device__ constexpr int tile_e_for_g(int g) { return g >= 16 ? 64 : (g >= 4 ? 32 : 16); }
+
 // Positions per TILES tile for a width k.
 __host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
   if (k == 1) return kWave * 8;
-  return static_cast<int64_t>(kWave / group_for_k(k)) * kTileE;
+  const int g = group_for_k(k);
+  return static_cast<int64_t>(kWave / g) * tile_e_for_g(g);
 }
 
 }  // namespace fgx

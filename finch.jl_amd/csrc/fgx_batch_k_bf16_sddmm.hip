@@ -1,0 +1,6 @@
+// fgx_batch_k_bf16_sddmm.hip — batch kernel instantiated for bf16_t values with SDDMM units.
+#include "fgx_batch_impl.hpp"
+
+namespace fgx {
+FGX_BATCH_DEFINE(batch_kernel_bf16_sddmm, bf16_t, true)
+}  // namespace fgx

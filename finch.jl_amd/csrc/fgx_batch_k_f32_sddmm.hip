@@ -1,0 +1,6 @@
+// fgx_batch_k_f32_sddmm.hip — batch kernel instantiated for float values with SDDMM units.
+#include "fgx_batch_impl.hpp"
+
+namespace fgx {
+FGX_BATCH_DEFINE(batch_kernel_f32_sddmm, float, true)
+}  // namespace fgx

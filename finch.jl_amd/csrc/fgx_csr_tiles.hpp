@@ -25,6 +25,12 @@ namespace fgx {
 constexpr int kWavesPerBlock = 4;
 constexpr int kWinCap = 512;  // fibre ends cached in LDS per wave
 
+// SpMM tiles stage their coordinates/values in LDS (1) or load them from
+// global memory chunk by chunk (0).
+#ifndef FGX_SPMM_STAGE
+#define FGX_SPMM_STAGE 1
+#endif
+
 // Fibres that a wave tile [q0, qend) touches: r_lo owns q0 (0 for the first
 // tile), r_hi owns qend (the row a carry continues into; m if none).
 template <typename TP>
@@ -102,16 +108,14 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
   const int lane = lane_id();
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  int64_t r_lo, r_hi, pos_lo;
-  FiberWindow<TP> win =
-      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
-  const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
-
   const int64_t a = q0 + static_cast<int64_t>(lane) * V;
   const int64_t b = (a + V < qend) ? a + V : qend;
   const bool active = a < qend || (a == 0 && m > 0);
 
-  // Stream the run's values and coordinates.
+  // Stream the run's values and coordinates first: these loads depend only
+  // on the tile index, so they are in flight while the tile's fibres are
+  // located (the window set-up ends in a fence that would otherwise hold
+  // them back).
   TV vv[V];
   TI jj[V];
   if (active && b - a == V) {
@@ -125,14 +129,21 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
       jj[s] = in ? ld_stream(idx + a + s) : TI{};
     }
   }
-  // Gather x (independent loads, issued together).
-  TA prod[V];
+  // Gather x (independent loads, issued together), then locate the tile's
+  // fibres while the gathers are in flight.
+  // (The loaded values are used only after the set-up, so the set-up does not
+  // wait for them.)
+  TA xg[V];
 #pragma unroll
   for (int s = 0; s < V; s++) {
     const bool in = a + s < b;
-    prod[s] = TA(0);
-    if (in) prod[s] = widen<TA>(vv[s]) * x[static_cast<int64_t>(jj[s]) - base];
+    xg[s] = TA(0);
+    if (in) xg[s] = x[static_cast<int64_t>(jj[s]) - base];
   }
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win =
+      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
+  const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
 
   TA acc = 0;
   bool has_emit = false;
@@ -150,7 +161,7 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
           acc = 0;
           r++;
         }
-        acc += prod[s];
+        acc += widen<TA>(vv[s]) * xg[s];
       }
     }
     while (r < m && r <= r_win_last && win.end_of(r) <= b) {
@@ -261,7 +272,8 @@ __device__ __forceinline__ void st_dense_nt(TA* __restrict__ p, int valid_cols, 
   st_dense<KV, TA>(p, valid_cols, o);
 }
 
-template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U,
+          bool STAGE = (FGX_SPMM_STAGE != 0)>
 __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                           const TV* __restrict__ val, int64_t base, int64_t m,
                                           int64_t nnz, int64_t k, int64_t kofs,
@@ -269,7 +281,8 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
                                           TA* __restrict__ C, int64_t ldc, int64_t tile,
                                           const int64_t* __restrict__ table, int64_t* lds_win,
                                           int64_t* carry_row_out,
-                                          TA* carry_val_out /* G*KV entries */) {
+                                          TA* carry_val_out /* G*KV entries */,
+                                          TI* lds_j = nullptr, TA* lds_v = nullptr) {
   constexpr int NG = kWave / G;
   constexpr int W = NG * E;
   static_assert(E % G == 0, "E must be a multiple of G");
@@ -280,10 +293,34 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
   const int gbase = g * G;
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  // STAGE: the tile's coordinates and values, loaded coalesced before the
+  // fibre set-up (so the loads overlap it) and parked in LDS afterwards; the
+  // gather loop then reads them as LDS broadcasts, with no global round trip.
+  constexpr int NS = STAGE ? W / kWave : 1;
+  TI sj[NS];
+  TA sv[NS];
+  if constexpr (STAGE) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const int64_t q = q0 + s * kWave + lane;
+      sj[s] = q < qend ? ld_stream(idx + q) : static_cast<TI>(base);
+      sv[s] = q < qend ? widen<TA>(ld_stream(val + q)) : TA(0);
+    }
+  }
   int64_t r_lo, r_hi, pos_lo;
   FiberWindow<TP> win =
       tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
   const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
+  if constexpr (STAGE) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      lds_j[s * kWave + lane] = sj[s];
+      lds_v[s * kWave + lane] = sv[s];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
 
   const int64_t col0 = kofs + static_cast<int64_t>(gl) * KV;
   const int valid_cols = static_cast<int>(col0 < k ? (k - col0 < KV ? k - col0 : KV) : 0);
@@ -315,23 +352,20 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
     for (int c = 0; c < KV; c++) acc[c] = 0;
   };
 
-  // Chunks of G positions: lane gl of the group loads position a+c0+gl.
+  if constexpr (STAGE) {
+    // Batches of U positions; the group's lanes read each position's
+    // coordinate and value from LDS (broadcast) and gather their B slices.
+    const TI* gj = lds_j + g * E;
+    const TA* gv = lds_v + g * E;
 #pragma unroll 1
-  for (int c0 = 0; c0 < E; c0 += G) {
-    const int64_t qc = a + c0 + gl;
-    const bool inc = qc < b;
-    const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + qc)) - base : 0;
-    const TA vc = inc ? widen<TA>(ld_stream(val + qc)) : TA(0);
-#pragma unroll
-    for (int e0 = 0; e0 < G; e0 += U) {
+    for (int e0 = 0; e0 < E; e0 += U) {
       TA bv[U][KV];
       TA ve[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int src = gbase + ((e0 + u) % G);
-        const int64_t je = shfl(jc, src);
-        ve[u] = shfl(vc, src);
-        const bool in = (e0 + u < G) && (a + c0 + e0 + u < b) && valid_cols > 0;
+        const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
+        ve[u] = gv[e0 + u];
+        const bool in = (a + e0 + u < b) && valid_cols > 0;
         if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
         else {
 #pragma unroll
@@ -340,11 +374,46 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
       }
 #pragma unroll
       for (int u = 0; u < U; u++) {
-        const int64_t q = a + c0 + e0 + u;
-        if (e0 + u < G && q < b) {
+        const int64_t q = a + e0 + u;
+        if (q < b) {
           while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
 #pragma unroll
           for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
+        }
+      }
+    }
+  } else {
+    // Chunks of G positions: lane gl of the group loads position a+c0+gl.
+#pragma unroll 1
+    for (int c0 = 0; c0 < E; c0 += G) {
+      const int64_t qc = a + c0 + gl;
+      const bool inc = qc < b;
+      const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + qc)) - base : 0;
+      const TA vc = inc ? widen<TA>(ld_stream(val + qc)) : TA(0);
+#pragma unroll
+      for (int e0 = 0; e0 < G; e0 += U) {
+        TA bv[U][KV];
+        TA ve[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int src = gbase + ((e0 + u) % G);
+          const int64_t je = shfl(jc, src);
+          ve[u] = shfl(vc, src);
+          const bool in = (e0 + u < G) && (a + c0 + e0 + u < b) && valid_cols > 0;
+          if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
+          else {
+#pragma unroll
+            for (int c = 0; c < KV; c++) bv[u][c] = 0;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int64_t q = a + c0 + e0 + u;
+          if (e0 + u < G && q < b) {
+            while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
+#pragma unroll
+            for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
+          }
         }
       }
     }

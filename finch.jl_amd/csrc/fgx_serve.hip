@@ -178,9 +178,8 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
     if (i == 0) combo = ic;
     if (ic != combo || operands[i].val_type != operands[0].val_type)
       return set_error(FGX_EUNSUP, "fgx_serve_run: operands must share index and value types");
-    if (operands[i].val_type != FGX_F32 && operands[i].val_type != FGX_BF16 &&
-        operands[i].val_type != FGX_F16)
-      return set_error(FGX_EUNSUP, "fgx_serve_run: values must be F32, BF16 or F16");
+    if (operands[i].val_type != FGX_F32 && operands[i].val_type != FGX_BF16)
+      return set_error(FGX_EUNSUP, "fgx_serve_run: values must be F32 or BF16");
   }
   for (int64_t r = 0; r < n_reqs; r++) {
     const fgx_request& q = reqs[r];

@@ -6,10 +6,26 @@
 #pragma once
 #include "fgx_launch.hpp"
 
+#ifndef FGX_SPMM_E
+#define FGX_SPMM_E 64
+#endif
+#ifndef FGX_SPMM_U
+#define FGX_SPMM_U 8
+#endif
+// Minimum waves per SIMD the register allocator is held to (0: no hint).
+#ifndef FGX_SPMM_WPE
+#define FGX_SPMM_WPE 0
+#endif
+#if FGX_SPMM_WPE > 0
+#define FGX_SPMM_OCC __attribute__((amdgpu_waves_per_eu(FGX_SPMM_WPE)))
+#else
+#define FGX_SPMM_OCC
+#endif
+
 namespace fgx {
 
 template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
-__global__ __launch_bounds__(256) void spmm_csr_kernel(
+__global__ __launch_bounds__(256) FGX_SPMM_OCC void spmm_csr_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, int64_t k, int64_t kofs, const TA* __restrict__ B,
     int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, const int64_t* __restrict__ table,
@@ -18,9 +34,18 @@ __global__ __launch_bounds__(256) void spmm_csr_kernel(
   const int w = threadIdx.x / kWave;
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
+#if FGX_SPMM_STAGE
+  constexpr int W = (kWave / G) * E;
+  __shared__ TI lds_j[kWavesPerBlock][W];
+  __shared__ TA lds_v[kWavesPerBlock][W];
+  spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
+                                         tile, table, lds[w], carry_row + tile,
+                                         carry_val + tile * (G * KV), lds_j[w], lds_v[w]);
+#else
   spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
                                          tile, table, lds[w], carry_row + tile,
                                          carry_val + tile * (G * KV));
+#endif
 }
 
 // General-layout SpMM (any strides): one wave per row, lanes over columns.
@@ -49,8 +74,10 @@ template <typename TP, typename TI, typename TV, typename TA, int G, int KV>
 static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t kofs,
                        const TA* B, int64_t ldb, TA* C, int64_t ldc, hipStream_t s) {
   constexpr int NG = kWave / G;
-  constexpr int E = (G >= 16) ? 64 : (G >= 4 ? 32 : 16);
-  constexpr int U = G < 8 ? G : 8;
+  // Positions per lane group (E) and B rows per gather batch (U); wide
+  // groups take FGX_SPMM_E / FGX_SPMM_U.
+  constexpr int E = (G >= 16) ? FGX_SPMM_E : (G >= 4 ? 32 : 16);
+  constexpr int U = G < 8 ? G : (FGX_SPMM_U < G ? FGX_SPMM_U : G);
   constexpr int64_t W = static_cast<int64_t>(NG) * E;
   const int64_t m = A->n_outer;
   const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;

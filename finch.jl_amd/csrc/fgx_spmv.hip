@@ -8,9 +8,13 @@
 
 namespace fgx {
 
-// Positions per lane run for SpMV: 32 bytes of values per lane.
-template <typename TV> struct spmv_v { static constexpr int value = 8; };
-template <> struct spmv_v<double> { static constexpr int value = 4; };
+// Positions per lane run for SpMV (FGX_SPMV_RUN for 4-byte and smaller
+// values; half of it for FP64).
+#ifndef FGX_SPMV_RUN
+#define FGX_SPMV_RUN 8
+#endif
+template <typename TV> struct spmv_v { static constexpr int value = FGX_SPMV_RUN; };
+template <> struct spmv_v<double> { static constexpr int value = FGX_SPMV_RUN / 2; };
 
 template <typename TP, typename TI, typename TV, typename TA, int V>
 __global__ __launch_bounds__(256) void spmv_csr_kernel(

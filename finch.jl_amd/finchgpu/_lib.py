@@ -10,7 +10,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfinchgpu.so")
+# FINCHGPU_LIB selects another build of the library (e.g. a tile-shape variant).
+LIB_PATH = os.environ.get("FINCHGPU_LIB") or os.path.join(_HERE, "libfinchgpu.so")
 
 # fgx_status
 OK, EDIM, EPROTO, EUNSUP, EOOM, EADMIT, EHIP, EINVAL = range(8)

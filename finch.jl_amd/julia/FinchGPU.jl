@@ -1,0 +1,200 @@
+# FinchGPU.jl — Julia side of the MI355X backend: the binding Finch.jl would
+# load to reach libfinchgpu.so.  It mirrors finchgpu.h one to one and plugs
+# into Finch at the three points the backend replaces:
+#
+#   * a device, `GPU <: Finch.AbstractDevice` (src/architecture.jl:6-64,170-173),
+#     with `transfer` of level buffers (src/architecture.jl:163,269-294);
+#   * device-resident level buffers, `GPUVector{T} <: AbstractVector{T}`, used
+#     as the Ptr/Idx/Val parameters of SparseList/Element levels
+#     (src/tensors/levels/sparse_list_levels.jl:37-42, element_levels.jl:20-22);
+#   * routing of the recognised programs before code generation: the
+#     `spmv!`/`spmm!`/`sddmm!`/`ewise`/`spgemm` methods below are what the
+#     staged `execute` (src/execute.jl:101-126) and `LogicExecutor`
+#     (src/scheduler/LogicExecutor.jl:86-96) call when a program's tensors
+#     carry `GPUVector` buffers.
+#
+# Julia is not part of this build's toolchain, so this file is the reference
+# binding, exercised only where Julia is installed; the same C ABI is
+# exercised from Python (finchgpu/_lib.py) by the tests.
+module FinchGPU
+
+using Finch
+using Finch: AbstractDevice, DenseLevel, SparseListLevel, ElementLevel, Tensor, SwizzleArray
+
+const libfgx = joinpath(@__DIR__, "..", "finchgpu", "libfinchgpu.so")
+const libhip = "libamdhip64"
+
+# ---- C structs (finchgpu.h) -------------------------------------------------------
+struct FgxCompressed
+    n_outer::Int64
+    n_inner::Int64
+    nnz::Int64
+    ptr_type::Int32
+    idx_type::Int32
+    val_type::Int32
+    index_base::Int32
+    ptr::Ptr{Cvoid}
+    idx::Ptr{Cvoid}
+    val::Ptr{Cvoid}
+end
+
+const FGX_OK, FGX_EDIM, FGX_EPROTO, FGX_EUNSUP, FGX_EOOM, FGX_EADMIT, FGX_EHIP, FGX_EINVAL = 0:7
+const FGX_F64, FGX_F32, FGX_BF16, FGX_F16, FGX_I32, FGX_I64 = Int32.(0:5)
+const FGX_ROW_MAJOR, FGX_COL_MAJOR = Int32(0), Int32(1)
+
+fgx_dtype(::Type{Float64}) = FGX_F64
+fgx_dtype(::Type{Float32}) = FGX_F32
+fgx_dtype(::Type{Int32}) = FGX_I32
+fgx_dtype(::Type{Int64}) = FGX_I64
+
+struct FinchGPUError <: Exception
+    code::Int
+    msg::String
+end
+
+# fgx_status -> the exception Finch itself raises for the same condition.
+function check(code)
+    code == FGX_OK && return nothing
+    msg = unsafe_string(ccall((:fgx_last_error, libfgx), Cstring, ()))
+    code == FGX_EDIM && throw(DimensionMismatch(msg))
+    code == FGX_EPROTO && throw(Finch.FinchProtocolError(msg))
+    throw(FinchGPUError(code, msg))
+end
+
+# ---- device and buffers ---------------------------------------------------------------
+struct GPU <: AbstractDevice
+    id::Int
+    ctx::Ptr{Cvoid}
+end
+
+function GPU(id::Integer = 0)
+    ctx = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fgx_ctx_create, libfgx), Cint, (Cint, Ref{Ptr{Cvoid}}), id, ctx))
+    GPU(id, ctx[])
+end
+
+Finch.get_num_tasks(::GPU) = 1
+
+"""A device vector: the storage of a level buffer on the GPU (hipMalloc'd)."""
+mutable struct GPUVector{T} <: AbstractVector{T}
+    ptr::Ptr{T}
+    len::Int
+    function GPUVector{T}(len::Integer) where {T}
+        p = Ref{Ptr{Cvoid}}(C_NULL)
+        ccall((:hipMalloc, libhip), Cint, (Ref{Ptr{Cvoid}}, Csize_t), p, max(len, 1) * sizeof(T)) == 0 ||
+            throw(OutOfMemoryError())
+        v = new{T}(Ptr{T}(p[]), len)
+        finalizer(x -> ccall((:hipFree, libhip), Cint, (Ptr{Cvoid},), x.ptr), v)
+        v
+    end
+end
+Base.size(v::GPUVector) = (v.len,)
+Base.getindex(::GPUVector, ::Int) = error("GPUVector: element access on the host is not supported")
+Base.pointer(v::GPUVector) = v.ptr
+function Base.resize!(v::GPUVector{T}, n::Integer) where {T}
+    n <= v.len && (v.len = n; return v)
+    w = GPUVector{T}(n)
+    ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), w.ptr, v.ptr, v.len * sizeof(T), 3)
+    v.ptr, v.len = w.ptr, n
+    return v
+end
+
+function Finch.transfer(::GPU, v::Vector{T}) where {T}
+    w = GPUVector{T}(length(v))
+    ccall((:hipMemcpy, libhip), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Csize_t, Cint), w.ptr, v, length(v) * sizeof(T), 1)
+    w
+end
+
+# ---- operand descriptors ----------------------------------------------------------------
+# A Dense(SparseList(Element)) tensor with GPUVector buffers; swizzle(A, 2, 1)
+# is the same descriptor read row-wise.
+function descriptor(A::Tensor)
+    dense = A.lvl
+    sl = dense.lvl
+    el = sl.lvl
+    FgxCompressed(dense.shape, sl.shape, length(sl.idx), fgx_dtype(eltype(sl.ptr)),
+                  fgx_dtype(eltype(sl.idx)), fgx_dtype(eltype(el.val)), 1,
+                  pointer(sl.ptr), pointer(sl.idx), pointer(el.val))
+end
+
+# ---- routed programs ------------------------------------------------------------------------
+# y .= 0; for j = _, i = _; y[i] += A[i, j] * x[j]   (test/reference64/typical/typical_spmv_csc.txt)
+function spmv!(dev::GPU, y::GPUVector, A::Tensor, x::GPUVector)
+    d = descriptor(A)
+    check(ccall((:fgx_spmv_csc, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, d, x.ptr, y.ptr, C_NULL))
+    y
+end
+
+# y .= 0; for i = _, j = _; y[i] += A[j, i] * x[j] over swizzle(A, 2, 1)
+function spmv!(dev::GPU, y::GPUVector, S::SwizzleArray{(2, 1)}, x::GPUVector)
+    d = descriptor(S.body)
+    check(ccall((:fgx_spmv_csr, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, d, x.ptr, y.ptr, C_NULL))
+    y
+end
+
+# @einsum C[i, k] += A[i, j] * B[j, k]; B, C column-major (Finch's Dense(Dense)).
+function spmm!(dev::GPU, C::GPUVector, S::SwizzleArray{(2, 1)}, B::GPUVector, k::Integer)
+    d = descriptor(S.body)
+    check(ccall((:fgx_spmm_csr, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Int64, Ptr{Cvoid}, Int64, Cint, Ptr{Cvoid}, Int64,
+                 Cint, Ptr{Cvoid}),
+                dev.ctx, d, k, B.ptr, d.n_inner, FGX_COL_MAJOR, C.ptr, d.n_outer, FGX_COL_MAJOR, C_NULL))
+    C
+end
+
+# compute(lazy(M) .* (lazy(X) * lazy(Y))): X m x k, Y k x n (column-major,
+# i.e. Y^T row-major with leading dimension k).
+function sddmm!(dev::GPU, out::GPUVector, S::SwizzleArray{(2, 1)}, X::GPUVector, Y::GPUVector, k::Integer)
+    d = descriptor(S.body)
+    check(ccall((:fgx_sddmm_csr, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Int64, Ptr{Cvoid}, Int64, Cint, Ptr{Cvoid}, Int64, Cint,
+                 Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, d, k, X.ptr, d.n_outer, FGX_COL_MAJOR, Y.ptr, k, FGX_ROW_MAJOR, out.ptr, C_NULL))
+    out
+end
+
+# Two-phase sparse outputs: symbolic -> resize! (freeze) -> numeric, as the
+# SparseList level's assemble/freeze protocol (sparse_list_levels.jl:301-327).
+function ewise(dev::GPU, op::Symbol, A::Tensor, B::Tensor)
+    opc = Dict(:* => 0, :+ => 1, :max => 2)[op]
+    da, db = descriptor(A), descriptor(B)
+    ptr = GPUVector{eltype(A.lvl.lvl.ptr)}(da.n_outer + 1)
+    nnz = Ref{Int64}(0)
+    check(ccall((:fgx_ewise_symbolic, libfgx), Cint,
+                (Ptr{Cvoid}, Cint, Ref{FgxCompressed}, Ref{FgxCompressed}, Ptr{Cvoid}, Ref{Int64}, Ptr{Cvoid}),
+                dev.ctx, opc, da, db, ptr.ptr, nnz, C_NULL))
+    idx = GPUVector{eltype(A.lvl.lvl.idx)}(nnz[])
+    val = GPUVector{eltype(A.lvl.lvl.lvl.val)}(nnz[])
+    check(ccall((:fgx_ewise_numeric, libfgx), Cint,
+                (Ptr{Cvoid}, Cint, Ref{FgxCompressed}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}),
+                dev.ctx, opc, da, db, ptr.ptr, idx.ptr, val.ptr, C_NULL))
+    Tensor(DenseLevel(SparseListLevel(ElementLevel{zero(eltype(val))}(val), A.lvl.lvl.shape, ptr, idx),
+                      A.lvl.shape))
+end
+
+# Gustavson C = A * B (column-compressed operands: C = (B^T A^T)^T, the
+# reference's column order, docs/examples/spgemm.jl:59-76).
+function spgemm(dev::GPU, A::Tensor, B::Tensor; semiring::Symbol = :plus_times)
+    sr = semiring === :min_plus ? 1 : 0
+    db, da = descriptor(B), descriptor(A)
+    ptr = GPUVector{eltype(B.lvl.lvl.ptr)}(db.n_outer + 1)
+    nnz = Ref{Int64}(0)
+    check(ccall((:fgx_spgemm_symbolic, libfgx), Cint,
+                (Ptr{Cvoid}, Cint, Ref{FgxCompressed}, Ref{FgxCompressed}, Ptr{Cvoid}, Ref{Int64}, Ptr{Cvoid}),
+                dev.ctx, sr, db, da, ptr.ptr, nnz, C_NULL))
+    idx = GPUVector{eltype(A.lvl.lvl.idx)}(nnz[])
+    val = GPUVector{eltype(A.lvl.lvl.lvl.val)}(nnz[])
+    check(ccall((:fgx_spgemm_numeric, libfgx), Cint,
+                (Ptr{Cvoid}, Cint, Ref{FgxCompressed}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}),
+                dev.ctx, sr, db, da, ptr.ptr, idx.ptr, val.ptr, C_NULL))
+    fill = semiring === :min_plus ? typemax(eltype(val)) : zero(eltype(val))
+    Tensor(DenseLevel(SparseListLevel(ElementLevel{fill}(val), A.lvl.lvl.shape, ptr, idx), B.lvl.shape))
+end
+
+end # module

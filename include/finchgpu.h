@@ -187,7 +187,7 @@ int fgx_convert(fgx_ctx* ctx, const void* src, int src_type, void* dst, int dst_
 /* ---- batched form ------------------------------------------------------- */
 
 /* One item of a batch: C = A*B with B n_inner x n_rhs and C n_outer x n_rhs,
- * both row-major and dense F32 (A's values F32, BF16 or F16). */
+ * both row-major and dense F32 (A's values F32 or BF16). */
 typedef struct fgx_batch_item {
   fgx_compressed A;
   int64_t n_rhs;

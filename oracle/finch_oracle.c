@@ -136,7 +136,7 @@ DEF_SPMM_CSR(float, f32)
  * (benchmark/benchmarks.jl:141-168) is the @finch nest
  *   for i, j (stored in A), k: C[i,j] += A[i,j] * X[i,k] * Y[j,k].
  * There is no golden for it; this restatement sums the dot over k ascending
- * and scales once (the order the survey of the fused plan leaves open), and
+ * and scales once (Finch's fused plan fixes no order a test can observe), and
  * is pinned by the closed form A o (X Y^T) in FP64.  X is n_outer x k and Y
  * n_inner x k, row-major.  Serial: Finch's SparseList output is not
  * concurrent (src/tensors/levels/sparse_list_levels.jl:185-188).

@@ -88,7 +88,7 @@ def test_ewise_edges(dev):
             assert np.array_equal(cval, rval)
 
 
-def test_ewise_config3_size(dev):
+def test_ewise_200k_poisson(dev):
     # 200k x 200k, Poisson(8) and Poisson(24) rows, FP32 values.
     m = n = 200_000
     (ap, ai, av), (bp, bi, bv) = _pair(0, m, n, 8.0, 24.0, np.float32)
@@ -133,8 +133,8 @@ def test_spgemm_finch_fixtures(dev, name):
 
 
 @pytest.mark.parametrize("itype", [torch.int32, torch.int64])
-def test_spgemm_config3_int32_and_min_plus(dev, itype):
-    # Config 3: 200k x 200k, Poisson(8) x Poisson(24) rows; Int32 U{1..9}
+def test_spgemm_200k_poisson_int32_and_min_plus(dev, itype):
+    # 200k x 200k, Poisson(8) x Poisson(24) rows; Int32 U{1..9}
     # plus-times and FP32 min-plus, both bit-exact.
     m = 200_000
     rng = np.random.default_rng(0)

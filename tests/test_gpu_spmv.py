@@ -71,11 +71,11 @@ def test_zero_based_buffers(dev, goldens):
         assert ops.spmv_csc(A, e).cpu().tolist() == dense[:, j].tolist()
 
 
-# ---- config 1: 10k x 10k FP64, density 1e-3, CSC and swizzled CSR ----------------
+# ---- 10k x 10k FP64 at density 1e-3 (fsprand), CSC and swizzled CSR -------------
 
 @pytest.mark.parametrize("itype,ptype", IDX)
 @pytest.mark.parametrize("base", [1, 0])
-def test_config1_fp64(dev, itype, ptype, base):
+def test_fsprand_10k_fp64(dev, itype, ptype, base):
     n = 10_000
     ptr, idx, val = synth.fsprand(n, n, 1e-3, seed=0)
     rng = np.random.default_rng(0)
@@ -207,16 +207,16 @@ def test_sddmm_fp64(dev):
     assert rel_err(out, ref) <= FP64_REL
 
 
-# ---- config 2 at full size (1M x 1M, ~16M nnz, Zipf(1.5) rows) ----------------------------
+# ---- 1M x 1M, ~16M nnz, Zipf(1.5) rows, at full size --------------------------------------
 
 @pytest.fixture(scope="module")
-def config2():
+def powerlaw_1m():
     ptr, idx, val = synth.powerlaw_csr(1_000_000, 1_000_000, 16_000_000, seed=0)
     return ptr, idx, val
 
 
-def test_config2_spmv_spmm_sddmm(dev, config2):
-    ptr, idx, val = config2
+def test_powerlaw_1m_spmv_spmm_sddmm(dev, powerlaw_1m):
+    ptr, idx, val = powerlaw_1m
     m = n = 1_000_000
     A = operand(ptr, idx, val, m, n, dev, torch.int32)
     nth = max(1, min(16, oracle.max_threads()))
@@ -236,8 +236,8 @@ def test_config2_spmv_spmm_sddmm(dev, config2):
     assert rel_err(out, refS) <= FP32_NORMWISE
 
 
-def test_bf16_values_fp32_accumulate(dev, config2):
-    ptr, idx, val = config2
+def test_bf16_values_fp32_accumulate(dev, powerlaw_1m):
+    ptr, idx, val = powerlaw_1m
     n = 1_000_000
     A = operand(ptr, idx, val, n, n, dev, torch.int32)
     Ab = A.to_values(torch.bfloat16)
