@@ -48,6 +48,20 @@ def algorithmic_bytes(m, n, nnz, k, kind, sv=4, si=4, sp=4, sd=4):
     raise ValueError(kind)
 
 
+def measured_traffic():
+    """HBM bytes per SpMM launch, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy of known size); None if absent."""
+    path = os.path.join(HERE, "profiles", "spmm_csr_powerlaw_1m_k64.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,7 +204,9 @@ def main():
         "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": measured_traffic(),
+                     "traffic_source": "profiles/spmm_csr_powerlaw_1m_k64.md (rocprofv3 "
+                                       "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel": "spmm_csr_kernel + fixup_kernel (one fgx_spmm_csr call)",
                      "algorithmic_bytes": by_spmm, "launch_ms": round(spmm_ev_ms, 5)},
         "spmv": {"gbs": round(gbs(by_spmv, spmv_ms), 2), "ms": round(spmv_ms, 5),

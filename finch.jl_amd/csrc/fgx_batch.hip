@@ -12,7 +12,7 @@ namespace fgx {
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
     const int64_t* __restrict__ carry_row, const float* __restrict__ carry_val) {
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; u < n_fix; u += nw) {
     const BatchWork wk = fix[u];
@@ -55,16 +55,18 @@ __global__ __launch_bounds__(256) void row_stats_kernel(const BatchDesc* __restr
 
 // ---- host ----------------------------------------------------------------------------------
 static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
-  if (p->n_work > 0) {
+  for (int c = 0; c < p->n_cls; c++) {
+    const int cls = p->cls_id[c];
+    const int64_t w0 = p->cls_w0[c], w1 = p->cls_w1[c];
+    if (w1 <= w0) continue;
     int rc;
     if (p->val_type == FGX_F32)
-      rc = p->has_sddmm ? batch_kernel_f32_sddmm(p, s) : batch_kernel_f32(p, s);
+      rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, s) : batch_spmm_f32(p, cls, w0, w1, s);
     else if (p->val_type == FGX_BF16)
-      rc = p->has_sddmm ? batch_kernel_bf16_sddmm(p, s) : batch_kernel_bf16(p, s);
+      rc = cls == kClsSddmm ? batch_sddmm_bf16(p, cls, w0, w1, s) : batch_spmm_bf16(p, cls, w0, w1, s);
     else
       return set_error(FGX_EUNSUP, "batch: value type %d (F32 or BF16)", p->val_type);
     FGX_TRY(rc);
-    FGX_TRY(launch_check("batch_kernel"));
   }
   if (p->n_fix > 0) {
     const int64_t g = std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
@@ -92,6 +94,8 @@ static void free_plan_buffers(fgx_batch_plan* p) {
   if (p->d_fix) (void)hipFree(p->d_fix);
   if (p->d_carry_row) (void)hipFree(p->d_carry_row);
   if (p->d_carry_val) (void)hipFree(p->d_carry_val);
+  if (p->d_tables) (void)hipFree(p->d_tables);
+  p->d_tables = nullptr;
   p->exec = nullptr;
   p->graph = nullptr;
   p->capture_stream = nullptr;
@@ -115,25 +119,40 @@ static int dev_alloc_copy(T** out, const T* src, int64_t n, hipStream_t s) {
   return FGX_OK;
 }
 
-int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const BatchWork* work,
+void order_by_class(fgx_batch_plan* p, const BatchDesc* desc, BatchWork* work, int64_t n) {
+  std::stable_sort(work, work + n, [&](const BatchWork& a, const BatchWork& b) {
+    return unit_class(desc[a.item]) < unit_class(desc[b.item]);
+  });
+  p->n_cls = 0;
+  for (int64_t u = 0; u < n;) {
+    const int cls = unit_class(desc[work[u].item]);
+    int64_t v = u;
+    while (v < n && unit_class(desc[work[v].item]) == cls) v++;
+    p->cls_id[p->n_cls] = cls;
+    p->cls_w0[p->n_cls] = u;
+    p->cls_w1[p->n_cls] = v;
+    p->n_cls++;
+    u = v;
+  }
+}
+
+int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const BatchWork* work_in,
                  int64_t n_work, int64_t carry_tiles, int64_t carry_vals, int num_cus,
                  hipStream_t s) {
+  std::vector<BatchWork> work(work_in, work_in + n_work);
+  order_by_class(p, desc, work.data(), n_work);
   std::vector<BatchWork> fix;
   for (int64_t u = 0; u < n_work; u++)
-    if (work[u].kind == BK_TILES) fix.push_back(work[u]);
+    if (work[static_cast<size_t>(u)].kind == BK_TILES) fix.push_back(work[static_cast<size_t>(u)]);
   FGX_TRY(dev_alloc_copy(&p->d_desc, desc, n_desc, s));
-  FGX_TRY(dev_alloc_copy(&p->d_work, work, n_work, s));
+  FGX_TRY(dev_alloc_copy(&p->d_work, work.data(), n_work, s));
   FGX_TRY(dev_alloc_copy(&p->d_fix, fix.data(), static_cast<int64_t>(fix.size()), s));
   FGX_TRY(dev_alloc_copy<int64_t>(&p->d_carry_row, nullptr, carry_tiles, s));
   FGX_TRY(dev_alloc_copy<float>(&p->d_carry_val, nullptr, carry_vals, s));
   FGX_HIP(hipStreamSynchronize(s));
   p->n_work = n_work;
   p->n_fix = static_cast<int64_t>(fix.size());
-  // A persistent grid: a few blocks per CU, a multiple of the 8 XCDs.
-  const int64_t want = (n_work + kWavesPerBlock - 1) / kWavesPerBlock;
-  int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(num_cus) * 8);
-  if (grid >= 8) grid -= grid % 8;
-  p->grid = std::max<int64_t>(grid, 1);
+  p->num_cus = num_cus;  // the launch sizes its persistent grid from this
   return FGX_OK;
 }
 
@@ -153,7 +172,7 @@ static int grow_device(T** buf, int64_t* cap, int64_t need) {
 }
 
 int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
-                        const BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
+                        BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
                         int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
                         hipStream_t s) {
   FGX_TRY(grow_device(&p->d_desc, &p->cap_desc, nd));
@@ -161,27 +180,19 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   FGX_TRY(grow_device(&p->d_fix, &p->cap_fix, nf));
   FGX_TRY(grow_device(&p->d_carry_row, &p->cap_crow, carry_tiles));
   FGX_TRY(grow_device(&p->d_carry_val, &p->cap_cval, carry_vals));
+  order_by_class(p, h_desc, h_work, nw);
   if (nd > 0) FGX_HIP(hipMemcpyAsync(p->d_desc, h_desc, nd * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
   if (nw > 0) FGX_HIP(hipMemcpyAsync(p->d_work, h_work, nw * sizeof(BatchWork), hipMemcpyHostToDevice, s));
   if (nf > 0) FGX_HIP(hipMemcpyAsync(p->d_fix, h_fix, nf * sizeof(BatchWork), hipMemcpyHostToDevice, s));
   p->n_work = nw;
   p->n_fix = nf;
   p->has_sddmm = has_sddmm;
-  const int64_t want = (nw + kWavesPerBlock - 1) / kWavesPerBlock;
-  int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(num_cus) * 8);
-  if (grid >= 8) grid -= grid % 8;
-  p->grid = std::max<int64_t>(grid, 1);
+  p->num_cus = num_cus;
   return launch_raw(p, s);
 }
 
 void batch_free(fgx_batch_plan* p) { free_plan_buffers(p); }
 
-// Picks row-split vs nnz-split and the work-unit size from the item's shape
-// and row-length statistics.  Row-split keeps one lane group per row with no
-// carries and wins while rows are even; once the longest row is many times
-// the mean (power-law rows), one row would serialise a group for too long
-// and the nnz-balanced tiles take over.  Units aim at ~256 KB of traffic so
-// that thousands of them spread over the device.
 int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out,
                     hipStream_t s) {
   if (n <= 0) return FGX_OK;
@@ -218,10 +229,17 @@ int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* ou
   return rc;
 }
 
+// Picks row-split vs nnz-split and the work-unit size from the item's shape
+// and row-length statistics.  Units aim at ~256 KB of traffic so that
+// thousands of them spread over the device.
 TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
                               int flags) {
   const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
-  bool tiles = max_row > 256 && static_cast<double>(max_row) > 16.0 * std::max(mean, 1.0);
+  (void)mean;
+  // Merge-path tiles keep every wave busy with a fixed number of positions
+  // and stage their coordinates in LDS; the row-split form walks its rows
+  // one lane group at a time with a dependent load chain per row.
+  bool tiles = true;
   if (flags & FGX_BATCH_FORCE_ROWSPLIT) tiles = false;
   if (flags & FGX_BATCH_FORCE_NNZSPLIT) tiles = true;
   const double target = 256.0 * 1024.0;
@@ -321,14 +339,26 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   const int vbytes = elem_size(p->val_type);
   const int ibytes = elem_size(items && count ? items[0].A.idx_type : FGX_I32);
   std::vector<BatchWork> work;
+  std::vector<char> is_tiles(static_cast<size_t>(count), 0);
   int64_t carry_tiles = 0, carry_vals = 0;
+  // Units are no coarser than needed to give every resident wave of the
+  // device work many times over (small plans get one-tile units).
+  int64_t total_tiles = 0;
+  for (int64_t i = 0; i < count; i++) {
+    const BatchDesc& d = desc[static_cast<size_t>(i)];
+    const int64_t W = tile_width_for_k(d.k);
+    total_tiles += d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
+  }
+  const int64_t unit_cap = std::max<int64_t>(
+      1, total_tiles / std::max<int64_t>(1, static_cast<int64_t>(ctx->num_cus) * 256));
   for (int64_t i = 0; i < count; i++) {
     BatchDesc& d = desc[static_cast<size_t>(i)];
     p->total_nnz += d.nnz;
     p->alg_bytes += d.nnz * (vbytes + ibytes) + (d.m + 1) * elem_size(items[i].A.ptr_type) +
                     d.n * d.k * 4 + d.m * d.k * 4;
     if (d.m == 0) continue;
-    const TunePick pk = autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags);
+    TunePick pk = autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags);
+    if (pk.kind == BK_TILES) pk.unit = std::min(pk.unit, unit_cap);
     if (pk.kind == BK_TILES) {
       const int64_t W = tile_width_for_k(d.k);
       const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
@@ -339,6 +369,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       carry_vals += nt * width;
       for (int64_t t = 0; t < nt; t += pk.unit)
         work.push_back(BatchWork{static_cast<int32_t>(i), BK_TILES, t, std::min(nt, t + pk.unit)});
+      is_tiles[static_cast<size_t>(i)] = 1;
       p->n_tiles++;
     } else {
       for (int64_t r = 0; r < d.m; r += pk.unit)
@@ -346,8 +377,35 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       p->n_rows++;
     }
   }
-  int rc = batch_upload(p, desc.data(), count, work.data(), static_cast<int64_t>(work.size()),
-                        carry_tiles, carry_vals, ctx->num_cus, s);
+  // Tile tables of the TILES items, in one allocation.
+  int64_t table_entries = 0;
+  std::vector<int64_t> table_off(static_cast<size_t>(count), -1);
+  for (int64_t i = 0; i < count; i++) {
+    const BatchDesc& d = desc[static_cast<size_t>(i)];
+    if (!is_tiles[static_cast<size_t>(i)]) continue;
+    const int64_t W = tile_width_for_k(d.k);
+    table_off[static_cast<size_t>(i)] = table_entries;
+    table_entries += (d.nnz > 0 ? (d.nnz + W - 1) / W : 1) + 1;
+  }
+  int rc = FGX_OK;
+  if (table_entries > 0) {
+    if (hipMalloc(reinterpret_cast<void**>(&p->d_tables),
+                  static_cast<size_t>(table_entries) * sizeof(int64_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      p->d_tables = nullptr;
+      rc = set_error(FGX_EOOM, "fgx_batch_plan_create: tile tables");
+    }
+    for (int64_t i = 0; i < count && rc == FGX_OK; i++) {
+      const int64_t off = table_off[static_cast<size_t>(i)];
+      if (off < 0) continue;
+      BatchDesc& d = desc[static_cast<size_t>(i)];
+      d.table = p->d_tables + off;
+      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k), s, p->d_tables + off);
+    }
+  }
+  if (rc == FGX_OK)
+    rc = batch_upload(p, desc.data(), count, work.data(), static_cast<int64_t>(work.size()),
+                      carry_tiles, carry_vals, ctx->num_cus, s);
   if (rc != FGX_OK) {
     free_plan_buffers(p);
     delete p;

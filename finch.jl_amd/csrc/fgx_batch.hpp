@@ -4,14 +4,14 @@
 // The host cuts every item into work units of similar cost (bytes moved),
 // in one of three kinds:
 //   ROWS  — rows [begin, end) of a SpMM/SpMV item, one lane group per row
-//           (row-split: cheapest when row lengths are even);
+//           (row-split: no carries, no tile table);
 //   TILES — nnz tiles [begin, end) of a SpMM/SpMV item, merge-path tiles of
-//           fgx_csr_tiles.hpp with carries finished by the fix-up launch
-//           (for skewed rows);
+//           fgx_csr_tiles.hpp with carries finished by the fix-up launch;
 //   SDDMM — 64-position sub-tiles [begin, end) of an SDDMM item.
-// A grid sized to fill the device walks the queue in a fixed stride, so
-// the launch (and its hipGraph) is the same on every replay and results do
-// not depend on timing.
+// Units are grouped by class (SDDMM, SpMV, SpMM per lane-group width) and
+// each class runs as one persistent kernel whose resident grid walks the
+// class's units in a fixed stride, so the launches (and their hipGraph) are
+// the same on every replay and results do not depend on timing.
 #pragma once
 #include "fgx_csr_tiles.hpp"
 #include "fgx_internal.hpp"
@@ -60,6 +60,17 @@ __host__ __device__ inline int group_for_k(int64_t k) {
 // Positions per lane group of a TILES tile, for G lanes per position.
 __host__ __device__ constexpr int tile_e_for_g(int g) { return g >= 16 ? 64 : (g >= 4 ? 32 : 16); }
 
+// Work-unit classes: the batch runs one persistent kernel per class, so each
+// kernel carries the registers and LDS of one unit body.  SpMM classes are
+// the lane-group width G (2..64).
+constexpr int kClsSddmm = 0;
+constexpr int kClsSpmv = 1;
+constexpr int kMaxClasses = 8;
+
+__host__ __device__ inline int unit_class(const BatchDesc& d) {
+  return d.op == BO_SDDMM ? kClsSddmm : (d.k == 1 ? kClsSpmv : group_for_k(d.k));
+}
+
 // Positions per TILES tile for a width k.
 __host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
   if (k == 1) return kWave * 8;
@@ -76,7 +87,13 @@ struct fgx_batch_plan {
   int combo = 0;
   int64_t count = 0;
   int64_t grid = 0;
+  int num_cus = 0;
   int64_t n_work = 0, n_rows = 0, n_tiles = 0, n_sddmm = 0;
+  // Work units sorted by class: class cls_id[c] owns units [cls_w0[c], cls_w1[c]).
+  int n_cls = 0;
+  int cls_id[fgx::kMaxClasses] = {};
+  int64_t cls_w0[fgx::kMaxClasses] = {}, cls_w1[fgx::kMaxClasses] = {};
+  int64_t* d_tables = nullptr;  // tile tables of the plan's TILES items
   int64_t total_nnz = 0;
   int64_t alg_bytes = 0;
   fgx::BatchDesc* d_desc = nullptr;
@@ -104,12 +121,15 @@ TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int v
 // Uploads host descriptor/work arrays into p's (grown) device buffers on
 // stream s and launches; the previous launch on p must have finished.
 int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
-                        const BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
+                        BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
                         int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
                         hipStream_t s);
 void batch_free(fgx_batch_plan* p);
 // Maximum row length of each compressed operand (one block per operand).
 int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out, hipStream_t s);
+// Sorts work units by class (stable, so each class keeps the host's order)
+// and records the class ranges in p.
+void order_by_class(fgx_batch_plan* p, const BatchDesc* desc, BatchWork* work, int64_t n);
 // Builds (but does not launch) a plan's device tables from descriptors and
 // work units already on the host; used by batch plans and the server.
 int batch_launch(fgx_batch_plan* p, hipStream_t s);

@@ -3,6 +3,8 @@
 // and the kernel that walks the work queue.  Instantiated per value type in
 // fgx_batch_k_*.hip.
 #pragma once
+#include <algorithm>
+#include <atomic>
 #include "fgx_batch.hpp"
 #include "fgx_sddmm_core.hpp"
 
@@ -73,8 +75,8 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
 }
 
 // ---- SDDMM units: 64 positions per sub-tile, one per lane ----------------------------------
-// `lds` holds the wave's fibre window (kSdWin entries) followed by the X and
-// Y staging tiles of the matrix-core path.
+// `lds` holds the wave's fibre window (kSdWin entries) followed by the Y
+// staging tile of the matrix-core path.
 template <typename TP, typename TI, typename TV>
 __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int64_t st1,
                                            unsigned char* lds) {
@@ -84,8 +86,7 @@ __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int6
   const int64_t base = d.base;
   const int lane = lane_id();
   int64_t* win_lds = reinterpret_cast<int64_t*>(lds);
-  float(*xs)[kSdPitch] = reinterpret_cast<float(*)[kSdPitch]>(lds + kSdWin * sizeof(int64_t));
-  float(*ys)[kSdPitch] = xs + kWave;
+  float(*ys)[kSdPitch] = reinterpret_cast<float(*)[kSdPitch]>(lds + kSdWin * sizeof(int64_t));
   for (int64_t st = st0; st < st1; st++) {
     const int64_t q0 = st * kWave;
     const int64_t qend = (q0 + kWave < d.nnz) ? q0 + kWave : d.nnz;
@@ -104,7 +105,7 @@ __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int6
     float dot = 0.f;
     if (d.mfma) {
       f32x4 c = {0.f, 0.f, 0.f, 0.f};
-      sddmm_sub_mfma(d.B, d.ldb, d.B2, d.ldc, d.k, in, i, j, xs, ys, c);
+      sddmm_sub_mfma(d.B, d.ldb, d.B2, d.ldc, d.k, in, i, j, ys, c);
       dot = mfma_diag(c);
     } else if (in) {
       const float* xr = d.B + i * d.ldb;
@@ -116,16 +117,26 @@ __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int6
 }
 
 // ---- TILES units ------------------------------------------------------------------------------
+// Positions per TILES tile for G lanes per position.
+template <int G>
+constexpr int tile_w_for_g() { return (kWave / G) * tile_e_for_g(G); }
+
+// The tile's coordinates and values are staged in LDS after the fibre window
+// (`lds` + kWinCap entries), as in the single-operand SpMM kernel.
 template <int G, typename TP, typename TI, typename TV>
 __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
   constexpr int U = G < 8 ? G : 8;
+  constexpr int W = tile_w_for_g<G>();
+  static_assert(sizeof(TI) + sizeof(float) <= 12, "TILES staging holds 12 bytes per position");
+  TI* lj = reinterpret_cast<TI*>(lds + kWinCap);
+  float* lv = reinterpret_cast<float*>(lj + W);
   for (int64_t t = t0; t < t1; t++)
-    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U, false>(
+    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U, true>(
         static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
         static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t,
-        d.table, lds,
-        carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4));
+        d.table, lds, carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4),
+        lj, lv);
 }
 
 template <typename TP, typename TI, typename TV>
@@ -149,72 +160,113 @@ __device__ __forceinline__ int64_t xcd_block() {
   return (b % kXcd) * (nb / kXcd) + b / kXcd;
 }
 
-// Per-wave LDS of the batch kernel: the fibre window of ROWS/TILES units, or
-// (with SDDMM units) the SDDMM window plus X/Y staging tiles.
-constexpr int kLdsPlain = kWinCap * static_cast<int>(sizeof(int64_t));
-constexpr int kLdsSddmm = kSdWin * static_cast<int>(sizeof(int64_t)) + 2 * kWave * kSdPitch * 4;
+// Per-wave LDS of a class's kernel: the fibre window (ROWS/TILES), plus the
+// staged coordinates/values of a TILES tile (12 bytes per position covers
+// 8-byte indices), or the SDDMM window plus X/Y staging tiles.
+template <int CLS>
+constexpr int lds_bytes_for() {
+  return CLS == kClsSddmm
+             ? kSdWin * static_cast<int>(sizeof(int64_t)) + kWave * kSdPitch * 4
+             : (CLS == kClsSpmv ? kWinCap * static_cast<int>(sizeof(int64_t))
+                                : kWinCap * static_cast<int>(sizeof(int64_t)) +
+                                      tile_w_for_g<CLS>() * 12);
+}
 
-template <typename TP, typename TI, typename TV, bool SDDMM>
+// One persistent kernel per unit class (SDDMM, SpMV, SpMM of group width G),
+// over the work units [w0, w1) of that class: each kernel holds the
+// registers and LDS of one unit body only.
+template <typename TP, typename TI, typename TV, int CLS>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
-    const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t n_work,
-    int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
-  constexpr int kWaveLds = SDDMM ? (kLdsSddmm > kLdsPlain ? kLdsSddmm : kLdsPlain) : kLdsPlain;
-  __shared__ __align__(16) unsigned char smem[kWavesPerBlock][kWaveLds];
-  int64_t* lds[kWavesPerBlock];
-#pragma unroll
-  for (int i = 0; i < kWavesPerBlock; i++) lds[i] = reinterpret_cast<int64_t*>(smem[i]);
-  const int w = threadIdx.x / kWave;
+    const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t w0,
+    int64_t w1, int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
+  __shared__ __align__(16) unsigned char smem[kWavesPerBlock][lds_bytes_for<CLS>()];
+  const int w = wave_in_block();
+  int64_t* lds = reinterpret_cast<int64_t*>(smem[w]);
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
-  for (int64_t u = xcd_block() * kWavesPerBlock + w; u < n_work; u += nw) {
+  for (int64_t u = w0 + xcd_block() * kWavesPerBlock + w; u < w1; u += nw) {
     const BatchWork wk = work[u];
     const BatchDesc d = desc[wk.item];
-    if (wk.kind == BK_ROWS) {
-      if (d.k == 1) {
-        rows_spmv<TP, TI, TV>(d, wk.begin, wk.end);
-      } else {
-        switch (group_for_k(d.k)) {
-          case 2: rows_spmm<2, TP, TI, TV>(d, wk.begin, wk.end); break;
-          case 4: rows_spmm<4, TP, TI, TV>(d, wk.begin, wk.end); break;
-          case 8: rows_spmm<8, TP, TI, TV>(d, wk.begin, wk.end); break;
-          case 16: rows_spmm<16, TP, TI, TV>(d, wk.begin, wk.end); break;
-          case 32: rows_spmm<32, TP, TI, TV>(d, wk.begin, wk.end); break;
-          default: rows_spmm<64, TP, TI, TV>(d, wk.begin, wk.end); break;
-        }
-      }
-    } else if (wk.kind == BK_TILES) {
-      if (d.k == 1) {
-        tiles_spmv<TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val);
-      } else {
-        switch (group_for_k(d.k)) {
-          case 2: tiles_spmm<2, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-          case 4: tiles_spmm<4, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-          case 8: tiles_spmm<8, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-          case 16: tiles_spmm<16, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-          case 32: tiles_spmm<32, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-          default: tiles_spmm<64, TP, TI, TV>(d, wk.begin, wk.end, lds[w], carry_row, carry_val); break;
-        }
-      }
-    } else if constexpr (SDDMM) {
+    if constexpr (CLS == kClsSddmm) {
       sddmm_unit<TP, TI, TV>(d, wk.begin, wk.end, smem[w]);
+    } else if constexpr (CLS == kClsSpmv) {
+      if (wk.kind == BK_ROWS) rows_spmv<TP, TI, TV>(d, wk.begin, wk.end);
+      else tiles_spmv<TP, TI, TV>(d, wk.begin, wk.end, lds, carry_row, carry_val);
+    } else {
+      if (wk.kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV>(d, wk.begin, wk.end);
+      else tiles_spmm<CLS, TP, TI, TV>(d, wk.begin, wk.end, lds, carry_row, carry_val);
     }
   }
 }
 
-// Launches batch_kernel<., ., TV, SDDMM> for the plan's index types.
-#define FGX_BATCH_DEFINE(NAME, TYPE, SDDMM_)                                                 \
-  int NAME(const fgx_batch_plan* p, hipStream_t s) {                                         \
-    using TV = TYPE;                                                                         \
-    FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo),                                        \
-                     hipLaunchKernelGGL((batch_kernel<TP, TI, TV, SDDMM_>),                  \
-                                        dim3(static_cast<unsigned>(p->grid)),                \
-                                        dim3(kBatchThreads), 0, s, p->d_desc, p->d_work,     \
-                                        p->n_work, p->d_carry_row, p->d_carry_val));         \
-    return FGX_OK;                                                                           \
+// Persistent grid: as many blocks as can be resident at once (so no block
+// starts after others have drained the queue), a multiple of the 8 XCDs,
+// and no more than the work needs.
+inline int64_t persistent_grid(int64_t n_work, int num_cus, int blocks_per_cu) {
+  const int64_t want = (n_work + kWavesPerBlock - 1) / kWavesPerBlock;
+  int64_t grid = std::min<int64_t>(want, static_cast<int64_t>(num_cus) *
+                                             std::max(1, blocks_per_cu));
+  if (grid >= 8) grid -= grid % 8;
+  return std::max<int64_t>(grid, 1);
+}
+
+template <typename TP, typename TI, typename TV, int CLS>
+int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, hipStream_t s) {
+  static std::atomic<int> occ{0};
+  int o = occ.load();
+  if (o == 0) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<TP, TI, TV, CLS>,
+                                                     kBatchThreads, 0) != hipSuccess ||
+        n < 1) {
+      (void)hipGetLastError();
+      n = 1;
+    }
+    occ.store(n);
+    o = n;
+  }
+  const int64_t grid = persistent_grid(w1 - w0, p->num_cus, o);
+  p->grid = std::max<int64_t>(p->grid, grid);
+  hipLaunchKernelGGL((batch_kernel<TP, TI, TV, CLS>), dim3(static_cast<unsigned>(grid)),
+                     dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, w0, w1, p->d_carry_row,
+                     p->d_carry_val);
+  return launch_check("batch_kernel");
+}
+
+// Launchers of one value type: NAME(p, cls, w0, w1, s) runs class `cls`
+// over work units [w0, w1) for the plan's index types.
+#define FGX_BATCH_DEFINE_SPMM(NAME, TYPE)                                                   \
+  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s) {            \
+    using TV = TYPE;                                                                        \
+    int rc = set_error(FGX_EUNSUP, "batch: unit class %d", cls);                           \
+    FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo), {                                     \
+      switch (cls) {                                                                        \
+        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, s); break;        \
+        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, s); break;                      \
+        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, s); break;                      \
+        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, s); break;                      \
+        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, s); break;                    \
+        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, s); break;                    \
+        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, s); break;                    \
+        default: break;                                                                     \
+      }                                                                                     \
+    });                                                                                     \
+    if (rc == FGX_OK) clear_error();                                                        \
+    return rc;                                                                              \
   }
 
-int batch_kernel_f32(const fgx_batch_plan* p, hipStream_t s);
-int batch_kernel_f32_sddmm(const fgx_batch_plan* p, hipStream_t s);
-int batch_kernel_bf16(const fgx_batch_plan* p, hipStream_t s);
-int batch_kernel_bf16_sddmm(const fgx_batch_plan* p, hipStream_t s);
+#define FGX_BATCH_DEFINE_SDDMM(NAME, TYPE)                                                  \
+  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s) {            \
+    using TV = TYPE;                                                                        \
+    if (cls != kClsSddmm) return set_error(FGX_EUNSUP, "batch: unit class %d", cls);       \
+    int rc = FGX_OK;                                                                        \
+    FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo),                                       \
+                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, s)));             \
+    return rc;                                                                              \
+  }
+
+int batch_spmm_f32(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
+int batch_sddmm_f32(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
+int batch_spmm_bf16(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
+int batch_sddmm_bf16(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
 
 }  // namespace fgx

@@ -2,5 +2,5 @@
 #include "fgx_batch_impl.hpp"
 
 namespace fgx {
-FGX_BATCH_DEFINE(batch_kernel_bf16_sddmm, bf16_t, true)
+FGX_BATCH_DEFINE_SDDMM(batch_sddmm_bf16, bf16_t)
 }  // namespace fgx

@@ -52,13 +52,8 @@ void free_tile_tables(fgx_ctx* ctx) {
   ctx->tables.clear();
 }
 
-int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t** out) {
+int fill_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t* d) {
   const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
-  int64_t* d = nullptr;
-  if (hipMalloc(reinterpret_cast<void**>(&d), (ntiles + 1) * sizeof(int64_t)) != hipSuccess) {
-    (void)hipGetLastError();
-    return set_error(FGX_EOOM, "tile table allocation failed");
-  }
   const unsigned blocks = static_cast<unsigned>((ntiles + 1 + 255) / 256);
   if (A->ptr_type == FGX_I32)
     hipLaunchKernelGGL((tile_table_kernel<int32_t>), dim3(blocks), dim3(256), 0, s,
@@ -68,7 +63,17 @@ int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t*
     hipLaunchKernelGGL((tile_table_kernel<int64_t>), dim3(blocks), dim3(256), 0, s,
                        static_cast<const int64_t*>(A->ptr), static_cast<int64_t>(A->index_base),
                        A->n_outer, A->nnz, W, ntiles, d);
-  int rc = launch_check("tile_table_kernel");
+  return launch_check("tile_table_kernel");
+}
+
+int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t** out) {
+  const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
+  int64_t* d = nullptr;
+  if (hipMalloc(reinterpret_cast<void**>(&d), (ntiles + 1) * sizeof(int64_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(FGX_EOOM, "tile table allocation failed");
+  }
+  const int rc = fill_tile_table(A, W, s, d);
   if (rc != FGX_OK) {
     (void)hipFree(d);
     return rc;

@@ -97,6 +97,13 @@ __device__ __forceinline__ int64_t pos_at(const TP* ptr, int64_t base, int64_t r
   return static_cast<int64_t>(ptr[r]) - base;
 }
 
+// Index of the calling wave within its block, as a wave-uniform (scalar)
+// value: everything derived from it (tile numbers, work units, descriptors)
+// then lives in scalar registers.
+__device__ __forceinline__ int wave_in_block() {
+  return __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / kWave);
+}
+
 // Smallest r in [lo, hi) with pos(r+1) > target, or hi if none — the fiber
 // that owns stored position `target` (skipping empty fibers).  Cooperative
 // over a group of `G` lanes (G = 32 or 64, aligned) that all pass the same

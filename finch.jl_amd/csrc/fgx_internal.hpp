@@ -78,6 +78,8 @@ int tile_table(fgx_ctx* ctx, const fgx_compressed* A, int64_t W, hipStream_t s,
 void free_tile_tables(fgx_ctx* ctx);
 // Builds an uncached tile table; *out is hipMalloc'd and owned by the caller.
 int build_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t** out);
+// Writes the tile table of A for width W (ntiles + 1 entries) into d.
+int fill_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t* d);
 
 // Checks the descriptor (non-null buffers, supported type combination).
 int check_compressed(const fgx_compressed* A, const char* name);

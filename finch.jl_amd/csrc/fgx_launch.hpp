@@ -13,7 +13,7 @@ __global__ __launch_bounds__(256) void fixup_kernel(const int64_t* __restrict__ 
                                                     const TA* __restrict__ carry_val,
                                                     int64_t ntiles, int width, TA* __restrict__ out,
                                                     int64_t ld, int64_t kofs, int64_t k) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave_in_block();
   if (t >= ntiles) return;
   fixup_tile<TA>(carry_row, carry_val, ntiles, t, width, out, ld, kofs, k);
 }

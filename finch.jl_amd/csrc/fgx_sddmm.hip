@@ -17,9 +17,8 @@ __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
     const float* __restrict__ Y, int64_t ldy, float* __restrict__ out, int64_t ntiles,
     const int64_t* __restrict__ table) {
   __shared__ int64_t lds[kWavesPerBlock][kSdWin];
-  __shared__ __align__(16) float xs[kWavesPerBlock][kWave][kSdPitch];
   __shared__ __align__(16) float ys[kWavesPerBlock][kWave][kSdPitch];
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
   constexpr int64_t W = kWave * kSddmmSub;
@@ -41,7 +40,7 @@ __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
       i = win.find(q);
     }
     f32x4 c = {0.f, 0.f, 0.f, 0.f};
-    sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, xs[w], ys[w], c);
+    sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, ys[w], c);
     const float dot = mfma_diag(c);
     if (in) __builtin_nontemporal_store(a * dot, out + q);
     if (q0 + (sub + 1) * kWave >= qend) break;
@@ -55,7 +54,7 @@ __global__ __launch_bounds__(256) void sddmm_valu_kernel(
     int64_t x_cs, const TA* __restrict__ Y, int64_t y_rs, int64_t y_cs, TA* __restrict__ out,
     int64_t ntiles) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
   const int lane = lane_id();

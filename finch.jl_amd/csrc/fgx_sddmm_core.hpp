@@ -20,40 +20,49 @@ constexpr int kSdChunk = 16;     // dense columns staged per pass
 constexpr int kSdPitch = 20;     // LDS row pitch (floats): conflict-free 16-byte reads
 constexpr int kSdWin = 256;      // fibre ends cached per wave
 
-// One 64-position sub-tile: lane t owns position q0+t.  The X rows (of the
-// positions' rows) and Y rows (of their columns) are staged through LDS 16
-// columns at a time with coalesced 16-byte loads (4 lanes per row), then
-// each lane feeds its own X and Y values to the matrix core.
+// One 64-position sub-tile: lane t owns position q0+t.  The Y rows (of the
+// positions' columns: random rows) are staged through LDS 16 columns at a
+// time with coalesced 16-byte loads (4 lanes per row), then each lane reads
+// back its own row's values.  The X rows (of the positions' rows) are read
+// by each lane directly: positions of one row share its address, so a wave
+// touches only the few rows its positions fall in.  Each lane then feeds its
+// X and Y values to the matrix core.
 __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int64_t ldx,
                                                const float* __restrict__ Y, int64_t ldy,
                                                int64_t k, bool in, int64_t i, int64_t j,
-                                               float (*xs)[kSdPitch], float (*ys)[kSdPitch],
-                                               f32x4& c) {
+                                               float (*ys)[kSdPitch], f32x4& c) {
   const int lane = lane_id();
   const int part = lane & 3;
   for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
+    float4 yv[4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
       const int t = (lane >> 2) + 16 * s;
-      const int64_t it = shfl(i, t);
       const int64_t jt = shfl(j, t);
       const int tin = shfl(static_cast<int>(in), t);
       const int64_t col = c0 + 4 * part;
-      float4 xv = make_float4(0.f, 0.f, 0.f, 0.f), yv = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (tin && col < k) {
-        xv = *reinterpret_cast<const float4*>(X + it * ldx + col);
-        yv = *reinterpret_cast<const float4*>(Y + jt * ldy + col);
-      }
-      *reinterpret_cast<float4*>(&xs[t][4 * part]) = xv;
-      *reinterpret_cast<float4*>(&ys[t][4 * part]) = yv;
+      yv[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (tin && col < k) yv[s] = *reinterpret_cast<const float4*>(Y + jt * ldy + col);
+    }
+    float4 xv[kSdChunk / 4];
+#pragma unroll
+    for (int q = 0; q < kSdChunk / 4; q++) {
+      const int64_t col = c0 + 4 * q;
+      xv[q] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (in && col < k) xv[q] = *reinterpret_cast<const float4*>(X + i * ldx + col);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      const int t = (lane >> 2) + 16 * s;
+      *reinterpret_cast<float4*>(&ys[t][4 * part]) = yv[s];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int kk = 0; kk < kSdChunk; kk += 4) {
-      const float4 a4 = *reinterpret_cast<const float4*>(&xs[lane][kk]);
-      const float4 b4 = *reinterpret_cast<const float4*>(&ys[lane][kk]);
+    for (int q = 0; q < kSdChunk / 4; q++) {
+      const float4 a4 = xv[q];
+      const float4 b4 = *reinterpret_cast<const float4*>(&ys[lane][4 * q]);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.x, b4.x, c, 0, 0, 0);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.y, b4.y, c, 0, 0, 0);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.z, b4.z, c, 0, 0, 0);

@@ -31,7 +31,16 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kServeStreams = 4;
-constexpr double kUnitTarget = 256.0 * 1024.0;  // bytes of traffic per work unit
+constexpr double kUnitTarget = 256.0 * 1024.0;  // bytes of traffic per work unit, at most
+// A request is cut into at least this many units (when it has that many
+// tiles), so that a launch holding a few requests still spreads over every
+// wave slot of the device instead of queueing tiles behind a few waves.
+constexpr int64_t kUnitsPerRequest = 4096;
+
+int64_t spread_unit(int64_t unit, int64_t n) {
+  const int64_t fine = (n + kUnitsPerRequest - 1) / kUnitsPerRequest;
+  return std::max<int64_t>(1, std::min(unit, fine));
+}
 
 // First-fit allocator over a device arena, for request outputs the server owns.
 class Arena {
@@ -393,8 +402,10 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
             d.table = oi.table_for(kWave);
             has_sddmm = true;
             const int64_t nsub = (A.nnz + kWave - 1) / kWave;
-            const int64_t per = std::max<int64_t>(
-                1, static_cast<int64_t>(kUnitTarget / (kWave * (8.0 + 8.0 * static_cast<double>(k)))));
+            const int64_t per = spread_unit(
+                std::max<int64_t>(1, static_cast<int64_t>(
+                                         kUnitTarget / (kWave * (8.0 + 8.0 * static_cast<double>(k))))),
+                nsub);
             for (int64_t t = 0; t < nsub; t += per)
               hwork.push_back(BatchWork{item, BK_SDDMM, t, std::min(nsub, t + per)});
           } else {
@@ -409,14 +420,16 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
               d.carry_val0 = carry_vals;
               carry_tiles += nt;
               carry_vals += nt * width;
-              for (int64_t t = 0; t < nt; t += pk.unit) {
-                const BatchWork w{item, BK_TILES, t, std::min(nt, t + pk.unit)};
+              const int64_t unit = spread_unit(pk.unit, nt);
+              for (int64_t t = 0; t < nt; t += unit) {
+                const BatchWork w{item, BK_TILES, t, std::min(nt, t + unit)};
                 hwork.push_back(w);
                 hfix.push_back(w);
               }
             } else if (A.n_outer > 0) {
-              for (int64_t r0 = 0; r0 < A.n_outer; r0 += pk.unit)
-                hwork.push_back(BatchWork{item, BK_ROWS, r0, std::min(A.n_outer, r0 + pk.unit)});
+              const int64_t unit = spread_unit(pk.unit, A.n_outer);
+              for (int64_t r0 = 0; r0 < A.n_outer; r0 += unit)
+                hwork.push_back(BatchWork{item, BK_ROWS, r0, std::min(A.n_outer, r0 + unit)});
             }
           }
           hdesc.push_back(d);

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(256) FGX_SPMM_OCC void spmm_csr_kernel(
     int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, const int64_t* __restrict__ table,
     int64_t* __restrict__ carry_row, TA* __restrict__ carry_val) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
 #if FGX_SPMM_STAGE
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void spmm_csr_strided_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t k, const TA* __restrict__ B, int64_t b_rs, int64_t b_cs,
     TA* __restrict__ C, int64_t c_rs, int64_t c_cs) {
-  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + threadIdx.x / kWave;
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave_in_block();
   if (row >= m) return;
   const int lane = lane_id();
   const int64_t q0 = pos_at(ptr, base, row), q1 = pos_at(ptr, base, row + 1);

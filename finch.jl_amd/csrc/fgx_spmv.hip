@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void spmv_csr_kernel(
     int64_t ntiles, int vec, const int64_t* __restrict__ table, int64_t* __restrict__ carry_row,
     TA* __restrict__ carry_val) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
   spmv_tile<TP, TI, TV, TA, V>(ptr, idx, val, base, m, nnz, x, y, tile, vec != 0, table, lds[w],
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(256) void spmv_csc_kernel(
     int64_t base, int64_t n_outer, int64_t nnz, const TA* __restrict__ x, TA* __restrict__ y,
     int64_t ntiles, int vec) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
-  const int w = threadIdx.x / kWave;
+  const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
   constexpr int W = kWave * V;
