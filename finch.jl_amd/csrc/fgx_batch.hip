@@ -4,26 +4,43 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 namespace fgx {
 
-// Carries of TILES units: one wave per unit, over the unit's tiles in order.
+// Carries of TILES units, finished in tile order.  Persistent form
+// (seg_off == nullptr): one wave per unit, over the unit's tiles.  Segmented
+// form: entries are whole requests' tile ranges, one wave per tile.
+__device__ __forceinline__ void fixup_item_tile(const BatchDesc& d, int64_t t,
+                                                const int64_t* __restrict__ carry_row,
+                                                const float* __restrict__ carry_val) {
+  const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
+  const int64_t W = tile_width_for_k(d.k);
+  const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
+  const int64_t ld = d.k == 1 ? 1 : d.ldc;
+  fixup_tile<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width, d.C,
+                    ld, 0, d.k);
+}
+
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
-    const int64_t* __restrict__ carry_row, const float* __restrict__ carry_val) {
+    const int64_t* __restrict__ seg_off, int64_t total, const int64_t* __restrict__ carry_row,
+    const float* __restrict__ carry_val) {
   const int w = wave_in_block();
+  if (seg_off) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
+    if (t >= total) return;
+    const int64_t u = find_segment(seg_off, n_fix, t);
+    const BatchWork wk = fix[u];
+    fixup_item_tile(desc[wk.item], wk.begin + (t - seg_off[u]), carry_row, carry_val);
+    return;
+  }
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; u < n_fix; u += nw) {
     const BatchWork wk = fix[u];
     const BatchDesc d = desc[wk.item];
-    const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
-    const int64_t W = tile_width_for_k(d.k);
-    const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
-    const int64_t ld = d.k == 1 ? 1 : d.ldc;
-    for (int64_t t = wk.begin; t < wk.end; t++)
-      fixup_tile<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width,
-                        d.C, ld, 0, d.k);
+    for (int64_t t = wk.begin; t < wk.end; t++) fixup_item_tile(d, t, carry_row, carry_val);
   }
 }
 
@@ -59,19 +76,28 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     const int cls = p->cls_id[c];
     const int64_t w0 = p->cls_w0[c], w1 = p->cls_w1[c];
     if (w1 <= w0) continue;
+    const int64_t* seg = p->segmented ? p->d_segoff + p->cls_seg0[c] : nullptr;
+    const int64_t total = p->segmented ? p->cls_total[c] : 0;
+    if (p->segmented && total == 0) continue;
     int rc;
     if (p->val_type == FGX_F32)
-      rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, s) : batch_spmm_f32(p, cls, w0, w1, s);
+      rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, seg, total, s)
+                            : batch_spmm_f32(p, cls, w0, w1, seg, total, s);
     else if (p->val_type == FGX_BF16)
-      rc = cls == kClsSddmm ? batch_sddmm_bf16(p, cls, w0, w1, s) : batch_spmm_bf16(p, cls, w0, w1, s);
+      rc = cls == kClsSddmm ? batch_sddmm_bf16(p, cls, w0, w1, seg, total, s)
+                            : batch_spmm_bf16(p, cls, w0, w1, seg, total, s);
     else
       return set_error(FGX_EUNSUP, "batch: value type %d (F32 or BF16)", p->val_type);
     FGX_TRY(rc);
   }
-  if (p->n_fix > 0) {
-    const int64_t g = std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+  if (p->n_fix > 0 && (!p->segmented || p->fix_total > 0)) {
+    const int64_t g = p->segmented
+                          ? (p->fix_total + kWavesPerBlock - 1) / kWavesPerBlock
+                          : std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
     hipLaunchKernelGGL(batch_fixup_kernel, dim3(static_cast<unsigned>(g)), dim3(kBatchThreads), 0,
-                       s, p->d_desc, p->d_fix, p->n_fix, p->d_carry_row, p->d_carry_val);
+                       s, p->d_desc, p->d_fix, p->n_fix,
+                       p->segmented ? p->d_segoff + p->fix_seg0 : nullptr, p->fix_total,
+                       p->d_carry_row, p->d_carry_val);
     FGX_TRY(launch_check("batch_fixup_kernel"));
   }
   return FGX_OK;
@@ -89,6 +115,18 @@ static void free_plan_buffers(fgx_batch_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->capture_stream) (void)hipStreamDestroy(p->capture_stream);
+  if (p->d_stage) {
+    // Segmented launches: desc/work/fix/seg_off are views into d_stage.
+    (void)hipFree(p->d_stage);
+    p->d_desc = nullptr;
+    p->d_work = nullptr;
+    p->d_fix = nullptr;
+  }
+  if (p->h_stage) (void)hipHostFree(p->h_stage);
+  p->d_stage = nullptr;
+  p->h_stage = nullptr;
+  p->d_segoff = nullptr;
+  p->cap_stage = p->cap_hstage = 0;
   if (p->d_desc) (void)hipFree(p->d_desc);
   if (p->d_work) (void)hipFree(p->d_work);
   if (p->d_fix) (void)hipFree(p->d_fix);
@@ -171,19 +209,98 @@ static int grow_device(T** buf, int64_t* cap, int64_t need) {
   return FGX_OK;
 }
 
+static size_t align16(size_t n) { return (n + 15) & ~static_cast<size_t>(15); }
+
 int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
                         BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
                         int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
                         hipStream_t s) {
-  FGX_TRY(grow_device(&p->d_desc, &p->cap_desc, nd));
-  FGX_TRY(grow_device(&p->d_work, &p->cap_work, nw));
-  FGX_TRY(grow_device(&p->d_fix, &p->cap_fix, nf));
   FGX_TRY(grow_device(&p->d_carry_row, &p->cap_crow, carry_tiles));
   FGX_TRY(grow_device(&p->d_carry_val, &p->cap_cval, carry_vals));
   order_by_class(p, h_desc, h_work, nw);
-  if (nd > 0) FGX_HIP(hipMemcpyAsync(p->d_desc, h_desc, nd * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
-  if (nw > 0) FGX_HIP(hipMemcpyAsync(p->d_work, h_work, nw * sizeof(BatchWork), hipMemcpyHostToDevice, s));
-  if (nf > 0) FGX_HIP(hipMemcpyAsync(p->d_fix, h_fix, nf * sizeof(BatchWork), hipMemcpyHostToDevice, s));
+  p->segmented = true;
+  // Descriptors in entry order (entry u's at u, with its kind), so a wave
+  // reads its descriptor with one load; fix-up entries follow the renumbering.
+  std::vector<BatchDesc> sdesc(static_cast<size_t>(nw));
+  std::vector<int32_t> renum(static_cast<size_t>(nd), -1);
+  for (int64_t u = 0; u < nw; u++) {
+    sdesc[static_cast<size_t>(u)] = h_desc[h_work[u].item];
+    sdesc[static_cast<size_t>(u)].kind = h_work[u].kind;
+    if (h_work[u].begin != 0)
+      return set_error(FGX_EINVAL, "batch: segmented entries must start at 0");
+    renum[static_cast<size_t>(h_work[u].item)] = static_cast<int32_t>(u);
+    h_work[u].item = static_cast<int32_t>(u);
+  }
+  std::vector<BatchWork> sfix(h_fix, h_fix + nf);
+  for (auto& f : sfix) {
+    f.item = renum[static_cast<size_t>(f.item)];
+    if (f.item < 0) return set_error(FGX_EINVAL, "batch: fix-up entry without a work entry");
+  }
+  h_desc = sdesc.data();
+  nd = nw;
+  h_fix = sfix.data();
+  // Per-class prefix counts of the entries' tiles, then the fix-up's.
+  std::vector<int64_t> seg;
+  seg.reserve(static_cast<size_t>(nw + nf + p->n_cls + 1));
+  for (int c = 0; c < p->n_cls; c++) {
+    p->cls_seg0[c] = static_cast<int64_t>(seg.size());
+    int64_t acc = 0;
+    for (int64_t u = p->cls_w0[c]; u < p->cls_w1[c]; u++) {
+      seg.push_back(acc);
+      acc += h_work[u].end - h_work[u].begin;
+    }
+    seg.push_back(acc);
+    p->cls_total[c] = acc;
+  }
+  p->fix_seg0 = static_cast<int64_t>(seg.size());
+  int64_t facc = 0;
+  for (int64_t u = 0; u < nf; u++) {
+    seg.push_back(facc);
+    facc += h_fix[u].end - h_fix[u].begin;
+  }
+  seg.push_back(facc);
+  p->fix_total = facc;
+  // One pinned staging buffer, one copy: [desc | work | fix | seg_off].
+  const size_t o_desc = 0;
+  const size_t o_work = align16(o_desc + static_cast<size_t>(nd) * sizeof(BatchDesc));
+  const size_t o_fix = align16(o_work + static_cast<size_t>(nw) * sizeof(BatchWork));
+  const size_t o_seg = align16(o_fix + static_cast<size_t>(nf) * sizeof(BatchWork));
+  const size_t bytes = o_seg + seg.size() * sizeof(int64_t);
+  if (static_cast<int64_t>(bytes) > p->cap_hstage) {
+    const size_t want = std::max(bytes, static_cast<size_t>(2 * p->cap_hstage) + 4096);
+    if (p->h_stage) (void)hipHostFree(p->h_stage);
+    p->h_stage = nullptr;
+    p->cap_hstage = 0;
+    if (hipHostMalloc(&p->h_stage, want) != hipSuccess) {
+      (void)hipGetLastError();
+      p->h_stage = nullptr;
+      return set_error(FGX_EOOM, "batch: pinned staging allocation failed");
+    }
+    p->cap_hstage = static_cast<int64_t>(want);
+  }
+  if (static_cast<int64_t>(bytes) > p->cap_stage) {
+    const size_t want = std::max(bytes, static_cast<size_t>(2 * p->cap_stage) + 4096);
+    if (p->d_stage) (void)hipFree(p->d_stage);
+    p->d_stage = nullptr;
+    p->cap_stage = 0;
+    if (hipMalloc(&p->d_stage, want) != hipSuccess) {
+      (void)hipGetLastError();
+      p->d_stage = nullptr;
+      return set_error(FGX_EOOM, "batch: staging allocation failed");
+    }
+    p->cap_stage = static_cast<int64_t>(want);
+  }
+  char* h = static_cast<char*>(p->h_stage);
+  if (nd > 0) std::memcpy(h + o_desc, h_desc, static_cast<size_t>(nd) * sizeof(BatchDesc));
+  if (nw > 0) std::memcpy(h + o_work, h_work, static_cast<size_t>(nw) * sizeof(BatchWork));
+  if (nf > 0) std::memcpy(h + o_fix, h_fix, static_cast<size_t>(nf) * sizeof(BatchWork));
+  std::memcpy(h + o_seg, seg.data(), seg.size() * sizeof(int64_t));
+  FGX_HIP(hipMemcpyAsync(p->d_stage, p->h_stage, bytes, hipMemcpyHostToDevice, s));
+  char* d = static_cast<char*>(p->d_stage);
+  p->d_desc = reinterpret_cast<BatchDesc*>(d + o_desc);
+  p->d_work = reinterpret_cast<BatchWork*>(d + o_work);
+  p->d_fix = reinterpret_cast<BatchWork*>(d + o_fix);
+  p->d_segoff = reinterpret_cast<int64_t*>(d + o_seg);
   p->n_work = nw;
   p->n_fix = nf;
   p->has_sddmm = has_sddmm;

@@ -36,6 +36,8 @@ struct BatchDesc {
   int32_t op;
   int32_t vec;     // 16-byte aligned ptr/idx/val runs (SpMV tiles)
   int32_t mfma;    // SDDMM on matrix cores (k % 4 == 0, aligned rows)
+  int32_t kind;    // segmented launches: the BatchKind of this entry
+  int32_t pad;
 };
 
 struct BatchWork {
@@ -88,6 +90,19 @@ struct fgx_batch_plan {
   int64_t count = 0;
   int64_t grid = 0;
   int num_cus = 0;
+  // Segmented launches (the serving loop): each work entry is a request's
+  // whole range and one wave runs each tile, so the hardware interleaves
+  // the blocks of launches on different streams; batch plans use resident
+  // (persistent) grids instead.  The staged arrays live in one device
+  // buffer filled by one copy per launch.
+  bool segmented = false;
+  int64_t* d_segoff = nullptr;                    // view into d_stage
+  int64_t cls_seg0[fgx::kMaxClasses] = {};        // class c's prefix array in d_segoff
+  int64_t cls_total[fgx::kMaxClasses] = {};       // tiles of class c
+  int64_t fix_seg0 = 0, fix_total = 0;            // the fix-up entries' prefix array
+  void* d_stage = nullptr;
+  void* h_stage = nullptr;                        // pinned
+  int64_t cap_stage = 0, cap_hstage = 0;
   int64_t n_work = 0, n_rows = 0, n_tiles = 0, n_sddmm = 0;
   // Work units sorted by class: class cls_id[c] owns units [cls_w0[c], cls_w1[c]).
   int n_cls = 0;

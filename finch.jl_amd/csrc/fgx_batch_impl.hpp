@@ -74,46 +74,18 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
   }
 }
 
-// ---- SDDMM units: 64 positions per sub-tile, one per lane ----------------------------------
+// ---- SDDMM units: wave tiles of kWave * kSddmmSub positions ---------------------------------
 // `lds` holds the wave's fibre window (kSdWin entries) followed by the Y
 // staging tile of the matrix-core path.
 template <typename TP, typename TI, typename TV>
-__device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t st0, int64_t st1,
+__device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t t0, int64_t t1,
                                            unsigned char* lds) {
-  const TP* ptr = static_cast<const TP*>(d.ptr);
-  const TI* idx = static_cast<const TI*>(d.idx);
-  const TV* val = static_cast<const TV*>(d.val);
-  const int64_t base = d.base;
-  const int lane = lane_id();
   int64_t* win_lds = reinterpret_cast<int64_t*>(lds);
   float(*ys)[kSdPitch] = reinterpret_cast<float(*)[kSdPitch]>(lds + kSdWin * sizeof(int64_t));
-  for (int64_t st = st0; st < st1; st++) {
-    const int64_t q0 = st * kWave;
-    const int64_t qend = (q0 + kWave < d.nnz) ? q0 + kWave : d.nnz;
-    int64_t r_lo, r_hi, pos_lo;
-    FiberWindow<TP> win = tile_setup(ptr, base, d.m, q0, qend, d.table, st, win_lds, &r_lo, &r_hi,
-                                     &pos_lo, kSdWin);
-    const int64_t q = q0 + lane;
-    const bool in = q < qend;
-    int64_t i = 0, j = 0;
-    float a = 0.f;
-    if (in) {
-      j = static_cast<int64_t>(ld_stream(idx + q)) - base;
-      a = widen<float>(ld_stream(val + q));
-      i = win.find(q);
-    }
-    float dot = 0.f;
-    if (d.mfma) {
-      f32x4 c = {0.f, 0.f, 0.f, 0.f};
-      sddmm_sub_mfma(d.B, d.ldb, d.B2, d.ldc, d.k, in, i, j, ys, c);
-      dot = mfma_diag(c);
-    } else if (in) {
-      const float* xr = d.B + i * d.ldb;
-      const float* yr = d.B2 + j * d.ldc;
-      for (int64_t kk = 0; kk < d.k; kk++) dot += xr[kk] * yr[kk];
-    }
-    if (in) d.C[q] = a * dot;
-  }
+  for (int64_t t = t0; t < t1; t++)
+    sddmm_tile<TP, TI, TV>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+                           static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, d.B, d.ldb,
+                           d.B2, d.ldc, d.C, t, d.table, win_lds, ys, d.mfma != 0);
 }
 
 // ---- TILES units ------------------------------------------------------------------------------
@@ -172,29 +144,72 @@ constexpr int lds_bytes_for() {
                                       tile_w_for_g<CLS>() * 12);
 }
 
-// One persistent kernel per unit class (SDDMM, SpMV, SpMM of group width G),
-// over the work units [w0, w1) of that class: each kernel holds the
-// registers and LDS of one unit body only.
+// Largest s in [0, n) with seg_off[s] <= t (seg_off nondecreasing,
+// seg_off[0] = 0 <= t): the segment holding tile t.  A 64-ary search, one
+// coalesced probe per step; the result is wave-uniform.
+__device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg_off, int64_t n,
+                                                int64_t t) {
+  int64_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int64_t step = (hi - lo + kWave - 1) / kWave;
+    const int64_t p = lo + static_cast<int64_t>(lane_id()) * step;
+    const bool le = p < hi && seg_off[p] <= t;
+    const int c = __popcll(ballot(le)) - 1;
+    lo += static_cast<int64_t>(c) * step;
+    hi = lo + step < hi ? lo + step : hi;
+  }
+  return lo;
+}
+
 template <typename TP, typename TI, typename TV, int CLS>
+__device__ __forceinline__ void run_unit(const BatchDesc& d, int32_t kind, int64_t b, int64_t e,
+                                         unsigned char* smem, int64_t* carry_row,
+                                         float* carry_val) {
+  int64_t* lds = reinterpret_cast<int64_t*>(smem);
+  if constexpr (CLS == kClsSddmm) {
+    sddmm_unit<TP, TI, TV>(d, b, e, smem);
+  } else if constexpr (CLS == kClsSpmv) {
+    if (kind == BK_ROWS) rows_spmv<TP, TI, TV>(d, b, e);
+    else tiles_spmv<TP, TI, TV>(d, b, e, lds, carry_row, carry_val);
+  } else {
+    if (kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV>(d, b, e);
+    else tiles_spmm<CLS, TP, TI, TV>(d, b, e, lds, carry_row, carry_val);
+  }
+}
+
+// One kernel per unit class (SDDMM, SpMV, SpMM of group width G), over the
+// work entries [w0, w1) of that class: each kernel holds the registers and
+// LDS of one unit body only.  Two launch forms:
+//   * persistent (seg_off == nullptr): a resident grid walks the entries as
+//     work units in a fixed stride;
+//   * segmented (seg_off != nullptr): entry u is a whole request's tiles
+//     [0, n_u) with its descriptor at desc[u], seg_off[i] the tiles before
+//     entry w0 + i, and one wave runs each of the `total` tiles, so the
+//     hardware can interleave the blocks of launches on different streams.
+// The two forms are separate instantiations (SEG), so the single-tile
+// segmented body is not held to the registers of the persistent unit loop.
+template <typename TP, typename TI, typename TV, int CLS, bool SEG>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t w0,
-    int64_t w1, int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
+    int64_t w1, const int64_t* __restrict__ seg_off, int64_t total,
+    int64_t* __restrict__ carry_row, float* __restrict__ carry_val) {
   __shared__ __align__(16) unsigned char smem[kWavesPerBlock][lds_bytes_for<CLS>()];
   const int w = wave_in_block();
-  int64_t* lds = reinterpret_cast<int64_t*>(smem[w]);
+  if constexpr (SEG) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
+    if (t >= total) return;
+    // Segmented entries start at 0 and desc[u] is entry u's descriptor.
+    const int64_t u = w0 + find_segment(seg_off, w1 - w0, t);
+    const BatchDesc d = desc[u];
+    const int64_t b = t - seg_off[u - w0];
+    run_unit<TP, TI, TV, CLS>(d, d.kind, b, b + 1, smem[w], carry_row, carry_val);
+    return;
+  }
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = w0 + xcd_block() * kWavesPerBlock + w; u < w1; u += nw) {
     const BatchWork wk = work[u];
-    const BatchDesc d = desc[wk.item];
-    if constexpr (CLS == kClsSddmm) {
-      sddmm_unit<TP, TI, TV>(d, wk.begin, wk.end, smem[w]);
-    } else if constexpr (CLS == kClsSpmv) {
-      if (wk.kind == BK_ROWS) rows_spmv<TP, TI, TV>(d, wk.begin, wk.end);
-      else tiles_spmv<TP, TI, TV>(d, wk.begin, wk.end, lds, carry_row, carry_val);
-    } else {
-      if (wk.kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV>(d, wk.begin, wk.end);
-      else tiles_spmm<CLS, TP, TI, TV>(d, wk.begin, wk.end, lds, carry_row, carry_val);
-    }
+    run_unit<TP, TI, TV, CLS>(desc[wk.item], wk.kind, wk.begin, wk.end, smem[w], carry_row,
+                              carry_val);
   }
 }
 
@@ -209,44 +224,58 @@ inline int64_t persistent_grid(int64_t n_work, int num_cus, int blocks_per_cu) {
   return std::max<int64_t>(grid, 1);
 }
 
-template <typename TP, typename TI, typename TV, int CLS>
-int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, hipStream_t s) {
-  static std::atomic<int> occ{0};
-  int o = occ.load();
-  if (o == 0) {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<TP, TI, TV, CLS>,
-                                                     kBatchThreads, 0) != hipSuccess ||
-        n < 1) {
-      (void)hipGetLastError();
-      n = 1;
+template <typename TP, typename TI, typename TV, int CLS, bool SEG>
+int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off, int64_t total,
+                hipStream_t s) {
+  int64_t grid;
+  if constexpr (SEG) {
+    grid = std::max<int64_t>(1, (total + kWavesPerBlock - 1) / kWavesPerBlock);
+  } else {
+    static std::atomic<int> occ{0};
+    int o = occ.load();
+    if (o == 0) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<TP, TI, TV, CLS, false>,
+                                                       kBatchThreads, 0) != hipSuccess ||
+          n < 1) {
+        (void)hipGetLastError();
+        n = 1;
+      }
+      occ.store(n);
+      o = n;
     }
-    occ.store(n);
-    o = n;
+    grid = persistent_grid(w1 - w0, p->num_cus, o);
   }
-  const int64_t grid = persistent_grid(w1 - w0, p->num_cus, o);
   p->grid = std::max<int64_t>(p->grid, grid);
-  hipLaunchKernelGGL((batch_kernel<TP, TI, TV, CLS>), dim3(static_cast<unsigned>(grid)),
-                     dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, w0, w1, p->d_carry_row,
-                     p->d_carry_val);
+  hipLaunchKernelGGL((batch_kernel<TP, TI, TV, CLS, SEG>), dim3(static_cast<unsigned>(grid)),
+                     dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, w0, w1, seg_off, total,
+                     p->d_carry_row, p->d_carry_val);
   return launch_check("batch_kernel");
 }
 
-// Launchers of one value type: NAME(p, cls, w0, w1, s) runs class `cls`
-// over work units [w0, w1) for the plan's index types.
+template <typename TP, typename TI, typename TV, int CLS>
+int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off,
+                 int64_t total, hipStream_t s) {
+  return seg_off ? launch_form<TP, TI, TV, CLS, true>(p, w0, w1, seg_off, total, s)
+                 : launch_form<TP, TI, TV, CLS, false>(p, w0, w1, nullptr, 0, s);
+}
+
+// Launchers of one value type: NAME(p, cls, w0, w1, seg_off, total, s) runs
+// class `cls` over work entries [w0, w1) for the plan's index types.
 #define FGX_BATCH_DEFINE_SPMM(NAME, TYPE)                                                   \
-  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s) {            \
+  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
+           int64_t total, hipStream_t s) {                                                  \
     using TV = TYPE;                                                                        \
     int rc = set_error(FGX_EUNSUP, "batch: unit class %d", cls);                           \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo), {                                     \
       switch (cls) {                                                                        \
-        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, s); break;        \
-        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, s); break;                      \
-        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, s); break;                      \
-        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, s); break;                      \
-        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, s); break;                    \
-        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, s); break;                    \
-        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, s); break;                    \
+        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, seg_off, total, s); break;        \
+        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, seg_off, total, s); break;                      \
+        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, seg_off, total, s); break;                      \
+        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, seg_off, total, s); break;                      \
+        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, seg_off, total, s); break;                    \
+        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, seg_off, total, s); break;                    \
+        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, seg_off, total, s); break;                    \
         default: break;                                                                     \
       }                                                                                     \
     });                                                                                     \
@@ -255,18 +284,22 @@ int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, hipStream_t s) {
   }
 
 #define FGX_BATCH_DEFINE_SDDMM(NAME, TYPE)                                                  \
-  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s) {            \
+  int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
+           int64_t total, hipStream_t s) {                                                  \
     using TV = TYPE;                                                                        \
     if (cls != kClsSddmm) return set_error(FGX_EUNSUP, "batch: unit class %d", cls);       \
     int rc = FGX_OK;                                                                        \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo),                                       \
-                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, s)));             \
+                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, seg_off, total, s)));             \
     return rc;                                                                              \
   }
 
-int batch_spmm_f32(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
-int batch_sddmm_f32(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
-int batch_spmm_bf16(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
-int batch_sddmm_bf16(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, hipStream_t s);
+#define FGX_BATCH_LAUNCH_ARGS \
+  fgx_batch_plan *p, int cls, int64_t w0, int64_t w1, const int64_t *seg_off, int64_t total, \
+      hipStream_t s
+int batch_spmm_f32(FGX_BATCH_LAUNCH_ARGS);
+int batch_sddmm_f32(FGX_BATCH_LAUNCH_ARGS);
+int batch_spmm_bf16(FGX_BATCH_LAUNCH_ARGS);
+int batch_sddmm_bf16(FGX_BATCH_LAUNCH_ARGS);
 
 }  // namespace fgx

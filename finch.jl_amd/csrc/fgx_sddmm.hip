@@ -21,30 +21,8 @@ __global__ __launch_bounds__(256) void sddmm_mfma_kernel(
   const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
-  constexpr int64_t W = kWave * kSddmmSub;
-  const int lane = lane_id();
-  const int64_t q0 = tile * W;
-  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  int64_t r_lo, r_hi, pos_lo;
-  FiberWindow<TP> win = tile_setup(ptr, base, m, q0, qend, table, tile, lds[w], &r_lo, &r_hi,
-                                   &pos_lo, kSdWin);
-#pragma unroll 1
-  for (int sub = 0; sub < kSddmmSub; sub++) {
-    const int64_t q = q0 + sub * kWave + lane;
-    const bool in = q < qend;
-    int64_t i = 0, j = 0;
-    float a = 0.f;
-    if (in) {
-      j = static_cast<int64_t>(ld_stream(idx + q)) - base;
-      a = widen<float>(ld_stream(val + q));
-      i = win.find(q);
-    }
-    f32x4 c = {0.f, 0.f, 0.f, 0.f};
-    sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, ys[w], c);
-    const float dot = mfma_diag(c);
-    if (in) __builtin_nontemporal_store(a * dot, out + q);
-    if (q0 + (sub + 1) * kWave >= qend) break;
-  }
+  sddmm_tile<TP, TI, TV>(ptr, idx, val, base, m, nnz, k, X, ldx, Y, ldy, out, tile, table, lds[w],
+                         ys[w], true);
 }
 
 template <typename TP, typename TI, typename TV, typename TA>

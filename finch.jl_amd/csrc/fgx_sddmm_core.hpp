@@ -16,13 +16,14 @@ namespace fgx {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kSddmmSub = 4;     // 64-position sub-tiles per wave tile
-constexpr int kSdChunk = 16;     // dense columns staged per pass
-constexpr int kSdPitch = 20;     // LDS row pitch (floats): conflict-free 16-byte reads
+constexpr int kSdChunk = 32;     // dense columns staged per pass: one 128-byte line per row
+constexpr int kSdPitch = 36;     // LDS row pitch (floats): conflict-free 16-byte reads
 constexpr int kSdWin = 256;      // fibre ends cached per wave
 
 // One 64-position sub-tile: lane t owns position q0+t.  The Y rows (of the
-// positions' columns: random rows) are staged through LDS 16 columns at a
-// time with coalesced 16-byte loads (4 lanes per row), then each lane reads
+// positions' columns: random rows) are staged through LDS 32 columns (one
+// 128-byte line) at a time with coalesced 16-byte loads (8 lanes per row),
+// so each line is requested from L2 once; then each lane reads
 // back its own row's values.  The X rows (of the positions' rows) are read
 // by each lane directly: positions of one row share its address, so a wave
 // touches only the few rows its positions fall in.  Each lane then feeds its
@@ -31,13 +32,15 @@ __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int6
                                                const float* __restrict__ Y, int64_t ldy,
                                                int64_t k, bool in, int64_t i, int64_t j,
                                                float (*ys)[kSdPitch], f32x4& c) {
+  constexpr int kLanesPerRow = kSdChunk / 4;          // 16-byte slices per staged row
+  constexpr int kRowsPerLoad = kWave / kLanesPerRow;   // rows one wave load covers
   const int lane = lane_id();
-  const int part = lane & 3;
+  const int part = lane % kLanesPerRow;
   for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
-    float4 yv[4];
+    float4 yv[kLanesPerRow];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int t = (lane >> 2) + 16 * s;
+    for (int s = 0; s < kLanesPerRow; s++) {
+      const int t = lane / kLanesPerRow + kRowsPerLoad * s;
       const int64_t jt = shfl(j, t);
       const int tin = shfl(static_cast<int>(in), t);
       const int64_t col = c0 + 4 * part;
@@ -52,8 +55,8 @@ __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int6
       if (in && col < k) xv[q] = *reinterpret_cast<const float4*>(X + i * ldx + col);
     }
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int t = (lane >> 2) + 16 * s;
+    for (int s = 0; s < kLanesPerRow; s++) {
+      const int t = lane / kLanesPerRow + kRowsPerLoad * s;
       *reinterpret_cast<float4*>(&ys[t][4 * part]) = yv[s];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -79,6 +82,52 @@ __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int6
 __device__ __forceinline__ float mfma_diag(const f32x4& c) {
   const int d = lane_id() & 3;
   return d == 0 ? c[0] : (d == 1 ? c[1] : (d == 2 ? c[2] : c[3]));
+}
+
+// One wave tile of kWave * kSddmmSub stored positions: the fibre window is
+// set up once, then each 64-position sub-tile computes its dot products (on
+// the matrix cores when `mfma`, else on the VALU in column order) and
+// stores out[q] = A[q] * dot.  Shared by the SDDMM kernel and the batch /
+// serving kernels, so a position's result is the same in either.
+template <typename TP, typename TI, typename TV>
+__device__ __forceinline__ void sddmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
+                                           const TV* __restrict__ val, int64_t base, int64_t m,
+                                           int64_t nnz, int64_t k, const float* __restrict__ X,
+                                           int64_t ldx, const float* __restrict__ Y, int64_t ldy,
+                                           float* __restrict__ out, int64_t tile,
+                                           const int64_t* __restrict__ table, int64_t* lds_win,
+                                           float (*ys)[kSdPitch], bool mfma) {
+  constexpr int64_t W = kWave * kSddmmSub;
+  const int lane = lane_id();
+  const int64_t q0 = tile * W;
+  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win =
+      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo, kSdWin);
+#pragma unroll 1
+  for (int sub = 0; sub < kSddmmSub; sub++) {
+    const int64_t q = q0 + sub * kWave + lane;
+    const bool in = q < qend;
+    int64_t i = 0, j = 0;
+    float a = 0.f;
+    if (in) {
+      j = static_cast<int64_t>(ld_stream(idx + q)) - base;
+      a = widen<float>(ld_stream(val + q));
+      i = win.find(q);
+    }
+    float dot = 0.f;
+    if (mfma) {
+      f32x4 c = {0.f, 0.f, 0.f, 0.f};
+      sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, ys, c);
+      dot = mfma_diag(c);
+    } else if (in) {
+      const float* xr = X + i * ldx;
+      const float* yr = Y + j * ldy;
+      for (int64_t kk = 0; kk < k; kk++) dot += xr[kk] * yr[kk];
+    }
+    if (in) __builtin_nontemporal_store(a * dot, out + q);
+    if (q0 + (sub + 1) * kWave >= qend) break;
+  }
 }
 
 }  // namespace fgx

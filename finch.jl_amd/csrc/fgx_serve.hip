@@ -8,8 +8,8 @@
 //   * admission and launch: while a stream is idle, requests are admitted in
 //     FIFO order as long as the in-flight slots and the HBM budget allow
 //     (footprint = the bytes of a request's inputs and output), and fused
-//     into one persistent launch of the batch kernel (fgx_batch.hip) on
-//     that stream.  Several streams keep launches overlapping on the
+//     into one launch of the batch kernels (fgx_batch.hip) on that stream,
+//     one wave per tile of every admitted request.  Several streams keep launches overlapping on the
 //     device: requests that arrive while others run go into the next launch.
 // Each request's work units depend only on its operand and width, never on
 // the rest of the batch, so a result is bit-identical alone and in any mix.
@@ -17,6 +17,7 @@
 // Plain path (mode 1): each request, in arrival order, is one library call
 // followed by a stream synchronisation — the per-call GPU path.
 #include "fgx_batch.hpp"
+#include "fgx_sddmm_core.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -31,17 +32,6 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kServeStreams = 4;
-constexpr double kUnitTarget = 256.0 * 1024.0;  // bytes of traffic per work unit, at most
-// A request is cut into at least this many units (when it has that many
-// tiles), so that a launch holding a few requests still spreads over every
-// wave slot of the device instead of queueing tiles behind a few waves.
-constexpr int64_t kUnitsPerRequest = 4096;
-
-int64_t spread_unit(int64_t unit, int64_t n) {
-  const int64_t fine = (n + kUnitsPerRequest - 1) / kUnitsPerRequest;
-  return std::max<int64_t>(1, std::min(unit, fine));
-}
-
 // First-fit allocator over a device arena, for request outputs the server owns.
 class Arena {
  public:
@@ -98,8 +88,7 @@ class Arena {
 
 struct OperandInfo {
   int64_t max_row = 0;
-  // Tile tables of large operands, by width class: [0] SpMV, [1] SDDMM, and
-  // per SpMM width the table for tile_width_for_k(k) (looked up below).
+  // Tile tables of the operand, one per tile width it is requested at.
   std::vector<std::pair<int64_t, int64_t*>> tables;  // (W, table)
   const int64_t* table_for(int64_t W) const {
     for (auto& t : tables)
@@ -115,25 +104,8 @@ struct Launch {
   int32_t seq = 0;
   std::vector<int64_t> reqs;
   fgx_batch_plan plan;
-  BatchDesc* h_desc = nullptr;
-  BatchWork* h_work = nullptr;
-  BatchWork* h_fix = nullptr;
-  int64_t cap_desc = 0, cap_work = 0, cap_fix = 0;
 };
 
-template <typename T>
-int grow_pinned(T** buf, int64_t* cap, int64_t need) {
-  if (need <= *cap && *buf) return FGX_OK;
-  const int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * *cap, 256));
-  if (*buf) (void)hipHostFree(*buf);
-  *buf = nullptr;
-  *cap = 0;
-  if (hipHostMalloc(reinterpret_cast<void**>(buf), static_cast<size_t>(want) * sizeof(T)) !=
-      hipSuccess)
-    return set_error(FGX_EOOM, "serve: pinned staging allocation failed");
-  *cap = want;
-  return FGX_OK;
-}
 
 struct Sizes {
   int64_t in = 0, in2 = 0, out = 0;
@@ -216,7 +188,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
   std::vector<int64_t> maxrow(static_cast<size_t>(n_operands), 0);
   if (rc == FGX_OK) rc = max_row_lengths(operands, n_operands, combo, maxrow.data(), L[0].stream);
   // Tile tables for the widths each operand is requested at, where the
-  // autotuner picks nnz-split tiles, and for SDDMM on large operands.
+  // autotuner picks nnz-split tiles, and for SDDMM.
   if (rc == FGX_OK) {
     for (int64_t r = 0; r < n_reqs && rc == FGX_OK; r++) {
       const fgx_request& q = reqs[r];
@@ -226,7 +198,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
       int64_t W = 0;
       if (q.kind == FGX_REQ_SDDMM) {
-        if (A.n_outer >= 65536) W = kWave;
+        W = kWave * kSddmmSub;
       } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0).kind == BK_TILES) {
         W = tile_width_for_k(k);
       }
@@ -399,15 +371,11 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           if (q.kind == FGX_REQ_SDDMM) {
             d.op = BO_SDDMM;
             d.mfma = (k % 4 == 0) && aligned16(q.in) && aligned16(q.in2);
-            d.table = oi.table_for(kWave);
+            d.table = oi.table_for(kWave * kSddmmSub);
             has_sddmm = true;
-            const int64_t nsub = (A.nnz + kWave - 1) / kWave;
-            const int64_t per = spread_unit(
-                std::max<int64_t>(1, static_cast<int64_t>(
-                                         kUnitTarget / (kWave * (8.0 + 8.0 * static_cast<double>(k))))),
-                nsub);
-            for (int64_t t = 0; t < nsub; t += per)
-              hwork.push_back(BatchWork{item, BK_SDDMM, t, std::min(nsub, t + per)});
+            // One entry per request: the launch runs a wave per SDDMM tile.
+            const int64_t W = kWave * kSddmmSub;
+            hwork.push_back(BatchWork{item, BK_SDDMM, 0, (A.nnz + W - 1) / W});
           } else {
             d.op = BO_SPMM;
             const TunePick pk = autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0);
@@ -420,32 +388,20 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
               d.carry_val0 = carry_vals;
               carry_tiles += nt;
               carry_vals += nt * width;
-              const int64_t unit = spread_unit(pk.unit, nt);
-              for (int64_t t = 0; t < nt; t += unit) {
-                const BatchWork w{item, BK_TILES, t, std::min(nt, t + unit)};
-                hwork.push_back(w);
-                hfix.push_back(w);
-              }
+              const BatchWork w{item, BK_TILES, 0, nt};  // a wave per tile
+              hwork.push_back(w);
+              hfix.push_back(w);
             } else if (A.n_outer > 0) {
-              const int64_t unit = spread_unit(pk.unit, A.n_outer);
-              for (int64_t r0 = 0; r0 < A.n_outer; r0 += unit)
-                hwork.push_back(BatchWork{item, BK_ROWS, r0, std::min(A.n_outer, r0 + unit)});
+              hwork.push_back(BatchWork{item, BK_ROWS, 0, A.n_outer});  // a wave per row
             }
           }
           hdesc.push_back(d);
         }
         if (l.reqs.empty()) continue;
-        rc = grow_pinned(&l.h_desc, &l.cap_desc, static_cast<int64_t>(hdesc.size()));
-        if (rc == FGX_OK) rc = grow_pinned(&l.h_work, &l.cap_work, static_cast<int64_t>(hwork.size()));
-        if (rc == FGX_OK) rc = grow_pinned(&l.h_fix, &l.cap_fix, static_cast<int64_t>(hfix.size()));
-        if (rc != FGX_OK) break;
-        std::copy(hdesc.begin(), hdesc.end(), l.h_desc);
-        std::copy(hwork.begin(), hwork.end(), l.h_work);
-        std::copy(hfix.begin(), hfix.end(), l.h_fix);
         l.plan.val_type = operands[0].val_type;
         l.plan.combo = combo;
-        rc = batch_launch_staged(&l.plan, l.h_desc, static_cast<int64_t>(hdesc.size()), l.h_work,
-                                 static_cast<int64_t>(hwork.size()), l.h_fix,
+        rc = batch_launch_staged(&l.plan, hdesc.data(), static_cast<int64_t>(hdesc.size()),
+                                 hwork.data(), static_cast<int64_t>(hwork.size()), hfix.data(),
                                  static_cast<int64_t>(hfix.size()), carry_tiles, carry_vals,
                                  has_sddmm, ctx->num_cus, l.stream);
         if (rc == FGX_OK && hipEventRecord(l.done, l.stream) != hipSuccess)
@@ -463,9 +419,6 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
   for (auto& l : L) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     batch_free(&l.plan);
-    if (l.h_desc) (void)hipHostFree(l.h_desc);
-    if (l.h_work) (void)hipHostFree(l.h_work);
-    if (l.h_fix) (void)hipHostFree(l.h_fix);
     if (l.done) (void)hipEventDestroy(l.done);
     if (l.stream) (void)hipStreamDestroy(l.stream);
   }
