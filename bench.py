@@ -312,8 +312,8 @@ def bench_serving(dev, slo_ms, large):
     operands32 = small + big
     operands16 = [A.to_values(torch.bfloat16) for A in operands32]
     inputs = serve.Inputs(operands32, seed=1)
-    qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 100_000, 140_000,
-                200_000, 300_000]
+    qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 85_000, 100_000,
+                120_000, 140_000, 200_000, 300_000]
 
     def sweep(ops_, mode):
         best, points = None, []

@@ -26,11 +26,15 @@ __device__ __forceinline__ void fixup_item_tile(const BatchDesc& d, int64_t t,
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
     const int64_t* __restrict__ seg_off, int64_t total, const int64_t* __restrict__ carry_row,
-    const float* __restrict__ carry_val) {
+    const float* __restrict__ carry_val, BatchDesc single, int use_single) {
   const int w = wave_in_block();
   if (seg_off) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
     if (t >= total) return;
+    if (use_single) {  // one entry: its descriptor is an argument
+      fixup_item_tile(single, t, carry_row, carry_val);
+      return;
+    }
     const int64_t u = find_segment(seg_off, n_fix, t);
     const BatchWork wk = fix[u];
     fixup_item_tile(desc[wk.item], wk.begin + (t - seg_off[u]), carry_row, carry_val);
@@ -79,13 +83,14 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     const int64_t* seg = p->segmented ? p->d_segoff + p->cls_seg0[c] : nullptr;
     const int64_t total = p->segmented ? p->cls_total[c] : 0;
     if (p->segmented && total == 0) continue;
+    const BatchDesc* one = (p->segmented && w1 - w0 == 1) ? &p->cls_single[c] : nullptr;
     int rc;
     if (p->val_type == FGX_F32)
-      rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, seg, total, s)
-                            : batch_spmm_f32(p, cls, w0, w1, seg, total, s);
+      rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, seg, total, one, s)
+                            : batch_spmm_f32(p, cls, w0, w1, seg, total, one, s);
     else if (p->val_type == FGX_BF16)
-      rc = cls == kClsSddmm ? batch_sddmm_bf16(p, cls, w0, w1, seg, total, s)
-                            : batch_spmm_bf16(p, cls, w0, w1, seg, total, s);
+      rc = cls == kClsSddmm ? batch_sddmm_bf16(p, cls, w0, w1, seg, total, one, s)
+                            : batch_spmm_bf16(p, cls, w0, w1, seg, total, one, s);
     else
       return set_error(FGX_EUNSUP, "batch: value type %d (F32 or BF16)", p->val_type);
     FGX_TRY(rc);
@@ -94,10 +99,12 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     const int64_t g = p->segmented
                           ? (p->fix_total + kWavesPerBlock - 1) / kWavesPerBlock
                           : std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
+    const bool one = p->segmented && p->n_fix == 1;
     hipLaunchKernelGGL(batch_fixup_kernel, dim3(static_cast<unsigned>(g)), dim3(kBatchThreads), 0,
                        s, p->d_desc, p->d_fix, p->n_fix,
                        p->segmented ? p->d_segoff + p->fix_seg0 : nullptr, p->fix_total,
-                       p->d_carry_row, p->d_carry_val);
+                       p->d_carry_row, p->d_carry_val, one ? p->fix_single : BatchDesc{},
+                       one ? 1 : 0);
     FGX_TRY(launch_check("batch_fixup_kernel"));
   }
   return FGX_OK;
@@ -252,6 +259,9 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
     seg.push_back(acc);
     p->cls_total[c] = acc;
   }
+  for (int c = 0; c < p->n_cls; c++)
+    if (p->cls_w1[c] - p->cls_w0[c] == 1) p->cls_single[c] = h_desc[p->cls_w0[c]];
+  if (nf == 1) p->fix_single = h_desc[h_fix[0].item];
   p->fix_seg0 = static_cast<int64_t>(seg.size());
   int64_t facc = 0;
   for (int64_t u = 0; u < nf; u++) {

@@ -100,6 +100,8 @@ struct fgx_batch_plan {
   int64_t cls_seg0[fgx::kMaxClasses] = {};        // class c's prefix array in d_segoff
   int64_t cls_total[fgx::kMaxClasses] = {};       // tiles of class c
   int64_t fix_seg0 = 0, fix_total = 0;            // the fix-up entries' prefix array
+  fgx::BatchDesc cls_single[fgx::kMaxClasses] = {};  // class c's descriptor if it has one entry
+  fgx::BatchDesc fix_single = {};                  // the fix-up entry's, if there is one
   void* d_stage = nullptr;
   void* h_stage = nullptr;                        // pinned
   int64_t cap_stage = 0, cap_hstage = 0;

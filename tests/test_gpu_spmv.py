@@ -183,7 +183,7 @@ def test_spmm_layouts_and_long_rows(dev, layout):
 
 # ---- SDDMM ------------------------------------------------------------------------------
 
-@pytest.mark.parametrize("k", [4, 32, 64, 7])
+@pytest.mark.parametrize("k", [4, 32, 36, 64, 100, 7])
 @pytest.mark.parametrize("name", ["ragged", "long_row", "sparse_rows"])
 def test_sddmm(dev, k, name):
     ptr, idx, val, m, n = EDGE[name]
