@@ -72,6 +72,8 @@ def main():
                     help="approximate CPU time budget of the baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-form section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--no-sparse", action="store_true",
+                    help="skip the coiteration / SpGEMM and 10k FP64 sections")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--serve-slo-ms", type=float, default=5.0,
                     help="p99 latency target at which served req/s are compared")
@@ -173,6 +175,11 @@ def main():
                          f"0..{rows} of A ({q1} nnz) x the full 1M x 64 B; median of {reps} "
                          f"reps, {tmed * 1e3:.1f} ms each"}
 
+    sparse = None
+    if not args.no_sparse:
+        sparse = bench_sparse_sparse(dev, with_cpu=not args.no_cpu)
+        sparse["fp64_10k"] = bench_fp64_10k(dev)
+        print("# sparse-sparse done", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
         del C, X, out
@@ -217,6 +224,7 @@ def main():
                   "gflops": round(2 * nnz * K / (sddmm_ms * 1e-3) / 1e9, 1),
                   "pct_of_hbm_peak": round(100 * gbs(by_sddmm, sddmm_ms) / HBM_PEAK_GBS, 2)},
         "cpu_baseline": cpu,
+        "sparse_sparse": sparse,
         "batched": batched,
         "serving": served,
     }
@@ -237,6 +245,104 @@ def _events_ms(fn, reps, warmup=1):
     e.record()
     torch.cuda.synchronize()
     return s.elapsed_time(e) / reps
+
+
+def _csr_bytes(A, vbytes=4, ibytes=4, pbytes=4):
+    return (A.n_outer + 1) * pbytes + A.nnz * (ibytes + vbytes)
+
+
+def _wall_ms(fn, reps):
+    """Mean wall time of fn() with a device synchronisation after each call
+    (the two-phase calls read nnz back to the host between their phases)."""
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        fn()
+        torch.cuda.synchronize()
+    return (time.perf_counter() - t) / reps * 1e3
+
+
+def bench_sparse_sparse(dev, with_cpu=True):
+    """Coiteration and Gustavson SpGEMM: A, B 200,000 x 200,000 CSR with
+    Poisson(8) / Poisson(24) nnz per row and uniform columns.  A.*B, A+B and
+    max.(A,B) on FP32 values; C = A*B with Int32 values U{1..9} (plus-times)
+    and with FP32 values (min-plus).  Each call is the whole two-phase
+    operation (symbolic, nnz read back, allocation, numeric).  The CPU leg is
+    the oracle's serial loops (Finch runs these serially: SparseList outputs
+    are not concurrent)."""
+    import numpy as np
+    import torch
+
+    from finchgpu import ops, synth
+
+    n = 200_000
+    ap, ai, av = synth.poisson_csr(n, n, 8, seed=21)
+    bp, bi, bv = synth.poisson_csr(n, n, 24, seed=22)
+    up = lambda p_, i_, v_: ops.Compressed(torch.from_numpy(p_.astype(np.int32)).to(dev),
+                                           torch.from_numpy(i_.astype(np.int32)).to(dev),
+                                           torch.from_numpy(v_).to(dev), n, n, 1)
+    A, B = up(ap, ai, av), up(bp, bi, bv)
+    gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
+    out = {"workload": f"200k x 200k CSR, A Poisson(8) nnz={A.nnz}, B Poisson(24) nnz={B.nnz}, "
+                       "Int32 ptr/idx"}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+    for op in ("mul", "add", "max"):
+        C = ops.ewise(op, A, B)
+        ms = _wall_ms(lambda: ops.ewise(op, A, B), 10)
+        by = _csr_bytes(A) + _csr_bytes(B) + _csr_bytes(C)
+        row = {"ms": round(ms, 3), "nnz_out": C.nnz, "algorithmic_bytes": by, "gbs": gbs(by, ms)}
+        if with_cpu:
+            t = time.perf_counter()
+            oracle.ewise(op, ap, ai, av, bp, bi, bv)
+            row["cpu_ms_1thread"] = round((time.perf_counter() - t) * 1e3, 1)
+        out[f"ewise_{op}"] = row
+    rng = np.random.default_rng(23)
+    Ai = up(ap, ai, rng.integers(1, 10, size=ai.size).astype(np.int32))
+    Bi = up(bp, bi, rng.integers(1, 10, size=bi.size).astype(np.int32))
+    blen = np.diff(bp)
+    products = int(blen[ai - 1].sum())
+    for name, sr, X, Y in (("spgemm_plus_times_i32", "plus_times", Ai, Bi),
+                           ("spgemm_min_plus_f32", "min_plus", A, B)):
+        C = ops.spgemm(sr, X, Y)
+        ms = _wall_ms(lambda: ops.spgemm(sr, X, Y), 5)
+        by = _csr_bytes(X) + _csr_bytes(Y) + _csr_bytes(C)
+        row = {"ms": round(ms, 3), "nnz_out": C.nnz, "products": products,
+               "gproducts_per_s": round(products / (ms * 1e-3) / 1e9, 3),
+               "algorithmic_bytes": by, "gbs": gbs(by, ms)}
+        if with_cpu:
+            rows = n // 10   # bounded sample: the first tenth of the rows
+            xv = X.val.cpu().numpy()
+            yv = Y.val.cpu().numpy()
+            q1 = int(ap[rows] - 1)
+            t = time.perf_counter()
+            oracle.spgemm(sr, rows, n, ap[: rows + 1], ai[:q1], xv[:q1], bp, bi, yv)
+            row["cpu_ms_1thread_first_tenth_rows"] = round((time.perf_counter() - t) * 1e3, 1)
+        out[name] = row
+    return out
+
+
+def bench_fp64_10k(dev):
+    """The parity configuration: 10,000 x 10,000 FP64 at density 1e-3 (fsprand
+    semantics), CSC SpMV and the swizzled CSR ordering; single calls are
+    latency-bound (1.6 MB), so this reports latency."""
+    import numpy as np
+    import torch
+
+    from finchgpu import ops, synth
+
+    ptr, idx, val = synth.fsprand(10_000, 10_000, 1e-3, seed=0)
+    A = ops.Compressed(torch.from_numpy(ptr).to(dev), torch.from_numpy(idx).to(dev),
+                       torch.from_numpy(val).to(dev), 10_000, 10_000, 1)
+    x = torch.from_numpy(synth.dense(10_000, 1, seed=1, dtype=np.float64, kind="sym")[:, 0]).to(dev)
+    y = torch.empty(10_000, dtype=torch.float64, device=dev)
+    csc = _events_ms(lambda: ops.spmv_csc(A, x, y), 50)
+    csr = _events_ms(lambda: ops.spmv_csr(A, x, y), 50)
+    return {"nnz": A.nnz, "csc_spmv_us": round(csc * 1e3, 2), "csr_spmv_us": round(csr * 1e3, 2)}
 
 
 def bench_batched(dev, count):

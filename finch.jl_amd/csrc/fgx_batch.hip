@@ -305,7 +305,13 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   if (nw > 0) std::memcpy(h + o_work, h_work, static_cast<size_t>(nw) * sizeof(BatchWork));
   if (nf > 0) std::memcpy(h + o_fix, h_fix, static_cast<size_t>(nf) * sizeof(BatchWork));
   std::memcpy(h + o_seg, seg.data(), seg.size() * sizeof(int64_t));
-  FGX_HIP(hipMemcpyAsync(p->d_stage, p->h_stage, bytes, hipMemcpyHostToDevice, s));
+  // When every class (and the fix-up) has a single entry, the kernels take
+  // their descriptors as arguments and read nothing staged: skip the copy.
+  bool all_single = nf <= 1;
+  for (int c = 0; c < p->n_cls && all_single; c++)
+    all_single = p->cls_w1[c] - p->cls_w0[c] == 1;
+  if (!all_single)
+    FGX_HIP(hipMemcpyAsync(p->d_stage, p->h_stage, bytes, hipMemcpyHostToDevice, s));
   char* d = static_cast<char*>(p->d_stage);
   p->d_desc = reinterpret_cast<BatchDesc*>(d + o_desc);
   p->d_work = reinterpret_cast<BatchWork*>(d + o_work);

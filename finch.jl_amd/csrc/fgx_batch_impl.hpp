@@ -11,10 +11,19 @@
 namespace fgx {
 
 // ---- ROWS units ----------------------------------------------------------------------
-// SpMV: kSpmvRowGroup lanes per row, strided over the row, then a shuffle tree.
+// Row-split: a lane group owns a row at a time, with no carries, tile table
+// or fix-up.  Each row's coordinates/values are loaded a chunk at a time in
+// one round trip and the dependent gathers are issued together, so a short
+// row costs two or three round trips.
+
+// SpMV: kSpmvRowGroup lanes per row; each lane takes every G-th position of
+// a chunk of kRowsSpmvRun * G positions, then a shuffle tree per row.
+constexpr int kRowsSpmvRun = 4;
+
 template <typename TP, typename TI, typename TV>
 __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_t r1) {
   constexpr int G = kSpmvRowGroup;
+  constexpr int S = kRowsSpmvRun;
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
   const TP* ptr = static_cast<const TP*>(d.ptr);
@@ -24,20 +33,38 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
   for (int64_t r = r0 + g; r < r1; r += kWave / G) {
     const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
     float acc = 0.f;
-    for (int64_t q = q0 + gl; q < q1; q += G)
-      acc += widen<float>(ld_stream(val + q)) * d.B[static_cast<int64_t>(ld_stream(idx + q)) - base];
+    for (int64_t c = q0; c < q1; c += S * G) {
+      TI jj[S];
+      TV vv[S];
+#pragma unroll
+      for (int u = 0; u < S; u++) {
+        const int64_t q = c + gl + u * G;
+        jj[u] = q < q1 ? ld_stream(idx + q) : static_cast<TI>(base);
+        vv[u] = q < q1 ? ld_stream(val + q) : TV{};
+      }
+      float xg[S];
+#pragma unroll
+      for (int u = 0; u < S; u++)
+        xg[u] = c + gl + u * G < q1 ? d.B[static_cast<int64_t>(jj[u]) - base] : 0.f;
+#pragma unroll
+      for (int u = 0; u < S; u++) acc += widen<float>(vv[u]) * xg[u];
+    }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
     if (gl == 0) d.C[r] = acc;
   }
 }
 
-// SpMM: G lanes per row, each lane a 16-byte slice of the dense width; four
-// stored positions in flight per lane.
+// SpMM: G lanes per row, each lane a 16-byte slice of the dense width.  A
+// row is walked in chunks of G positions (lane gl loads position c + gl),
+// broadcast within the group, with U rows of B gathered at a time; the sum
+// runs over the row's positions in order.
 template <int G, typename TP, typename TI, typename TV>
 __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_t r1) {
+  constexpr int U = G < 8 ? G : 8;
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
+  const int gbase = g * G;
   const TP* ptr = static_cast<const TP*>(d.ptr);
   const TI* idx = static_cast<const TI*>(d.idx);
   const TV* val = static_cast<const TV*>(d.val);
@@ -47,28 +74,33 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
   for (int64_t r = r0 + g; r < r1; r += kWave / G) {
     const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
-    int64_t q = q0;
-    for (; q + 4 <= q1; q += 4) {
-      float b[4][4];
-      float v[4];
+    for (int64_t c = q0; c < q1; c += G) {
+      const bool inc = c + gl < q1;
+      const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + c + gl)) - base : 0;
+      const float vcv = inc ? widen<float>(ld_stream(val + c + gl)) : 0.f;
+      const int cnt = static_cast<int>(q1 - c < G ? q1 - c : G);
+      for (int e0 = 0; e0 < cnt; e0 += U) {
+        float bv[U][4];
+        float ve[U];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int64_t j = static_cast<int64_t>(ld_stream(idx + q + u)) - base;
-        v[u] = widen<float>(ld_stream(val + q + u));
-        if (vc > 0) ld_dense<4, float>(d.B + j * d.ldb + col0, vc, b[u]);
+        for (int u = 0; u < U; u++) {
+          const int src = gbase + ((e0 + u) % G);
+          const int64_t je = shfl(jc, src);
+          ve[u] = shfl(vcv, src);
+          if (e0 + u < cnt && vc > 0) {
+            ld_dense<4, float>(d.B + je * d.ldb + col0, vc, bv[u]);
+          } else {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) bv[u][cc] = 0.f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (e0 + u < cnt) {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) acc[cc] += ve[u] * bv[u][cc];
+          }
       }
-#pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[c] += v[u] * b[u][c];
-    }
-    for (; q < q1; q++) {
-      const int64_t j = static_cast<int64_t>(ld_stream(idx + q)) - base;
-      const float v = widen<float>(ld_stream(val + q));
-      float b[4] = {0.f, 0.f, 0.f, 0.f};
-      if (vc > 0) ld_dense<4, float>(d.B + j * d.ldb + col0, vc, b);
-#pragma unroll
-      for (int c = 0; c < 4; c++) acc[c] += v * b[c];
     }
     if (vc > 0) st_dense<4, float>(d.C + r * d.ldc + col0, vc, acc);
   }

@@ -19,28 +19,37 @@ int set_error(int code, const char* fmt, ...) {
 
 void clear_error() { g_last_error[0] = '\0'; }
 
-int scratch(fgx_ctx* ctx, size_t bytes, void** out) {
+// Grows one of the context's device arenas to at least `bytes`.  The old
+// arena may still be in use by queued work, so the device is drained first.
+static int grow_arena(fgx_ctx* ctx, void** buf, size_t* cur, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
-  if (bytes > ctx->scratch_bytes) {
-    if (ctx->scratch) {
-      // The stream may still use the old arena.
+  if (bytes > *cur) {
+    if (*buf) {
       FGX_HIP(hipStreamSynchronize(ctx->stream));
       FGX_HIP(hipDeviceSynchronize());
-      FGX_HIP(hipFree(ctx->scratch));
-      ctx->scratch = nullptr;
-      ctx->scratch_bytes = 0;
+      FGX_HIP(hipFree(*buf));
+      *buf = nullptr;
+      *cur = 0;
     }
     size_t want = bytes + bytes / 4;
     want = (want + 4095) & ~static_cast<size_t>(4095);
-    if (hipMalloc(&ctx->scratch, want) != hipSuccess) {
+    if (hipMalloc(buf, want) != hipSuccess) {
       (void)hipGetLastError();
-      ctx->scratch = nullptr;
+      *buf = nullptr;
       return set_error(FGX_EOOM, "scratch allocation of %zu bytes failed", want);
     }
-    ctx->scratch_bytes = want;
+    *cur = want;
   }
-  *out = ctx->scratch;
+  *out = *buf;
   return FGX_OK;
+}
+
+int scratch(fgx_ctx* ctx, size_t bytes, void** out) {
+  return grow_arena(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes, out);
+}
+
+int dense_scratch(fgx_ctx* ctx, size_t bytes, void** out) {
+  return grow_arena(ctx, &ctx->dscratch, &ctx->dscratch_bytes, bytes, out);
 }
 
 constexpr size_t kMaxTables = 32;
@@ -213,6 +222,7 @@ int fgx_ctx_destroy(fgx_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   free_tile_tables(ctx);
   if (ctx->scratch) (void)hipFree(ctx->scratch);
+  if (ctx->dscratch) (void)hipFree(ctx->dscratch);
   if (ctx->host_i64) (void)hipHostFree(ctx->host_i64);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

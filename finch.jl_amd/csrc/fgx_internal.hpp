@@ -31,6 +31,10 @@ struct fgx_ctx {
   // context; calls on one context are serialised by `mu`.
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // Second arena for dense operands relaid out by a call (column-major
+  // dense operands transposed to k-fast rows).
+  void* dscratch = nullptr;
+  size_t dscratch_bytes = 0;
   // Small pinned host buffer for data-dependent sizes (symbolic nnz).
   int64_t* host_i64 = nullptr;
   std::vector<fgx_tile_table> tables;
@@ -60,6 +64,7 @@ void clear_error();
 
 // Scratch of at least `bytes` (grows, never shrinks).  Caller holds ctx->mu.
 int scratch(fgx_ctx* ctx, size_t bytes, void** out);
+int dense_scratch(fgx_ctx* ctx, size_t bytes, void** out);
 
 // The stream a call runs on: the caller's hipStream_t; NULL is the HIP null
 // (default) stream, as in rocBLAS, so calls order with the caller's other

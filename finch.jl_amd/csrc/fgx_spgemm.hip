@@ -327,24 +327,45 @@ __global__ __launch_bounds__(256) void spgemm_window_kernel(
   }
 }
 
-// Row bins: 0 wave team, 1 block team, 2 dense window.
+// Row bins: 0 wave team, 1 block team, 2 dense window.  Slots are taken
+// with one atomic per wave and bin (ballot + rank), not one per row: a
+// single counter hit by every row serialises the whole pass.  The order of
+// rows within a bin does not affect any output.
 template <typename TP, typename TI>
-__global__ void spgemm_bin_kernel(int64_t m, int64_t base, const TP* __restrict__ aptr,
-                                  const TI* __restrict__ aidx, const TP* __restrict__ bptr,
-                                  int64_t* __restrict__ bin_count, int64_t* __restrict__ bin_rows) {
+__global__ __launch_bounds__(256) void spgemm_bin_kernel(int64_t m, int64_t base,
+                                                         const TP* __restrict__ aptr,
+                                                         const TI* __restrict__ aidx,
+                                                         const TP* __restrict__ bptr,
+                                                         int64_t* __restrict__ bin_count,
+                                                         int64_t* __restrict__ bin_rows) {
   const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (row >= m) return;
-  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
-  int64_t prod = 0;
-  for (int64_t q = a0; q < a1 && prod <= kBlockCap; q++) {
-    const int64_t k = static_cast<int64_t>(aidx[q]) - base;
-    prod += pos_at(bptr, base, k + 1) - pos_at(bptr, base, k);
+  const bool valid = row < m;
+  int bin = 3;
+  if (valid) {
+    const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+    int64_t prod = 0;
+    for (int64_t q = a0; q < a1 && prod <= kBlockCap; q++) {
+      const int64_t k = static_cast<int64_t>(aidx[q]) - base;
+      prod += pos_at(bptr, base, k + 1) - pos_at(bptr, base, k);
+    }
+    const int64_t key = prod > a1 - a0 ? prod : a1 - a0;
+    bin = key <= kWaveCap ? 0 : (key <= kBlockCap ? 1 : 2);
   }
-  const int64_t key = prod > a1 - a0 ? prod : a1 - a0;
-  const int bin = key <= kWaveCap ? 0 : (key <= kBlockCap ? 1 : 2);
-  const unsigned long long slot =
-      atomicAdd(reinterpret_cast<unsigned long long*>(bin_count + bin), 1ull);
-  bin_rows[bin * m + static_cast<int64_t>(slot)] = row;
+  const int lane = lane_id();
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int b = 0; b < 3; b++) {
+    const uint64_t mask = ballot(bin == b);
+    if (mask == 0) continue;
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    unsigned long long first = 0;
+    if (lane == leader)
+      first = atomicAdd(reinterpret_cast<unsigned long long*>(bin_count + b),
+                        static_cast<unsigned long long>(__popcll(mask)));
+    first = shfl(first, leader);
+    if (bin == b)
+      bin_rows[b * m + static_cast<int64_t>(first) + __popcll(mask & below)] = row;
+  }
 }
 
 struct Bins {

@@ -69,6 +69,39 @@ __global__ __launch_bounds__(256) void spmm_csr_strided_kernel(
   }
 }
 
+// dst[r * ldd + c] = src[r + c * lds] for r < rows, c < cols: a column-major
+// matrix to row-major (and, viewed transposed, row-major to column-major).
+// 64 x 64 tiles through LDS, both sides coalesced; a 1-D grid of tiles.
+template <typename T>
+__global__ __launch_bounds__(256) void col_to_row_kernel(const T* __restrict__ src, int64_t rows,
+                                                         int64_t cols, int64_t lds,
+                                                         T* __restrict__ dst, int64_t ldd) {
+  __shared__ T tile[64][65];
+  const int64_t ntr = (rows + 63) / 64;
+  const int64_t id = blockIdx.x;
+  const int64_t r0 = (id % ntr) * 64, c0 = (id / ntr) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + tx, c = c0 + i;
+    if (r < rows && c < cols) tile[i][tx] = src[r + c * lds];
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) dst[r * ldd + c] = tile[tx][i];
+  }
+}
+
+template <typename T>
+static int col_to_row(const T* src, int64_t rows, int64_t cols, int64_t lds, T* dst, int64_t ldd,
+                      hipStream_t s) {
+  if (rows == 0 || cols == 0) return FGX_OK;
+  const int64_t tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
+  hipLaunchKernelGGL((col_to_row_kernel<T>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, s,
+                     src, rows, cols, lds, dst, ldd);
+  return launch_check("col_to_row_kernel");
+}
+
 // SpMM dispatch over the dense width: lane groups of G lanes x KV columns.
 template <typename TP, typename TI, typename TV, typename TA, int G, int KV>
 static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t kofs,
@@ -108,6 +141,32 @@ static int spmm_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, cons
   TA* C = static_cast<TA*>(Cv);
   const int64_t m = A->n_outer;
   if (m == 0 || k == 0) return FGX_OK;
+  if ((b_layout != FGX_ROW_MAJOR || c_layout != FGX_ROW_MAJOR) && k >= 4) {
+    // Column-major dense operands (Finch's Dense(Dense) layout): relay them
+    // out as k-fast rows in the context's dense arena, run the tiled kernels,
+    // and transpose the result back.  Costs one extra pass over B and C.
+    const int64_t n = A->n_inner;
+    const size_t bt = b_layout == FGX_ROW_MAJOR ? 0 : ((static_cast<size_t>(n * k) * sizeof(TA) + 255) & ~size_t(255));
+    const size_t ct = c_layout == FGX_ROW_MAJOR ? 0 : static_cast<size_t>(m * k) * sizeof(TA);
+    void* arena = nullptr;
+    FGX_TRY(dense_scratch(ctx, bt + ct, &arena));
+    const TA* Bf = B;
+    int64_t ldbf = ldb;
+    if (bt) {
+      TA* Bt = static_cast<TA*>(arena);
+      FGX_TRY(col_to_row<TA>(B, n, k, ldb, Bt, k, s));
+      Bf = Bt;
+      ldbf = k;
+    }
+    TA* Cf = c_layout == FGX_ROW_MAJOR ? C : reinterpret_cast<TA*>(static_cast<char*>(arena) + bt);
+    const int64_t ldcf = c_layout == FGX_ROW_MAJOR ? ldc : k;
+    FGX_TRY((spmm_csr_typed<TP, TI, TV>(ctx, A, k, Bf, ldbf, FGX_ROW_MAJOR, Cf, ldcf,
+                                        FGX_ROW_MAJOR, s)));
+    // Row-major m x k (ld k) is column-major k x m; its row-major form with
+    // ld ldc is the column-major m x k output.
+    if (ct) FGX_TRY(col_to_row<TA>(Cf, k, m, k, C, ldc, s));
+    return FGX_OK;
+  }
   if (b_layout != FGX_ROW_MAJOR || c_layout != FGX_ROW_MAJOR) {
     const int64_t b_rs = b_layout == FGX_ROW_MAJOR ? ldb : 1;
     const int64_t b_cs = b_layout == FGX_ROW_MAJOR ? 1 : ldb;
