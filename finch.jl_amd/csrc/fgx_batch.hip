@@ -368,11 +368,14 @@ int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* ou
 TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
                               int flags) {
   const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
-  (void)mean;
-  // Merge-path tiles keep every wave busy with a fixed number of positions
-  // and stage their coordinates in LDS; the row-split form walks its rows
-  // one lane group at a time with a dependent load chain per row.
-  bool tiles = true;
+  // Measured on MI355X (graph replays of 64-256 same-class items, row-split
+  // time / tile time): row-split wins only for wide dense operands (k >= 128,
+  // 32+ lanes per row) over even, long rows (Poisson, mean 32-128):
+  // 0.77-0.95.  Everywhere else merge-path tiles win, by 1.1-2x on even rows
+  // and by up to 40x on power-law rows, where one long row serialises a lane
+  // group.
+  const bool even = static_cast<double>(max_row) <= 2.0 * mean + 16.0;
+  bool tiles = !(k >= 128 && mean >= 24.0 && even);
   if (flags & FGX_BATCH_FORCE_ROWSPLIT) tiles = false;
   if (flags & FGX_BATCH_FORCE_NNZSPLIT) tiles = true;
   const double target = 256.0 * 1024.0;

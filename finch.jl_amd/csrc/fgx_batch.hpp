@@ -59,8 +59,7 @@ __host__ __device__ inline int group_for_k(int64_t k) {
   return g;
 }
 
-// Positions per lane group of a TILES tile, for G lanes per position.
-__host__ __device__ constexpr int tile_e_for_g(int g) { return g >= 16 ? 64 : (g >= 4 ? 32 : 16); }
+
 
 // Work-unit classes: the batch runs one persistent kernel per class, so each
 // kernel carries the registers and LDS of one unit body.  SpMM classes are

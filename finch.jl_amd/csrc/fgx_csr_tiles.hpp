@@ -25,6 +25,15 @@ namespace fgx {
 constexpr int kWavesPerBlock = 4;
 constexpr int kWinCap = 512;  // fibre ends cached in LDS per wave
 
+// Positions per lane group of an SpMM tile, for G lanes per position (the
+// batch and serving kernels use the same tiles).
+#ifndef FGX_TILE_E_WIDE
+#define FGX_TILE_E_WIDE 128  // positions per lane group for G >= 32
+#endif
+__host__ __device__ constexpr int tile_e_for_g(int g) {
+  return g >= 32 ? FGX_TILE_E_WIDE : (g >= 16 ? 64 : (g >= 4 ? 32 : 16));
+}
+
 // SpMM tiles stage their coordinates/values in LDS (1) or load them from
 // global memory chunk by chunk (0).
 #ifndef FGX_SPMM_STAGE

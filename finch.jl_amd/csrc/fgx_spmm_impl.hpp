@@ -109,7 +109,7 @@ static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t
   constexpr int NG = kWave / G;
   // Positions per lane group (E) and B rows per gather batch (U); wide
   // groups take FGX_SPMM_E / FGX_SPMM_U.
-  constexpr int E = (G >= 16) ? FGX_SPMM_E : (G >= 4 ? 32 : 16);
+  constexpr int E = (G >= 32) ? tile_e_for_g(G) : ((G >= 16) ? FGX_SPMM_E : (G >= 4 ? 32 : 16));
   constexpr int U = G < 8 ? G : (FGX_SPMM_U < G ? FGX_SPMM_U : G);
   constexpr int64_t W = static_cast<int64_t>(NG) * E;
   const int64_t m = A->n_outer;

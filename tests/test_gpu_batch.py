@@ -54,6 +54,24 @@ def skewed_items(dev):
     return items
 
 
+def wide_even_items(dev):
+    """Wide dense operands over even, long rows: the autotuner picks row-split."""
+    return [batch.BatchItem(batch.synth_matrix(1500, 40.0, 70 + i, dev),
+                            torch.randn((1500, 128), device=dev)) for i in range(3)]
+
+
+def test_autotuner_picks_rowsplit_for_wide_even_rows(dev, small_batch):
+    items = wide_even_items(dev) + skewed_items(dev)
+    plan = batch.BatchPlan(items)
+    st = plan.stats()
+    assert st["items_rowsplit"] == 3 and st["items_nnzsplit"] == 3
+    plan.run()
+    torch.cuda.synchronize()
+    for it in items:
+        assert rel_err(it.C.cpu().numpy(), host_ref(it)) <= FP32_NORMWISE
+    plan.close()
+
+
 @pytest.mark.parametrize("flags", [0, L.BATCH_NO_GRAPH, L.BATCH_FORCE_NNZSPLIT,
                                    L.BATCH_FORCE_ROWSPLIT])
 def test_batch_matches_oracle(dev, small_batch, flags):
