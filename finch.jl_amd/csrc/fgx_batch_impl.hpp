@@ -136,7 +136,7 @@ __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64
   TI* lj = reinterpret_cast<TI*>(lds + kWinCap);
   float* lv = reinterpret_cast<float*>(lj + W);
   for (int64_t t = t0; t < t1; t++)
-    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U, true>(
+    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U>(
         static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
         static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t,
         d.table, lds, carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4),

@@ -34,12 +34,6 @@ __host__ __device__ constexpr int tile_e_for_g(int g) {
   return g >= 32 ? FGX_TILE_E_WIDE : (g >= 16 ? 64 : (g >= 4 ? 32 : 16));
 }
 
-// SpMM tiles stage their coordinates/values in LDS (1) or load them from
-// global memory chunk by chunk (0).
-#ifndef FGX_SPMM_STAGE
-#define FGX_SPMM_STAGE 1
-#endif
-
 // Fibres that a wave tile [q0, qend) touches: r_lo owns q0 (0 for the first
 // tile), r_hi owns qend (the row a carry continues into; m if none).
 template <typename TP>
@@ -281,8 +275,7 @@ __device__ __forceinline__ void st_dense_nt(TA* __restrict__ p, int valid_cols, 
   st_dense<KV, TA>(p, valid_cols, o);
 }
 
-template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U,
-          bool STAGE = (FGX_SPMM_STAGE != 0)>
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
 __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                           const TV* __restrict__ val, int64_t base, int64_t m,
                                           int64_t nnz, int64_t k, int64_t kofs,
@@ -291,7 +284,7 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
                                           const int64_t* __restrict__ table, int64_t* lds_win,
                                           int64_t* carry_row_out,
                                           TA* carry_val_out /* G*KV entries */,
-                                          TI* lds_j = nullptr, TA* lds_v = nullptr) {
+                                          TI* lds_j, TA* lds_v) {
   constexpr int NG = kWave / G;
   constexpr int W = NG * E;
   static_assert(E % G == 0, "E must be a multiple of G");
@@ -299,37 +292,32 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
   const int lane = lane_id();
   const int g = lane / G;
   const int gl = lane % G;
-  const int gbase = g * G;
   const int64_t q0 = tile * W;
   const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  // STAGE: the tile's coordinates and values, loaded coalesced before the
-  // fibre set-up (so the loads overlap it) and parked in LDS afterwards; the
-  // gather loop then reads them as LDS broadcasts, with no global round trip.
-  constexpr int NS = STAGE ? W / kWave : 1;
+  // The tile's coordinates and values, loaded coalesced before the fibre
+  // set-up (so the loads overlap it) and parked in LDS afterwards; the gather
+  // loop then reads them as LDS broadcasts, with no global round trip.
+  constexpr int NS = W / kWave;
   TI sj[NS];
   TA sv[NS];
-  if constexpr (STAGE) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      const int64_t q = q0 + s * kWave + lane;
-      sj[s] = q < qend ? ld_stream(idx + q) : static_cast<TI>(base);
-      sv[s] = q < qend ? widen<TA>(ld_stream(val + q)) : TA(0);
-    }
+  for (int s = 0; s < NS; s++) {
+    const int64_t q = q0 + s * kWave + lane;
+    sj[s] = q < qend ? ld_stream(idx + q) : static_cast<TI>(base);
+    sv[s] = q < qend ? widen<TA>(ld_stream(val + q)) : TA(0);
   }
   int64_t r_lo, r_hi, pos_lo;
   FiberWindow<TP> win =
       tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
   const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
-  if constexpr (STAGE) {
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-      lds_j[s * kWave + lane] = sj[s];
-      lds_v[s * kWave + lane] = sv[s];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int s = 0; s < NS; s++) {
+    lds_j[s * kWave + lane] = sj[s];
+    lds_v[s * kWave + lane] = sv[s];
   }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
   const int64_t col0 = kofs + static_cast<int64_t>(gl) * KV;
   const int valid_cols = static_cast<int>(col0 < k ? (k - col0 < KV ? k - col0 : KV) : 0);
@@ -361,69 +349,32 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
     for (int c = 0; c < KV; c++) acc[c] = 0;
   };
 
-  if constexpr (STAGE) {
-    // Batches of U positions; the group's lanes read each position's
-    // coordinate and value from LDS (broadcast) and gather their B slices.
-    const TI* gj = lds_j + g * E;
-    const TA* gv = lds_v + g * E;
+  // Batches of U positions; the group's lanes read each position's
+  // coordinate and value from LDS (broadcast) and gather their B slices.
+  const TI* gj = lds_j + g * E;
+  const TA* gv = lds_v + g * E;
 #pragma unroll 1
-    for (int e0 = 0; e0 < E; e0 += U) {
-      TA bv[U][KV];
-      TA ve[U];
+  for (int e0 = 0; e0 < E; e0 += U) {
+    TA bv[U][KV];
+    TA ve[U];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
-        ve[u] = gv[e0 + u];
-        const bool in = (a + e0 + u < b) && valid_cols > 0;
-        if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
-        else {
+    for (int u = 0; u < U; u++) {
+      const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
+      ve[u] = gv[e0 + u];
+      const bool in = (a + e0 + u < b) && valid_cols > 0;
+      if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
+      else {
 #pragma unroll
-          for (int c = 0; c < KV; c++) bv[u][c] = 0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < U; u++) {
-        const int64_t q = a + e0 + u;
-        if (q < b) {
-          while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
-#pragma unroll
-          for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
-        }
+        for (int c = 0; c < KV; c++) bv[u][c] = 0;
       }
     }
-  } else {
-    // Chunks of G positions: lane gl of the group loads position a+c0+gl.
-#pragma unroll 1
-    for (int c0 = 0; c0 < E; c0 += G) {
-      const int64_t qc = a + c0 + gl;
-      const bool inc = qc < b;
-      const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + qc)) - base : 0;
-      const TA vc = inc ? widen<TA>(ld_stream(val + qc)) : TA(0);
 #pragma unroll
-      for (int e0 = 0; e0 < G; e0 += U) {
-        TA bv[U][KV];
-        TA ve[U];
+    for (int u = 0; u < U; u++) {
+      const int64_t q = a + e0 + u;
+      if (q < b) {
+        while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int src = gbase + ((e0 + u) % G);
-          const int64_t je = shfl(jc, src);
-          ve[u] = shfl(vc, src);
-          const bool in = (e0 + u < G) && (a + c0 + e0 + u < b) && valid_cols > 0;
-          if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
-          else {
-#pragma unroll
-            for (int c = 0; c < KV; c++) bv[u][c] = 0;
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const int64_t q = a + c0 + e0 + u;
-          if (e0 + u < G && q < b) {
-            while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
-#pragma unroll
-            for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
-          }
-        }
+        for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
       }
     }
   }

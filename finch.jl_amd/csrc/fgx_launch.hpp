@@ -27,6 +27,39 @@ static inline int launch_fixup(hipStream_t s, const int64_t* crow, const TA* cva
   return launch_check("fixup_kernel");
 }
 
+// dst[r * ldd + c] = src[r + c * lds] for r < rows, c < cols: a column-major
+// matrix to row-major (and, viewed transposed, row-major to column-major).
+// 64 x 64 tiles through LDS, both sides coalesced; a 1-D grid of tiles.
+template <typename T>
+__global__ __launch_bounds__(256) void col_to_row_kernel(const T* __restrict__ src, int64_t rows,
+                                                         int64_t cols, int64_t lds,
+                                                         T* __restrict__ dst, int64_t ldd) {
+  __shared__ T tile[64][65];
+  const int64_t ntr = (rows + 63) / 64;
+  const int64_t id = blockIdx.x;
+  const int64_t r0 = (id % ntr) * 64, c0 = (id / ntr) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + tx, c = c0 + i;
+    if (r < rows && c < cols) tile[i][tx] = src[r + c * lds];
+  }
+  __syncthreads();
+  for (int i = ty; i < 64; i += 4) {
+    const int64_t r = r0 + i, c = c0 + tx;
+    if (r < rows && c < cols) dst[r * ldd + c] = tile[tx][i];
+  }
+}
+
+template <typename T>
+static int col_to_row(const T* src, int64_t rows, int64_t cols, int64_t lds, T* dst, int64_t ldd,
+                      hipStream_t s) {
+  if (rows == 0 || cols == 0) return FGX_OK;
+  const int64_t tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
+  hipLaunchKernelGGL((col_to_row_kernel<T>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, s,
+                     src, rows, cols, lds, dst, ldd);
+  return launch_check("col_to_row_kernel");
+}
+
 }  // namespace fgx
 
 #define FGX_DISPATCH_VAL(vt, ...)                                              \

@@ -1,10 +1,12 @@
 // fgx_sddmm.hip — SDDMM: out[q] = A[q] * X[i,:].Y[j,:] on the pattern of A.
 //
-// Fast path (FP32 dense, row-major, width a multiple of 4): nnz-balanced wave
-// tiles of 4 x 64 positions, dot products on the matrix cores
-// (fgx_sddmm_core.hpp), X/Y rows staged through LDS.
+// Fast path (FP32 dense, width a multiple of 4): nnz-balanced wave tiles of
+// 4 x 64 positions, dot products on the matrix cores (fgx_sddmm_core.hpp),
+// Y rows staged through LDS.  Column-major X/Y (Finch's Dense(Dense)) are
+// first transposed to k-fast rows in the context's dense arena.
 //
-// General path (any layout, FP64): VALU dot products, one lane per position.
+// General path (FP64, other widths): VALU dot products, one lane per position.
+#include "fgx_launch.hpp"
 #include "fgx_sddmm_core.hpp"
 #include "fgx_internal.hpp"
 
@@ -63,6 +65,31 @@ static int sddmm_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const v
   TA* out = static_cast<TA*>(outv);
   bool mfma = false;
   if constexpr (sizeof(TA) == 4) {
+    if ((x_layout != FGX_ROW_MAJOR || y_layout != FGX_ROW_MAJOR) && k % 4 == 0) {
+      const int64_t n = A->n_inner;
+      const size_t xb = x_layout == FGX_ROW_MAJOR
+                            ? 0 : ((static_cast<size_t>(m * k) * sizeof(TA) + 255) & ~size_t(255));
+      const size_t yb = y_layout == FGX_ROW_MAJOR ? 0 : static_cast<size_t>(n * k) * sizeof(TA);
+      void* arena = nullptr;
+      FGX_TRY(dense_scratch(ctx, xb + yb, &arena));
+      const TA* Xf = X;
+      const TA* Yf = Y;
+      int64_t ldxf = ldx, ldyf = ldy;
+      if (xb) {
+        TA* Xt = static_cast<TA*>(arena);
+        FGX_TRY(col_to_row<TA>(X, m, k, ldx, Xt, k, s));
+        Xf = Xt;
+        ldxf = k;
+      }
+      if (yb) {
+        TA* Yt = reinterpret_cast<TA*>(static_cast<char*>(arena) + xb);
+        FGX_TRY(col_to_row<TA>(Y, n, k, ldy, Yt, k, s));
+        Yf = Yt;
+        ldyf = k;
+      }
+      return sddmm_typed<TP, TI, TV>(ctx, A, k, Xf, ldxf, FGX_ROW_MAJOR, Yf, ldyf, FGX_ROW_MAJOR,
+                                     outv, s);
+    }
     mfma = x_layout == FGX_ROW_MAJOR && y_layout == FGX_ROW_MAJOR && k % 4 == 0 && ldx % 4 == 0 &&
            ldy % 4 == 0 && aligned16(X) && aligned16(Y);
   }

@@ -12,20 +12,11 @@
 #ifndef FGX_SPMM_U
 #define FGX_SPMM_U 8
 #endif
-// Minimum waves per SIMD the register allocator is held to (0: no hint).
-#ifndef FGX_SPMM_WPE
-#define FGX_SPMM_WPE 0
-#endif
-#if FGX_SPMM_WPE > 0
-#define FGX_SPMM_OCC __attribute__((amdgpu_waves_per_eu(FGX_SPMM_WPE)))
-#else
-#define FGX_SPMM_OCC
-#endif
 
 namespace fgx {
 
 template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
-__global__ __launch_bounds__(256) FGX_SPMM_OCC void spmm_csr_kernel(
+__global__ __launch_bounds__(256) void spmm_csr_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, int64_t k, int64_t kofs, const TA* __restrict__ B,
     int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, const int64_t* __restrict__ table,
@@ -34,22 +25,17 @@ __global__ __launch_bounds__(256) FGX_SPMM_OCC void spmm_csr_kernel(
   const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
   if (tile >= ntiles) return;
-#if FGX_SPMM_STAGE
   constexpr int W = (kWave / G) * E;
   __shared__ TI lds_j[kWavesPerBlock][W];
   __shared__ TA lds_v[kWavesPerBlock][W];
   spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
                                          tile, table, lds[w], carry_row + tile,
                                          carry_val + tile * (G * KV), lds_j[w], lds_v[w]);
-#else
-  spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
-                                         tile, table, lds[w], carry_row + tile,
-                                         carry_val + tile * (G * KV));
-#endif
 }
 
 // General-layout SpMM (any strides): one wave per row, lanes over columns.
-// Serves column-major B/C (Finch's native Dense(Dense)) and odd widths.
+// Serves column-major B/C below width 4; wider ones are transposed to k-fast
+// rows first.
 template <typename TP, typename TI, typename TV, typename TA>
 __global__ __launch_bounds__(256) void spmm_csr_strided_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
@@ -67,39 +53,6 @@ __global__ __launch_bounds__(256) void spmm_csr_strided_kernel(
     }
     C[row * c_rs + c * c_cs] = acc;
   }
-}
-
-// dst[r * ldd + c] = src[r + c * lds] for r < rows, c < cols: a column-major
-// matrix to row-major (and, viewed transposed, row-major to column-major).
-// 64 x 64 tiles through LDS, both sides coalesced; a 1-D grid of tiles.
-template <typename T>
-__global__ __launch_bounds__(256) void col_to_row_kernel(const T* __restrict__ src, int64_t rows,
-                                                         int64_t cols, int64_t lds,
-                                                         T* __restrict__ dst, int64_t ldd) {
-  __shared__ T tile[64][65];
-  const int64_t ntr = (rows + 63) / 64;
-  const int64_t id = blockIdx.x;
-  const int64_t r0 = (id % ntr) * 64, c0 = (id / ntr) * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int i = ty; i < 64; i += 4) {
-    const int64_t r = r0 + tx, c = c0 + i;
-    if (r < rows && c < cols) tile[i][tx] = src[r + c * lds];
-  }
-  __syncthreads();
-  for (int i = ty; i < 64; i += 4) {
-    const int64_t r = r0 + i, c = c0 + tx;
-    if (r < rows && c < cols) dst[r * ldd + c] = tile[tx][i];
-  }
-}
-
-template <typename T>
-static int col_to_row(const T* src, int64_t rows, int64_t cols, int64_t lds, T* dst, int64_t ldd,
-                      hipStream_t s) {
-  if (rows == 0 || cols == 0) return FGX_OK;
-  const int64_t tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
-  hipLaunchKernelGGL((col_to_row_kernel<T>), dim3(static_cast<unsigned>(tiles)), dim3(256), 0, s,
-                     src, rows, cols, lds, dst, ldd);
-  return launch_check("col_to_row_kernel");
 }
 
 // SpMM dispatch over the dense width: lane groups of G lanes x KV columns.

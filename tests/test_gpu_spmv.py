@@ -181,7 +181,38 @@ def test_spmm_layouts_and_long_rows(dev, layout):
     assert rel_err(C, ref) <= FP32_NORMWISE
 
 
+@pytest.mark.parametrize("k", [3, 64, 100])
+def test_spmm_column_major_in_and_out(dev, k):
+    # Finch's Dense(Dense) operands are column-major on both sides: widths of
+    # 4 and more are transposed to k-fast rows, narrower ones run strided.
+    ptr, idx, val, m, n = EDGE["ragged"]
+    val = val.astype(np.float32)
+    B = synth.dense(n, k, seed=6)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    Bd = up(np.ascontiguousarray(B.T), dev).t()
+    Cd = torch.full((k, m), float("nan"), device=dev).t()
+    ops.spmm_csr(A, Bd, Cd)
+    ref = oracle.spmm_csr(ptr, idx, val.astype(np.float64), B.astype(np.float64))
+    assert rel_err(Cd.cpu().numpy(), ref) <= FP32_NORMWISE
+    if k >= 4:  # same bits as the row-major call
+        assert torch.equal(Cd.contiguous(), ops.spmm_csr(A, up(B, dev)))
+
+
 # ---- SDDMM ------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("k", [6, 32, 64])
+def test_sddmm_column_major(dev, k):
+    ptr, idx, val, m, n = EDGE["long_row"]
+    val = val.astype(np.float32)
+    X = synth.dense(m, k, seed=1)
+    Y = synth.dense(n, k, seed=2)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    Xc = up(np.ascontiguousarray(X.T), dev).t()
+    Yc = up(np.ascontiguousarray(Y.T), dev).t()
+    out = ops.sddmm_csr(A, Xc, Yc).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, val.astype(np.float64), X.astype(np.float64),
+                           Y.astype(np.float64))
+    assert rel_err(out, ref) <= FP32_NORMWISE
 
 @pytest.mark.parametrize("k", [4, 32, 36, 64, 100, 7])
 @pytest.mark.parametrize("name", ["ragged", "long_row", "sparse_rows"])
