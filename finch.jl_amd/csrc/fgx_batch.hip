@@ -26,13 +26,15 @@ __device__ __forceinline__ void fixup_item_tile(const BatchDesc& d, int64_t t,
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ fix, int64_t n_fix,
     const int64_t* __restrict__ seg_off, int64_t total, const int64_t* __restrict__ carry_row,
-    const float* __restrict__ carry_val, BatchDesc single, int use_single) {
+    const float* __restrict__ carry_val, SegInline inl) {
   const int w = wave_in_block();
   if (seg_off) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
     if (t >= total) return;
-    if (use_single) {  // one entry: its descriptor is an argument
-      fixup_item_tile(single, t, carry_row, carry_val);
+    if (inl.n > 0) {  // few entries: offsets and descriptors are arguments
+      int64_t u = 0;
+      for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= t ? 1 : 0;
+      fixup_item_tile(inl.desc[u], t - inl.seg[u], carry_row, carry_val);
       return;
     }
     const int64_t u = find_segment(seg_off, n_fix, t);
@@ -83,7 +85,7 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     const int64_t* seg = p->segmented ? p->d_segoff + p->cls_seg0[c] : nullptr;
     const int64_t total = p->segmented ? p->cls_total[c] : 0;
     if (p->segmented && total == 0) continue;
-    const BatchDesc* one = (p->segmented && w1 - w0 == 1) ? &p->cls_single[c] : nullptr;
+    const SegInline* one = (p->segmented && p->cls_inl[c].n > 0) ? &p->cls_inl[c] : nullptr;
     int rc;
     if (p->val_type == FGX_F32)
       rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, seg, total, one, s)
@@ -99,12 +101,11 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     const int64_t g = p->segmented
                           ? (p->fix_total + kWavesPerBlock - 1) / kWavesPerBlock
                           : std::min<int64_t>((p->n_fix + kWavesPerBlock - 1) / kWavesPerBlock, 4096);
-    const bool one = p->segmented && p->n_fix == 1;
+    const bool inl = p->segmented && p->fix_inl.n > 0;
     hipLaunchKernelGGL(batch_fixup_kernel, dim3(static_cast<unsigned>(g)), dim3(kBatchThreads), 0,
                        s, p->d_desc, p->d_fix, p->n_fix,
                        p->segmented ? p->d_segoff + p->fix_seg0 : nullptr, p->fix_total,
-                       p->d_carry_row, p->d_carry_val, one ? p->fix_single : BatchDesc{},
-                       one ? 1 : 0);
+                       p->d_carry_row, p->d_carry_val, inl ? p->fix_inl : SegInline{});
     FGX_TRY(launch_check("batch_fixup_kernel"));
   }
   return FGX_OK;
@@ -259,9 +260,17 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
     seg.push_back(acc);
     p->cls_total[c] = acc;
   }
-  for (int c = 0; c < p->n_cls; c++)
-    if (p->cls_w1[c] - p->cls_w0[c] == 1) p->cls_single[c] = h_desc[p->cls_w0[c]];
-  if (nf == 1) p->fix_single = h_desc[h_fix[0].item];
+  // Classes (and fix-up lists) with few entries pass them as kernel arguments.
+  for (int c = 0; c < p->n_cls; c++) {
+    SegInline& in = p->cls_inl[c];
+    const int64_t n = p->cls_w1[c] - p->cls_w0[c];
+    in.n = n <= kSegInline ? n : 0;
+    for (int64_t e = 0; e < in.n; e++) {
+      in.seg[e] = seg[static_cast<size_t>(p->cls_seg0[c] + e)];
+      in.desc[e] = h_desc[p->cls_w0[c] + e];
+    }
+  }
+  p->fix_inl.n = nf <= kSegInline ? nf : 0;
   p->fix_seg0 = static_cast<int64_t>(seg.size());
   int64_t facc = 0;
   for (int64_t u = 0; u < nf; u++) {
@@ -270,6 +279,10 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   }
   seg.push_back(facc);
   p->fix_total = facc;
+  for (int64_t e = 0; e < p->fix_inl.n; e++) {
+    p->fix_inl.seg[e] = seg[static_cast<size_t>(p->fix_seg0 + e)];
+    p->fix_inl.desc[e] = h_desc[h_fix[e].item];
+  }
   // One pinned staging buffer, one copy: [desc | work | fix | seg_off].
   const size_t o_desc = 0;
   const size_t o_work = align16(o_desc + static_cast<size_t>(nd) * sizeof(BatchDesc));
@@ -305,12 +318,11 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   if (nw > 0) std::memcpy(h + o_work, h_work, static_cast<size_t>(nw) * sizeof(BatchWork));
   if (nf > 0) std::memcpy(h + o_fix, h_fix, static_cast<size_t>(nf) * sizeof(BatchWork));
   std::memcpy(h + o_seg, seg.data(), seg.size() * sizeof(int64_t));
-  // When every class (and the fix-up) has a single entry, the kernels take
-  // their descriptors as arguments and read nothing staged: skip the copy.
-  bool all_single = nf <= 1;
-  for (int c = 0; c < p->n_cls && all_single; c++)
-    all_single = p->cls_w1[c] - p->cls_w0[c] == 1;
-  if (!all_single)
+  // When every class (and the fix-up) passes its entries as arguments, the
+  // kernels read nothing staged: skip the copy.
+  bool all_inline = nf == 0 || p->fix_inl.n > 0;
+  for (int c = 0; c < p->n_cls && all_inline; c++) all_inline = p->cls_inl[c].n > 0;
+  if (!all_inline)
     FGX_HIP(hipMemcpyAsync(p->d_stage, p->h_stage, bytes, hipMemcpyHostToDevice, s));
   char* d = static_cast<char*>(p->d_stage);
   p->d_desc = reinterpret_cast<BatchDesc*>(d + o_desc);

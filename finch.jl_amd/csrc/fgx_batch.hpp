@@ -40,6 +40,16 @@ struct BatchDesc {
   int32_t pad;
 };
 
+// Entries of a segmented launch passed by value as a kernel argument (when a
+// class has at most kSegInline of them), so a wave finds its entry and
+// descriptor without a memory round trip.  n == 0: read them from memory.
+constexpr int kSegInline = 12;
+struct SegInline {
+  int64_t n;
+  int64_t seg[kSegInline + 1];  // tiles before entry e
+  BatchDesc desc[kSegInline];
+};
+
 struct BatchWork {
   int32_t item;
   int32_t kind;
@@ -99,8 +109,8 @@ struct fgx_batch_plan {
   int64_t cls_seg0[fgx::kMaxClasses] = {};        // class c's prefix array in d_segoff
   int64_t cls_total[fgx::kMaxClasses] = {};       // tiles of class c
   int64_t fix_seg0 = 0, fix_total = 0;            // the fix-up entries' prefix array
-  fgx::BatchDesc cls_single[fgx::kMaxClasses] = {};  // class c's descriptor if it has one entry
-  fgx::BatchDesc fix_single = {};                  // the fix-up entry's, if there is one
+  fgx::SegInline cls_inl[fgx::kMaxClasses] = {};  // class c's entries, if few enough
+  fgx::SegInline fix_inl = {};                     // the fix-up entries, if few enough
   void* d_stage = nullptr;
   void* h_stage = nullptr;                        // pinned
   int64_t cap_stage = 0, cap_hstage = 0;

@@ -224,17 +224,20 @@ template <typename TP, typename TI, typename TV, int CLS, bool SEG>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t w0,
     int64_t w1, const int64_t* __restrict__ seg_off, int64_t total,
-    int64_t* __restrict__ carry_row, float* __restrict__ carry_val, BatchDesc single,
-    int use_single) {
+    int64_t* __restrict__ carry_row, float* __restrict__ carry_val, SegInline inl) {
   __shared__ __align__(16) unsigned char smem[kWavesPerBlock][lds_bytes_for<CLS>()];
   const int w = wave_in_block();
   if constexpr (SEG) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
     if (t >= total) return;
-    if (use_single) {
-      // A class with one entry (e.g. a large request launched alone) gets its
-      // descriptor as a kernel argument, like the single-operand kernels.
-      run_unit<TP, TI, TV, CLS>(single, single.kind, t, t + 1, smem[w], carry_row, carry_val);
+    if (inl.n > 0) {
+      // Few entries (e.g. a large request launched alone): offsets and
+      // descriptors are kernel arguments, read from the kernarg segment.
+      int64_t u = 0;
+      for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= t ? 1 : 0;
+      const int64_t b = t - inl.seg[u];
+      run_unit<TP, TI, TV, CLS>(inl.desc[u], inl.desc[u].kind, b, b + 1, smem[w], carry_row,
+                                carry_val);
       return;
     }
     // Segmented entries start at 0 and desc[u] is entry u's descriptor.
@@ -265,7 +268,7 @@ inline int64_t persistent_grid(int64_t n_work, int num_cus, int blocks_per_cu) {
 
 template <typename TP, typename TI, typename TV, int CLS, bool SEG>
 int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off, int64_t total,
-                const BatchDesc* single, hipStream_t s) {
+                const SegInline* inl, hipStream_t s) {
   int64_t grid;
   if constexpr (SEG) {
     grid = std::max<int64_t>(1, (total + kWavesPerBlock - 1) / kWavesPerBlock);
@@ -288,35 +291,34 @@ int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_of
   p->grid = std::max<int64_t>(p->grid, grid);
   hipLaunchKernelGGL((batch_kernel<TP, TI, TV, CLS, SEG>), dim3(static_cast<unsigned>(grid)),
                      dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, w0, w1, seg_off, total,
-                     p->d_carry_row, p->d_carry_val, single ? *single : BatchDesc{},
-                     single ? 1 : 0);
+                     p->d_carry_row, p->d_carry_val, inl ? *inl : SegInline{});
   return launch_check("batch_kernel");
 }
 
 template <typename TP, typename TI, typename TV, int CLS>
 int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off,
-                 int64_t total, const BatchDesc* single, hipStream_t s) {
-  return seg_off ? launch_form<TP, TI, TV, CLS, true>(p, w0, w1, seg_off, total, single, s)
+                 int64_t total, const SegInline* inl, hipStream_t s) {
+  return seg_off ? launch_form<TP, TI, TV, CLS, true>(p, w0, w1, seg_off, total, inl, s)
                  : launch_form<TP, TI, TV, CLS, false>(p, w0, w1, nullptr, 0, nullptr, s);
 }
 
-// Launchers of one value type: NAME(p, cls, w0, w1, seg_off, total, single, s)
+// Launchers of one value type: NAME(p, cls, w0, w1, seg_off, total, inl, s)
 // runs class `cls` over work entries [w0, w1) for the plan's index types
-// (`single`: the descriptor of a one-entry segmented class, or null).
+// (`inl`: the entries of a segmented class passed by value, or null).
 #define FGX_BATCH_DEFINE_SPMM(NAME, TYPE)                                                   \
   int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
-           int64_t total, const BatchDesc* single, hipStream_t s) {                         \
+           int64_t total, const SegInline* inl, hipStream_t s) {                            \
     using TV = TYPE;                                                                        \
     int rc = set_error(FGX_EUNSUP, "batch: unit class %d", cls);                           \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo), {                                     \
       switch (cls) {                                                                        \
-        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, seg_off, total, single, s); break;        \
-        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, seg_off, total, single, s); break;                      \
-        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, seg_off, total, single, s); break;                      \
-        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, seg_off, total, single, s); break;                      \
-        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, seg_off, total, single, s); break;                    \
-        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, seg_off, total, single, s); break;                    \
-        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, seg_off, total, single, s); break;                    \
+        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, seg_off, total, inl, s); break;        \
+        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, seg_off, total, inl, s); break;                    \
         default: break;                                                                     \
       }                                                                                     \
     });                                                                                     \
@@ -326,18 +328,18 @@ int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_o
 
 #define FGX_BATCH_DEFINE_SDDMM(NAME, TYPE)                                                  \
   int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
-           int64_t total, const BatchDesc* single, hipStream_t s) {                         \
+           int64_t total, const SegInline* inl, hipStream_t s) {                            \
     using TV = TYPE;                                                                        \
     if (cls != kClsSddmm) return set_error(FGX_EUNSUP, "batch: unit class %d", cls);       \
     int rc = FGX_OK;                                                                        \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo),                                       \
-                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, seg_off, total, single, s)));             \
+                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, seg_off, total, inl, s)));             \
     return rc;                                                                              \
   }
 
 #define FGX_BATCH_LAUNCH_ARGS \
   fgx_batch_plan *p, int cls, int64_t w0, int64_t w1, const int64_t *seg_off, int64_t total, \
-      const BatchDesc *single, hipStream_t s
+      const SegInline *inl, hipStream_t s
 int batch_spmm_f32(FGX_BATCH_LAUNCH_ARGS);
 int batch_sddmm_f32(FGX_BATCH_LAUNCH_ARGS);
 int batch_spmm_bf16(FGX_BATCH_LAUNCH_ARGS);
