@@ -42,12 +42,14 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
         jj[u] = q < q1 ? ld_stream(idx + q) : static_cast<TI>(base);
         vv[u] = q < q1 ? ld_stream(val + q) : TV{};
       }
+      // Unconditional gathers (past-the-row slots read x[0] and are not
+      // accumulated), so they issue back to back.
       float xg[S];
 #pragma unroll
-      for (int u = 0; u < S; u++)
-        xg[u] = c + gl + u * G < q1 ? d.B[static_cast<int64_t>(jj[u]) - base] : 0.f;
+      for (int u = 0; u < S; u++) xg[u] = d.B[static_cast<int64_t>(jj[u]) - base];
 #pragma unroll
-      for (int u = 0; u < S; u++) acc += widen<float>(vv[u]) * xg[u];
+      for (int u = 0; u < S; u++)
+        if (c + gl + u * G < q1) acc += widen<float>(vv[u]) * xg[u];
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);

@@ -133,15 +133,21 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
     }
   }
   // Gather x (independent loads, issued together), then locate the tile's
-  // fibres while the gathers are in flight.
-  // (The loaded values are used only after the set-up, so the set-up does not
-  // wait for them.)
+  // fibres while the gathers are in flight.  The gathers are unconditional:
+  // positions past the run read x[0] (valid whenever the tile holds a
+  // position) and are never accumulated.  A load under a per-position branch
+  // makes the compiler wait for every earlier load before issuing the next.
   TA xg[V];
+  if (qend > q0) {
 #pragma unroll
-  for (int s = 0; s < V; s++) {
-    const bool in = a + s < b;
-    xg[s] = TA(0);
-    if (in) xg[s] = x[static_cast<int64_t>(jj[s]) - base];
+    for (int s = 0; s < V; s++) {
+      const bool in = a + s < b;
+      const int64_t j = in ? static_cast<int64_t>(jj[s]) - base : 0;
+      xg[s] = x[j];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < V; s++) xg[s] = TA(0);
   }
   int64_t r_lo, r_hi, pos_lo;
   FiberWindow<TP> win =
