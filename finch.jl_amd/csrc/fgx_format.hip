@@ -1,0 +1,616 @@
+// fgx_format.hip — format conversion on the device.
+//
+//   * fgx_transpose: a compressed matrix stored along its other dimension
+//     (CSC <-> CSR), every stored entry kept, each output fibre sorted;
+//   * fgx_coo_to_compressed: coordinate lists -> Dense(SparseList), ordered
+//     by (outer, inner), duplicates combined with + in input order;
+//   * fgx_dropfills: the stored entries whose value differs from the fill.
+//
+// All three are deterministic.  The ordering step is a stable LSD radix sort
+// written for wave64: 8-bit digits; per pass, per-tile digit counts, one
+// exclusive scan of the (digit, tile) table, then a stable scatter whose
+// ranks inside a tile come from one ballot per digit bit.  No atomics decide
+// where an element goes, so a run reproduces bit for bit.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+
+#include "fgx_device.hpp"
+#include "fgx_internal.hpp"
+#include "fgx_scan.hpp"
+
+namespace fgx {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int64_t kSortTile = static_cast<int64_t>(kSortThreads) * kSortItems;
+constexpr int kDigitBits = 8;
+constexpr int kBins = 1 << kDigitBits;
+constexpr int kSortWaves = kSortThreads / kWave;
+static_assert(kSortThreads == kBins, "the digit tables give one digit to each thread");
+
+// Raw value words: values are moved, never interpreted, by the sort.
+template <int B> struct word_of;
+template <> struct word_of<2> { using type = uint16_t; };
+template <> struct word_of<4> { using type = uint32_t; };
+template <> struct word_of<8> { using type = uint64_t; };
+
+template <typename T>
+__device__ __forceinline__ uint32_t rebased(const T* __restrict__ a, int64_t base, int64_t e) {
+  return static_cast<uint32_t>(static_cast<int64_t>(a[e]) - base);
+}
+
+// Digit counts of one tile of keys: hist[d * ntiles + tile].
+template <typename TK>
+__global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(const TK* __restrict__ keys,
+                                                                 int64_t kbase, int64_t n,
+                                                                 int shift, int64_t ntiles,
+                                                                 int64_t* __restrict__ hist) {
+  __shared__ uint32_t h[kBins];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+#pragma unroll 4
+  for (int i = 0; i < kSortItems; i++) {
+    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + threadIdx.x;
+    if (e < n) atomicAdd(&h[(rebased(keys, kbase, e) >> shift) & (kBins - 1)], 1u);
+  }
+  __syncthreads();
+  hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter of one tile.  Element e (key digit d) goes to
+// offs[d * ntiles + tile] plus the number of elements of the tile before e
+// with digit d.  The tile is read in rounds of kSortThreads consecutive
+// elements (element order = round, then thread); in a round, the lanes of a
+// wave that share e's digit are found with kDigitBits ballots, and the waves
+// are ordered through the per-wave digit counts in LDS.  The whole tile is
+// loaded before the first round, so its loads are in flight together.
+template <typename TK, typename TQ, typename TW, typename TQO>
+__global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
+    const TK* __restrict__ keys, int64_t kbase, const TQ* __restrict__ pay, int64_t pbase,
+    const TW* __restrict__ val, int64_t n, int shift, int64_t ntiles,
+    const int64_t* __restrict__ offs, uint32_t* __restrict__ keys_out, TQO* __restrict__ pay_out,
+    int64_t pout_base, TW* __restrict__ val_out) {
+  __shared__ int64_t run[kBins];
+  __shared__ uint32_t wcnt[kSortWaves][kBins];
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int w = wave_in_block();
+  run[tid] = offs[static_cast<int64_t>(tid) * ntiles + blockIdx.x];
+#pragma unroll
+  for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0;
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  uint32_t k[kSortItems], p[kSortItems];
+  TW x[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
+    if (e < n) {
+      k[i] = rebased(keys, kbase, e);
+      p[i] = rebased(pay, pbase, e);
+      x[i] = val[e];
+    } else {
+      k[i] = 0;
+      p[i] = 0;
+      x[i] = TW(0);
+    }
+  }
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
+    const bool in = e < n;
+    const uint32_t d = (k[i] >> shift) & (kBins - 1);
+    uint64_t same = ballot(in);
+#pragma unroll
+    for (int b = 0; b < kDigitBits; b++) {
+      const bool bit = ((d >> b) & 1u) != 0;
+      const uint64_t m = ballot(bit);
+      same &= bit ? m : ~m;
+    }
+    const int rank = popc(same & below);
+    if (in && rank == 0) wcnt[w][d] = static_cast<uint32_t>(popc(same));
+    __syncthreads();
+    int64_t pos = run[d] + rank;
+    for (int v = 0; v < w; v++) pos += wcnt[v][d];
+    uint32_t tot = 0;
+#pragma unroll
+    for (int v = 0; v < kSortWaves; v++) tot += wcnt[v][tid];
+    __syncthreads();
+    run[tid] += tot;
+#pragma unroll
+    for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0;
+    if (in) {
+      keys_out[pos] = k[i];
+      pay_out[pos] = static_cast<TQO>(static_cast<int64_t>(p[i]) + pout_base);
+      val_out[pos] = x[i];
+    }
+    __syncthreads();
+  }
+}
+
+// fib[q] = the fibre that owns stored position q.  Each thread finds the
+// fibre of its first position by binary search over ptr and walks forward
+// over kRun consecutive positions.
+constexpr int kExpandRun = 8;
+template <typename TP>
+__global__ __launch_bounds__(256) void expand_fibres_kernel(const TP* __restrict__ ptr,
+                                                            int64_t base, int64_t n_outer,
+                                                            int64_t nnz, uint32_t* __restrict__ fib) {
+  const int64_t q0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kExpandRun;
+  if (q0 >= nnz) return;
+  int64_t lo = 0, hi = n_outer - 1;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (pos_at(ptr, base, mid + 1) <= q0) lo = mid + 1; else hi = mid;
+  }
+  int64_t o = lo, end = pos_at(ptr, base, o + 1);
+  uint32_t f[kExpandRun];
+#pragma unroll
+  for (int i = 0; i < kExpandRun; i++) {
+    const int64_t q = q0 + i;
+    while (q >= end && o + 1 < n_outer) {
+      o++;
+      end = pos_at(ptr, base, o + 1);
+    }
+    f[i] = static_cast<uint32_t>(o);
+  }
+  if (q0 + kExpandRun <= nnz) {
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    v4u* dst = reinterpret_cast<v4u*>(fib + q0);
+    dst[0] = v4u{f[0], f[1], f[2], f[3]};
+    dst[1] = v4u{f[4], f[5], f[6], f[7]};
+  } else {
+    for (int i = 0; i < kExpandRun && q0 + i < nnz; i++) fib[q0 + i] = f[i];
+  }
+}
+
+// ptr[j] = base + (first position p with keys[p] >= j) for j in [0, nk],
+// from keys sorted ascending: position p writes the fibres that start at p
+// (those after keys[p-1] up to keys[p]).  Keys are clamped to nk so that no
+// write leaves ptr.
+template <typename TP>
+__global__ __launch_bounds__(256) void ptr_from_keys_kernel(const uint32_t* __restrict__ keys,
+                                                            int64_t n, int64_t nk, int64_t base,
+                                                            TP* __restrict__ ptr) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p <= n;
+       p += stride) {
+    const int64_t lo = p == 0 ? 0 : std::min<int64_t>(keys[p - 1], nk) + 1;
+    const int64_t hi = p == n ? nk : std::min<int64_t>(keys[p], nk);
+    for (int64_t j = lo; j <= hi; j++) ptr[j] = static_cast<TP>(base + p);
+  }
+}
+
+// flag[0] |= 1 if any coordinate lies outside [base, base + dim).
+template <typename TI>
+__global__ __launch_bounds__(256) void coord_check_kernel(const TI* __restrict__ a, int64_t n,
+                                                          int64_t base, int64_t dim,
+                                                          int* __restrict__ flag) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  bool bad = false;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+    const int64_t c = static_cast<int64_t>(a[e]) - base;
+    bad = bad || c < 0 || c >= dim;
+  }
+  if (ballot(bad) != 0 && lane_id() == 0) atomicOr(flag, 1);
+}
+
+// head[p] = 1 where sorted entry p starts a run of equal (outer, inner).
+__global__ __launch_bounds__(256) void run_heads_kernel(const uint32_t* __restrict__ ko,
+                                                        const uint32_t* __restrict__ ki, int64_t n,
+                                                        int64_t* __restrict__ head) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+    head[p] = (p == 0 || ko[p] != ko[p - 1] || ki[p] != ki[p - 1]) ? 1 : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T combine_add(T a, T b) { return a + b; }
+template <>
+__device__ __forceinline__ int32_t combine_add<int32_t>(int32_t a, int32_t b) {
+  return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b));
+}
+
+// One thread per run head: the run's values summed left to right (foldl),
+// written at the run's rank u = uidx[p].
+template <typename TI, typename TV>
+__global__ __launch_bounds__(256) void combine_runs_kernel(
+    const uint32_t* __restrict__ ko, const uint32_t* __restrict__ ki, const TV* __restrict__ v,
+    const int64_t* __restrict__ uidx, int64_t n, int64_t base, uint32_t* __restrict__ ou,
+    TI* __restrict__ c_idx, TV* __restrict__ c_val) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride) {
+    const int64_t u = uidx[p];
+    if (uidx[p + 1] == u) continue;  // not a head
+    TV acc = v[p];
+    for (int64_t s = p + 1; s < n && uidx[s + 1] == uidx[s]; s++) acc = combine_add(acc, v[s]);
+    ou[u] = ko[p];
+    c_idx[u] = static_cast<TI>(static_cast<int64_t>(ki[p]) + base);
+    c_val[u] = acc;
+  }
+}
+
+// isequal(v, fill): NaN equals NaN; -0.0 differs from 0.0 (bits compared).
+__device__ __forceinline__ bool is_fill(double v, double f) {
+  if (v != v || f != f) return (v != v) && (f != f);
+  return __double_as_longlong(v) == __double_as_longlong(f);
+}
+__device__ __forceinline__ bool is_fill(float v, float f) {
+  if (v != v || f != f) return (v != v) && (f != f);
+  return __float_as_uint(v) == __float_as_uint(f);
+}
+__device__ __forceinline__ bool is_fill(int32_t v, int32_t f) { return v == f; }
+
+template <typename TV>
+__global__ __launch_bounds__(256) void keep_flags_kernel(const TV* __restrict__ val, int64_t n,
+                                                         TV fill, int64_t* __restrict__ keep) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
+    keep[q] = is_fill(val[q], fill) ? 0 : 1;
+}
+
+// c_ptr[o] = base + (kept entries before fibre o).
+template <typename TP>
+__global__ __launch_bounds__(256) void kept_ptr_kernel(const TP* __restrict__ ptr, int64_t base,
+                                                       int64_t n_outer,
+                                                       const int64_t* __restrict__ kept,
+                                                       TP* __restrict__ c_ptr) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; o <= n_outer;
+       o += stride)
+    c_ptr[o] = static_cast<TP>(base + kept[pos_at(ptr, base, o)]);
+}
+
+template <typename TI, typename TV>
+__global__ __launch_bounds__(256) void compact_kernel(const TI* __restrict__ idx,
+                                                      const TV* __restrict__ val, int64_t n,
+                                                      const int64_t* __restrict__ kept,
+                                                      TI* __restrict__ c_idx, TV* __restrict__ c_val) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
+    const int64_t u = kept[q];
+    if (kept[q + 1] != u) {
+      c_idx[u] = idx[q];
+      c_val[u] = val[q];
+    }
+  }
+}
+
+inline unsigned grid_for(int64_t n, int64_t per_block, int64_t cap = 65536) {
+  const int64_t b = (n + per_block - 1) / per_block;
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(b, cap)));
+}
+
+// Bits needed for keys in [0, nk).
+inline int key_bits(int64_t nk) {
+  int b = 0;
+  while (b < 63 && (int64_t(1) << b) < nk) b++;
+  return b;
+}
+
+// Carves 256-byte aligned pieces out of one scratch block.
+struct Carver {
+  char* p;
+  size_t used = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    T* r = reinterpret_cast<T*>(p + used);
+    used += (static_cast<size_t>(std::max<int64_t>(count, 1)) * sizeof(T) + 255) & ~size_t(255);
+    return r;
+  }
+};
+
+// Scratch of a sort over n elements with TW value words (two ping-pong sets
+// of keys / payloads / values, the digit table and its scan).
+template <typename TW>
+size_t sort_scratch_bytes(int64_t n) {
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const int64_t cells = static_cast<int64_t>(kBins) * ntiles;
+  const size_t per = 2 * (static_cast<size_t>(n) * (4 + 4 + sizeof(TW)) + 3 * 256);
+  return per + 2 * (static_cast<size_t>(cells + 1) * 8 + 256) + scan_work_bytes(cells) + 256;
+}
+
+template <typename TW>
+struct SortBufs {
+  uint32_t* k[2];
+  uint32_t* q[2];
+  TW* v[2];
+  int64_t* hist;
+  int64_t* offs;
+  void* work;
+  void carve(Carver& c, int64_t n) {
+    const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const int64_t cells = static_cast<int64_t>(kBins) * ntiles;
+    for (int i = 0; i < 2; i++) {
+      k[i] = c.take<uint32_t>(n);
+      q[i] = c.take<uint32_t>(n);
+      v[i] = c.take<TW>(n);
+    }
+    hist = c.take<int64_t>(cells + 1);
+    offs = c.take<int64_t>(cells + 1);
+    work = c.take<char>(static_cast<int64_t>(scan_work_bytes(cells)));
+  }
+};
+
+// One radix pass on digit `shift`: counts, scan, stable scatter.
+template <typename TK, typename TQ, typename TW, typename TQO>
+int sort_pass(const TK* keys, int64_t kbase, const TQ* pay, int64_t pbase, const TW* val, int64_t n,
+              int shift, SortBufs<TW>& S, uint32_t* kout, TQO* pout, int64_t pout_base, TW* vout,
+              hipStream_t s) {
+  const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  hipLaunchKernelGGL((sort_hist_kernel<TK>), dim3(static_cast<unsigned>(ntiles)),
+                     dim3(kSortThreads), 0, s, keys, kbase, n, shift, ntiles, S.hist);
+  FGX_TRY(launch_check("sort_hist_kernel"));
+  FGX_TRY(scan_counts_to_ptr<int64_t>(S.hist, static_cast<int64_t>(kBins) * ntiles, 0, S.offs,
+                                      S.work, nullptr, nullptr, s));
+  hipLaunchKernelGGL((sort_scatter_kernel<TK, TQ, TW, TQO>), dim3(static_cast<unsigned>(ntiles)),
+                     dim3(kSortThreads), 0, s, keys, kbase, pay, pbase, val, n, shift, ntiles,
+                     S.offs, kout, pout, pout_base, vout);
+  return launch_check("sort_scatter_kernel");
+}
+
+// Stable sort of n (key, payload, value) triples by key in [0, 2^bits).
+// Keys and payloads are read as (a[e] - base); the sorted keys go to kout,
+// the payloads to pout (+ pout_base, as TQO) and the values to vout.
+template <typename TK, typename TQ, typename TW, typename TQO>
+int radix_sort(const TK* keys, int64_t kbase, const TQ* pay, int64_t pbase, const TW* val,
+               int64_t n, int bits, SortBufs<TW>& S, uint32_t* kout, TQO* pout, int64_t pout_base,
+               TW* vout, hipStream_t s) {
+  const int passes = std::max(1, (bits + kDigitBits - 1) / kDigitBits);
+  if (passes == 1)
+    return sort_pass<TK, TQ, TW, TQO>(keys, kbase, pay, pbase, val, n, 0, S, kout, pout,
+                                      pout_base, vout, s);
+  FGX_TRY((sort_pass<TK, TQ, TW, uint32_t>(keys, kbase, pay, pbase, val, n, 0, S, S.k[0], S.q[0],
+                                           0, S.v[0], s)));
+  int cur = 0;
+  for (int pass = 1; pass < passes - 1; pass++) {
+    FGX_TRY((sort_pass<uint32_t, uint32_t, TW, uint32_t>(S.k[cur], 0, S.q[cur], 0, S.v[cur], n,
+                                                          pass * kDigitBits, S, S.k[cur ^ 1],
+                                                          S.q[cur ^ 1], 0, S.v[cur ^ 1], s)));
+    cur ^= 1;
+  }
+  return sort_pass<uint32_t, uint32_t, TW, TQO>(S.k[cur], 0, S.q[cur], 0, S.v[cur], n,
+                                                (passes - 1) * kDigitBits, S, kout, pout,
+                                                pout_base, vout, s);
+}
+
+template <typename TP>
+int write_ptr_from_keys(const uint32_t* keys, int64_t n, int64_t nk, int64_t base, TP* ptr,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((ptr_from_keys_kernel<TP>), dim3(grid_for(n + 1, 256)), dim3(256), 0, s, keys,
+                     n, nk, base, ptr);
+  return launch_check("ptr_from_keys_kernel");
+}
+
+// ---- transpose ------------------------------------------------------------------------------
+template <typename TP, typename TI, int VB>
+int transpose_typed(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_idx, void* t_val,
+                    hipStream_t s) {
+  using TW = typename word_of<VB>::type;
+  const int64_t n = A->nnz, base = A->index_base;
+  if (n == 0) return write_ptr_from_keys<TP>(nullptr, 0, A->n_inner, base, static_cast<TP*>(t_ptr), s);
+  const size_t bytes = sort_scratch_bytes<TW>(n) + 2 * (static_cast<size_t>(n) * 4 + 256);
+  void* scr;
+  FGX_TRY(scratch(ctx, bytes, &scr));
+  Carver c{static_cast<char*>(scr)};
+  uint32_t* fib = c.take<uint32_t>(n);
+  uint32_t* kfin = c.take<uint32_t>(n);
+  SortBufs<TW> S;
+  S.carve(c, n);
+  hipLaunchKernelGGL((expand_fibres_kernel<TP>), dim3(grid_for(n, 256 * kExpandRun, INT32_MAX)),
+                     dim3(256), 0, s, static_cast<const TP*>(A->ptr), base, A->n_outer, n, fib);
+  FGX_TRY(launch_check("expand_fibres_kernel"));
+  FGX_TRY((radix_sort<TI, uint32_t, TW, TI>(static_cast<const TI*>(A->idx), base, fib, 0,
+                                            static_cast<const TW*>(A->val), n,
+                                            key_bits(A->n_inner), S, kfin,
+                                            static_cast<TI*>(t_idx), base,
+                                            static_cast<TW*>(t_val), s)));
+  return write_ptr_from_keys<TP>(kfin, n, A->n_inner, base, static_cast<TP*>(t_ptr), s);
+}
+
+// ---- COO -> compressed ---------------------------------------------------------------------
+template <typename TI, typename TV>
+int coo_typed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t n, int64_t base,
+              const TI* outer, const TI* inner, const TV* val, TI* c_ptr, TI* c_idx, TV* c_val,
+              int64_t* c_nnz, hipStream_t s) {
+  if (n == 0) {
+    *c_nnz = 0;
+    return write_ptr_from_keys<TI>(nullptr, 0, n_outer, base, c_ptr, s);
+  }
+  const size_t n4 = static_cast<size_t>(n) * 4 + 256, nv = static_cast<size_t>(n) * sizeof(TV) + 256;
+  const size_t bytes = sort_scratch_bytes<TV>(n) + 5 * n4 + 2 * nv +
+                       (2 * static_cast<size_t>(n) + 1) * 8 + 512 + scan_work_bytes(n) + 1024;
+  void* scr;
+  FGX_TRY(scratch(ctx, bytes, &scr));
+  Carver c{static_cast<char*>(scr)};
+  uint32_t* k1 = c.take<uint32_t>(n);  // inner, after the sort by inner
+  uint32_t* q1 = c.take<uint32_t>(n);  // outer, after the sort by inner
+  TV* v1 = c.take<TV>(n);
+  uint32_t* ko = c.take<uint32_t>(n);  // outer, in (outer, inner) order
+  uint32_t* ki = c.take<uint32_t>(n);  // inner, in (outer, inner) order
+  TV* v2 = c.take<TV>(n);
+  uint32_t* ou = c.take<uint32_t>(n);  // outer of each distinct coordinate
+  int64_t* head = c.take<int64_t>(n);
+  int64_t* uidx = c.take<int64_t>(n + 1);
+  void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
+  int* flag = c.take<int>(1);
+  SortBufs<TV> S;
+  S.carve(c, n);
+  // Coordinates outside the shape are an argument error (fsparse requires
+  // 1 <= I[n][q] <= M[n]); checked before anything is placed.
+  FGX_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+  hipLaunchKernelGGL((coord_check_kernel<TI>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s,
+                     outer, n, base, n_outer, flag);
+  FGX_TRY(launch_check("coord_check_kernel"));
+  hipLaunchKernelGGL((coord_check_kernel<TI>), dim3(grid_for(n, 256, 4096)), dim3(256), 0, s,
+                     inner, n, base, n_inner, flag);
+  FGX_TRY(launch_check("coord_check_kernel"));
+  int* h_flag = reinterpret_cast<int*>(ctx->host_i64);
+  FGX_HIP(hipMemcpyAsync(h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+  FGX_HIP(hipStreamSynchronize(s));
+  if (*h_flag) return set_error(FGX_EDIM, "fgx_coo_to_compressed: coordinate outside the shape");
+  // Stable by inner, then stable by outer: (outer, inner) order with ties
+  // (duplicate coordinates) in input order.
+  FGX_TRY((radix_sort<TI, TI, TV, uint32_t>(inner, base, outer, base, val, n, key_bits(n_inner), S,
+                                            k1, q1, 0, v1, s)));
+  FGX_TRY((radix_sort<uint32_t, uint32_t, TV, uint32_t>(q1, 0, k1, 0, v1, n, key_bits(n_outer), S,
+                                                         ko, ki, 0, v2, s)));
+  hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, ko, ki, n, head);
+  FGX_TRY(launch_check("run_heads_kernel"));
+  int64_t u = 0;
+  FGX_TRY(scan_counts_to_ptr<int64_t>(head, n, 0, uidx, work, ctx->host_i64, &u, s));
+  hipLaunchKernelGGL((combine_runs_kernel<TI, TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, ko, ki,
+                     v2, uidx, n, base, ou, c_idx, c_val);
+  FGX_TRY(launch_check("combine_runs_kernel"));
+  FGX_TRY(write_ptr_from_keys<TI>(ou, u, n_outer, base, c_ptr, s));
+  *c_nnz = u;
+  return FGX_OK;
+}
+
+// ---- dropfills ------------------------------------------------------------------------------
+template <typename TP, typename TI, typename TV>
+int dropfills_typed(fgx_ctx* ctx, const fgx_compressed* A, TV fill, void* c_ptr, void* c_idx,
+                    void* c_val, int64_t* c_nnz, hipStream_t s) {
+  const int64_t n = A->nnz, base = A->index_base;
+  if (n == 0) {
+    *c_nnz = 0;
+    return write_ptr_from_keys<TP>(nullptr, 0, A->n_outer, base, static_cast<TP*>(c_ptr), s);
+  }
+  void* scr;
+  FGX_TRY(scratch(ctx, (2 * static_cast<size_t>(n) + 1) * 8 + 1024 + scan_work_bytes(n), &scr));
+  Carver c{static_cast<char*>(scr)};
+  int64_t* keep = c.take<int64_t>(n);
+  int64_t* kept = c.take<int64_t>(n + 1);
+  void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
+  const TV* val = static_cast<const TV*>(A->val);
+  hipLaunchKernelGGL((keep_flags_kernel<TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, val, n, fill,
+                     keep);
+  FGX_TRY(launch_check("keep_flags_kernel"));
+  int64_t u = 0;
+  FGX_TRY(scan_counts_to_ptr<int64_t>(keep, n, 0, kept, work, ctx->host_i64, &u, s));
+  hipLaunchKernelGGL((kept_ptr_kernel<TP>), dim3(grid_for(A->n_outer + 1, 256)), dim3(256), 0, s,
+                     static_cast<const TP*>(A->ptr), base, A->n_outer, kept,
+                     static_cast<TP*>(c_ptr));
+  FGX_TRY(launch_check("kept_ptr_kernel"));
+  hipLaunchKernelGGL((compact_kernel<TI, TV>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     static_cast<const TI*>(A->idx), val, n, kept, static_cast<TI*>(c_idx),
+                     static_cast<TV*>(c_val));
+  FGX_TRY(launch_check("compact_kernel"));
+  *c_nnz = u;
+  return FGX_OK;
+}
+
+}  // namespace
+}  // namespace fgx
+
+using namespace fgx;
+
+extern "C" {
+
+int fgx_transpose(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_idx, void* t_val,
+                  void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_transpose: NULL ctx");
+  FGX_TRY(check_compressed(A, "fgx_transpose"));
+  if (!t_ptr || (A->nnz > 0 && (!t_idx || !t_val)))
+    return set_error(FGX_EINVAL, "fgx_transpose: NULL output");
+  if (A->n_outer > UINT32_MAX || A->n_inner > UINT32_MAX || A->nnz > INT32_MAX)
+    return set_error(FGX_EUNSUP, "fgx_transpose: dimensions beyond 2^32 or nnz beyond 2^31");
+  if (A->n_outer == 0 && A->nnz > 0)
+    return set_error(FGX_EINVAL, "fgx_transpose: stored entries without fibres");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  const int vb = elem_size(A->val_type);
+  if (vb == 0) return set_error(FGX_EUNSUP, "fgx_transpose: value type");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  int rc = FGX_OK;
+  FGX_DISPATCH_IDX(ic, {
+    if (vb == 2) rc = transpose_typed<TP, TI, 2>(ctx, A, t_ptr, t_idx, t_val, s);
+    else if (vb == 4) rc = transpose_typed<TP, TI, 4>(ctx, A, t_ptr, t_idx, t_val, s);
+    else rc = transpose_typed<TP, TI, 8>(ctx, A, t_ptr, t_idx, t_val, s);
+  });
+  return rc;
+}
+
+int fgx_coo_to_compressed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t nnz,
+                          int idx_type, int val_type, int index_base, const void* outer,
+                          const void* inner, const void* val, void* c_ptr, void* c_idx,
+                          void* c_val, int64_t* c_nnz, void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_coo_to_compressed: NULL ctx");
+  if (n_outer < 0 || n_inner < 0 || nnz < 0)
+    return set_error(FGX_EINVAL, "fgx_coo_to_compressed: negative size");
+  if (index_base != 0 && index_base != 1)
+    return set_error(FGX_EINVAL, "fgx_coo_to_compressed: index_base must be 0 or 1");
+  if (!c_ptr || !c_nnz || (nnz > 0 && (!outer || !inner || !val || !c_idx || !c_val)))
+    return set_error(FGX_EINVAL, "fgx_coo_to_compressed: NULL argument");
+  if (n_outer > UINT32_MAX || n_inner > UINT32_MAX || nnz > INT32_MAX)
+    return set_error(FGX_EUNSUP, "fgx_coo_to_compressed: dimensions beyond 2^32 or nnz beyond 2^31");
+  if (idx_type != FGX_I32 && idx_type != FGX_I64)
+    return set_error(FGX_EUNSUP, "fgx_coo_to_compressed: coordinates must be I32 or I64");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+#define FGX_COO(TI_, TV_)                                                                       \
+  coo_typed<TI_, TV_>(ctx, n_outer, n_inner, nnz, index_base, static_cast<const TI_*>(outer),   \
+                      static_cast<const TI_*>(inner), static_cast<const TV_*>(val),             \
+                      static_cast<TI_*>(c_ptr), static_cast<TI_*>(c_idx), static_cast<TV_*>(c_val), \
+                      c_nnz, s)
+#define FGX_COO_VAL(TI_)                                                                  \
+  switch (val_type) {                                                                     \
+    case FGX_F64: return FGX_COO(TI_, double);                                            \
+    case FGX_F32: return FGX_COO(TI_, float);                                             \
+    case FGX_I32: return FGX_COO(TI_, int32_t);                                           \
+    default: return set_error(FGX_EUNSUP, "fgx_coo_to_compressed: value type");           \
+  }
+  if (idx_type == FGX_I32) {
+    FGX_COO_VAL(int32_t);
+  } else {
+    FGX_COO_VAL(int64_t);
+  }
+#undef FGX_COO_VAL
+#undef FGX_COO
+}
+
+int fgx_dropfills(fgx_ctx* ctx, const fgx_compressed* A, double fill, void* c_ptr, void* c_idx,
+                  void* c_val, int64_t* c_nnz, void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_dropfills: NULL ctx");
+  FGX_TRY(check_compressed(A, "fgx_dropfills"));
+  if (!c_ptr || !c_nnz || (A->nnz > 0 && (!c_idx || !c_val)))
+    return set_error(FGX_EINVAL, "fgx_dropfills: NULL output");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  int rc = FGX_OK;
+  switch (A->val_type) {
+    case FGX_F64:
+      FGX_DISPATCH_IDX(ic, rc = (dropfills_typed<TP, TI, double>(ctx, A, fill, c_ptr, c_idx, c_val,
+                                                                 c_nnz, s)));
+      break;
+    case FGX_F32:
+      FGX_DISPATCH_IDX(ic, rc = (dropfills_typed<TP, TI, float>(ctx, A, static_cast<float>(fill),
+                                                                c_ptr, c_idx, c_val, c_nnz, s)));
+      break;
+    case FGX_I32:
+      if (!(fill >= INT32_MIN && fill <= INT32_MAX) ||
+          fill != static_cast<double>(static_cast<int32_t>(fill)))
+        return set_error(FGX_EINVAL, "fgx_dropfills: fill is not an Int32 value");
+      FGX_DISPATCH_IDX(ic, rc = (dropfills_typed<TP, TI, int32_t>(ctx, A, static_cast<int32_t>(fill),
+                                                                  c_ptr, c_idx, c_val, c_nnz, s)));
+      break;
+    default:
+      return set_error(FGX_EUNSUP, "fgx_dropfills: value type (F64, F32 or I32)");
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -126,6 +126,10 @@ SIGNATURES = {
     "fgx_spgemm_symbolic": (_I, [_P, _I, _PC, _PC, _P, ctypes.POINTER(_I64), _P]),
     "fgx_spgemm_numeric": (_I, [_P, _I, _PC, _PC, _P, _P, _P, _P]),
     "fgx_convert": (_I, [_P, _P, _I, _P, _I, _I64, _P]),
+    "fgx_transpose": (_I, [_P, _PC, _P, _P, _P, _P]),
+    "fgx_coo_to_compressed": (_I, [_P, _I64, _I64, _I64, _I, _I, _I, _P, _P, _P, _P, _P, _P,
+                                   ctypes.POINTER(_I64), _P]),
+    "fgx_dropfills": (_I, [_P, _PC, ctypes.c_double, _P, _P, _P, ctypes.POINTER(_I64), _P]),
     "fgx_batch_plan_create": (_I, [_P, ctypes.POINTER(BatchItem), _I64, _I, ctypes.POINTER(_P)]),
     "fgx_batch_plan_run": (_I, [_P, _P, _P]),
     "fgx_batch_plan_stats": (_I, [_P, ctypes.POINTER(_I64)]),

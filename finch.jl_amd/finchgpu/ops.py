@@ -254,6 +254,62 @@ def spgemm(semiring: str, A: Compressed, B: Compressed) -> Compressed:
     return Compressed(cptr, cidx, cval, A.n_outer, B.n_inner, A.base)
 
 
+def transpose(A: Compressed) -> Compressed:
+    """The same matrix stored along its other dimension (CSC <-> CSR): fibre j
+    of the result lists, in ascending order, the fibres of A that store j."""
+    lib = L.load()
+    da = A.desc()
+    tptr = torch.empty(A.n_inner + 1, dtype=A.ptr.dtype, device=A.device)
+    tidx = torch.empty(A.nnz, dtype=A.idx.dtype, device=A.device)
+    tval = torch.empty(A.nnz, dtype=A.val.dtype, device=A.device)
+    L.check(lib.fgx_transpose(context(A.device).handle, ctypes.byref(da), _ptr(tptr), _ptr(tidx),
+                              _ptr(tval), _stream(A.device)), "fgx_transpose")
+    return Compressed(tptr, tidx, tval, A.n_inner, A.n_outer, A.base)
+
+
+def coo_to_compressed(outer: torch.Tensor, inner: torch.Tensor, val: torch.Tensor, n_outer: int,
+                      n_inner: int, base: int = 1) -> Compressed:
+    """Coordinate lists -> compressed fibres (fsparse, then Dense(SparseList)):
+    sorted by (outer, inner), duplicate coordinates summed in input order."""
+    if not (outer.numel() == inner.numel() == val.numel()):
+        raise L.DimensionMismatch(L.EDIM, "coordinate and value lists differ in length")
+    if outer.dtype != inner.dtype:
+        raise L.UnsupportedError(L.EUNSUP, "coordinate lists must share one integer type")
+    for name, t in (("outer", outer), ("inner", inner), ("val", val)):
+        if not t.is_cuda or not t.is_contiguous():
+            raise L.UnsupportedError(L.EUNSUP, f"{name} must be a contiguous device tensor")
+    lib = L.load()
+    dev = val.device
+    n = val.numel()
+    cptr = torch.empty(n_outer + 1, dtype=outer.dtype, device=dev)
+    cidx = torch.empty(n, dtype=outer.dtype, device=dev)
+    cval = torch.empty(n, dtype=val.dtype, device=dev)
+    nnz = ctypes.c_int64()
+    L.check(lib.fgx_coo_to_compressed(context(dev).handle, int(n_outer), int(n_inner), n,
+                                      fgx_dtype(outer.dtype), fgx_dtype(val.dtype), int(base),
+                                      _ptr(outer), _ptr(inner), _ptr(val), _ptr(cptr),
+                                      _ptr(cidx), _ptr(cval), ctypes.byref(nnz), _stream(dev)),
+            "fgx_coo_to_compressed")
+    u = nnz.value
+    return Compressed(cptr, cidx[:u].clone(), cval[:u].clone(), int(n_outer), int(n_inner),
+                      int(base))
+
+
+def dropfills(A: Compressed, fill: float = 0.0) -> Compressed:
+    """The stored entries of A that are not isequal to `fill`."""
+    lib = L.load()
+    da = A.desc()
+    cptr = torch.empty(A.n_outer + 1, dtype=A.ptr.dtype, device=A.device)
+    cidx = torch.empty(A.nnz, dtype=A.idx.dtype, device=A.device)
+    cval = torch.empty(A.nnz, dtype=A.val.dtype, device=A.device)
+    nnz = ctypes.c_int64()
+    L.check(lib.fgx_dropfills(context(A.device).handle, ctypes.byref(da), float(fill), _ptr(cptr),
+                              _ptr(cidx), _ptr(cval), ctypes.byref(nnz), _stream(A.device)),
+            "fgx_dropfills")
+    u = nnz.value
+    return Compressed(cptr, cidx[:u].clone(), cval[:u].clone(), A.n_outer, A.n_inner, A.base)
+
+
 def to_device(ptr, idx, val, n_outer, n_inner, base=1, device="cuda") -> Compressed:
     """Uploads host (numpy) compressed buffers."""
     t = lambda a: torch.as_tensor(a).to(device).contiguous()

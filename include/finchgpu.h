@@ -176,6 +176,40 @@ int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
                        const fgx_compressed* B, const void* c_ptr, void* c_idx,
                        void* c_val, void* stream);
 
+/* ---- format conversion on the device ------------------------------------- */
+
+/* Transpose: the same matrix stored along its other dimension (CSC <-> CSR).
+ * permutedims(A, (2, 1)) of a 2-D Dense(SparseList) tensor and the swizzled
+ * copy it runs (src/interface/copy.jl:32-40,64-71); Finch's concordize pass
+ * inserts it when loop order and storage order disagree
+ * (src/scheduler/optimize.jl:396-452).  Outputs, all in A's types and
+ * index_base: t_ptr (A.n_inner+1 entries), t_idx (A.nnz: A's fibre numbers)
+ * and t_val (A.nnz).  Every stored entry is kept, explicit zeros included;
+ * each output fibre is sorted.  Deterministic (stable radix sort). */
+int fgx_transpose(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_idx, void* t_val,
+                  void* stream);
+
+/* COO -> compressed: fsparse(I, J, V, (m, n)) (src/interface/fsparse.jl:64-97)
+ * copied into Dense(SparseList).  Entry q is (outer[q], inner[q], val[q])
+ * with coordinates in [base, base+n_outer) x [base, base+n_inner) (idx_type
+ * I32 or I64; val_type F64, F32 or I32).  Output: entries ordered by
+ * (outer, inner); each run of equal coordinates combined with `+` left to
+ * right in input order (fsparse.jl:85-90; I32 wraps); c_ptr (n_outer+1,
+ * idx_type) and c_idx/c_val of capacity nnz; *c_nnz = distinct coordinates
+ * (synchronises).  A coordinate outside the shape returns FGX_EDIM. */
+int fgx_coo_to_compressed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t nnz,
+                          int idx_type, int val_type, int index_base, const void* outer,
+                          const void* inner, const void* val, void* c_ptr, void* c_idx,
+                          void* c_val, int64_t* c_nnz, void* stream);
+
+/* dropfills(A) into the same format (src/interface/copy.jl:73-117): the
+ * stored entries whose value is not isequal to `fill` (NaN equals NaN, -0.0
+ * differs from 0.0).  Values F64, F32 or I32.  c_ptr (A.n_outer+1, A's
+ * ptr_type), c_idx/c_val of capacity A.nnz; *c_nnz = kept entries
+ * (synchronises). */
+int fgx_dropfills(fgx_ctx* ctx, const fgx_compressed* A, double fill, void* c_ptr, void* c_idx,
+                  void* c_val, int64_t* c_nnz, void* stream);
+
 /* ---- value-precision conversion ----------------------------------------- */
 
 /* dst[i] = (dst_type) src[i] for n elements (F32 <-> BF16/F16/F64), round to

@@ -362,3 +362,133 @@ int or_max_threads(void) {
   return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------------
+ * Format conversion.
+ *
+ * Transpose: permutedims(A, (2, 1)) of a 2-D Dense(SparseList) tensor
+ * (src/interface/copy.jl:64-71), i.e. the swizzled copy of copy.jl:32-40
+ * into the other storage order.  Every stored entry is kept (a written
+ * coordinate is stored even when it equals the fill, src/tensors/scalars.jl:
+ * 209-218); output fiber j lists the input fibers that store j in ascending
+ * order.  The textbook counting transpose: count, prefix sum, then place
+ * the entries in input order.  Values are copied as `vbytes`-byte words.
+ * ------------------------------------------------------------------------- */
+void or_transpose(i64 n_outer, i64 n_inner, const i64* ptr, const i64* idx, const void* val,
+                  int vbytes, int base, i64* tptr, i64* tidx, void* tval) {
+  i64* next = (i64*)calloc((size_t)(n_inner + 1), sizeof(i64));
+  const i64 nnz = ptr[n_outer] - ptr[0];
+  for (i64 q = 0; q < nnz; q++) next[idx[q] - base + 1]++;
+  for (i64 j = 0; j < n_inner; j++) next[j + 1] += next[j];
+  for (i64 j = 0; j <= n_inner; j++) tptr[j] = next[j] + base;
+  for (i64 o = 0; o < n_outer; o++) {
+    for (i64 q = ptr[o] - base; q < ptr[o + 1] - base; q++) {
+      const i64 j = idx[q] - base;
+      const i64 p = next[j]++;
+      tidx[p] = o + base;
+      memcpy((char*)tval + p * vbytes, (const char*)val + q * vbytes, (size_t)vbytes);
+    }
+  }
+  free(next);
+}
+
+/* Stable counting sort of perm[0..n) by key[perm[.]] in [0, nk). */
+static void stable_count_sort(i64 n, i64 nk, const i64* key, i64* perm, i64* tmp) {
+  i64* c = (i64*)calloc((size_t)(nk + 1), sizeof(i64));
+  for (i64 t = 0; t < n; t++) c[key[perm[t]] + 1]++;
+  for (i64 j = 0; j < nk; j++) c[j + 1] += c[j];
+  for (i64 t = 0; t < n; t++) tmp[c[key[perm[t]]]++] = perm[t];
+  memcpy(perm, tmp, sizeof(i64) * (size_t)n);
+  free(c);
+}
+
+/* ---------------------------------------------------------------------------
+ * COO -> compressed: fsparse(I, J, V, (m, n)) (src/interface/fsparse.jl:
+ * 64-97) copied into Dense(SparseList).  Entries are ordered by (outer,
+ * inner) with ties kept in input order (two stable counting sorts, inner
+ * first), and each run of equal coordinates is combined with `+` as
+ * foldl(combine, V[start:stop-1]) (fsparse.jl:85-90), left to right.  Int32
+ * sums wrap.  Returns the number of distinct coordinates; cptr gets n_outer+1
+ * entries.  (The reference's fsparse_lt, fsparse.jl:1-7, seeds its
+ * comparison with `I[1][j] < I[1][j]`, so it orders by the trailing
+ * coordinate only; the two agree whenever the leading coordinates arrive
+ * sorted within each fibre.  The full (outer, inner) order here is that of
+ * SparseArrays.sparse, which the fsparse docstring refers to.)
+ * ------------------------------------------------------------------------- */
+#define DEF_COO(T, SFX, ADD)                                                               \
+  i64 or_coo_##SFX(i64 n_outer, i64 n_inner, i64 nnz, const i64* outer, const i64* inner, \
+                   const T* val, int base, i64* cptr, i64* cidx, T* cval) {                \
+    i64* perm = (i64*)malloc(sizeof(i64) * (size_t)(nnz > 0 ? nnz : 1));                 \
+    i64* tmp = (i64*)malloc(sizeof(i64) * (size_t)(nnz > 0 ? nnz : 1));                  \
+    i64* ko = (i64*)calloc((size_t)(nnz > 0 ? nnz : 1), sizeof(i64));                    \
+    i64* ki = (i64*)calloc((size_t)(nnz > 0 ? nnz : 1), sizeof(i64));                    \
+    for (i64 q = 0; q < nnz; q++) {                                                        \
+      perm[q] = q;                                                                         \
+      ko[q] = outer[q] - base;                                                             \
+      ki[q] = inner[q] - base;                                                             \
+    }                                                                                      \
+    stable_count_sort(nnz, n_inner, ki, perm, tmp);                                        \
+    stable_count_sort(nnz, n_outer, ko, perm, tmp);                                        \
+    i64 u = 0;                                                                             \
+    for (i64 o = 0; o <= n_outer; o++) cptr[o] = 0;                                        \
+    for (i64 t = 0; t < nnz;) {                                                            \
+      const i64 q = perm[t];                                                               \
+      T acc = val[q];                                                                      \
+      i64 s = t + 1;                                                                       \
+      while (s < nnz && ko[perm[s]] == ko[q] && ki[perm[s]] == ki[q]) {                    \
+        acc = ADD(acc, val[perm[s]]);                                                      \
+        s++;                                                                               \
+      }                                                                                    \
+      cidx[u] = ki[q] + base;                                                              \
+      cval[u] = acc;                                                                       \
+      cptr[ko[q] + 1]++;                                                                   \
+      u++;                                                                                 \
+      t = s;                                                                               \
+    }                                                                                      \
+    cptr[0] = base;                                                                        \
+    for (i64 o = 0; o < n_outer; o++) cptr[o + 1] += cptr[o];                              \
+    free(perm);                                                                            \
+    free(tmp);                                                                             \
+    free(ko);                                                                              \
+    free(ki);                                                                              \
+    return u;                                                                              \
+  }
+#define COO_ADD(a, b) ((a) + (b))
+DEF_COO(double, f64, COO_ADD)
+DEF_COO(float, f32, COO_ADD)
+DEF_COO(int32_t, i32, wrap_add_i32)
+
+/* ---------------------------------------------------------------------------
+ * dropfills(A) into the same format (src/interface/copy.jl:73-117): keep each
+ * stored entry unless isequal(value, fill) — `isequal` treats NaN as equal
+ * to NaN and -0.0 as different from 0.0 (copy.jl:98-101).  Returns the kept
+ * count; cptr gets n_outer+1 entries.
+ * ------------------------------------------------------------------------- */
+#define DEF_DROPFILLS(T, SFX, ISEQ)                                                       \
+  i64 or_dropfills_##SFX(i64 n_outer, const i64* ptr, const i64* idx, const T* val,      \
+                         int base, T fill, i64* cptr, i64* cidx, T* cval) {               \
+    i64 u = 0;                                                                            \
+    cptr[0] = base;                                                                       \
+    for (i64 o = 0; o < n_outer; o++) {                                                   \
+      for (i64 q = ptr[o] - base; q < ptr[o + 1] - base; q++) {                           \
+        if (ISEQ(val[q], fill)) continue;                                                 \
+        cidx[u] = idx[q];                                                                 \
+        cval[u] = val[q];                                                                 \
+        u++;                                                                              \
+      }                                                                                   \
+      cptr[o + 1] = u + base;                                                             \
+    }                                                                                     \
+    return u;                                                                             \
+  }
+static inline int iseq_f64(double a, double b) {
+  if (isnan(a) || isnan(b)) return isnan(a) && isnan(b);
+  return a == b && signbit(a) == signbit(b);
+}
+static inline int iseq_f32(float a, float b) {
+  if (isnan(a) || isnan(b)) return isnan(a) && isnan(b);
+  return a == b && signbit(a) == signbit(b);
+}
+#define ISEQ_I32(a, b) ((a) == (b))
+DEF_DROPFILLS(double, f64, iseq_f64)
+DEF_DROPFILLS(float, f32, iseq_f32)
+DEF_DROPFILLS(int32_t, i32, ISEQ_I32)

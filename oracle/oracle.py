@@ -166,3 +166,50 @@ def spgemm(semiring, m, n, aptr, aidx, aval, bptr, bidx, bval, base=1):
                                      _p(bptr), _p(bidx), _p(bval), _INT(base), _p(cptr),
                                      _p(cidx), _p(cval))
     return cptr, cidx[:nnz].copy(), cval[:nnz].copy()
+
+
+def transpose(n_outer, n_inner, ptr, idx, val, base=1):
+    """Counting transpose: permutedims of a 2-D Dense(SparseList) (copy.jl:64-71)."""
+    val = np.ascontiguousarray(val)
+    ptr, idx = _i64(ptr), _i64(idx)
+    nnz = int(ptr[n_outer] - ptr[0])
+    tptr = np.empty(n_inner + 1, dtype=np.int64)
+    tidx = np.empty(max(1, nnz), dtype=np.int64)
+    tval = np.empty(max(1, nnz), dtype=val.dtype)
+    lib().or_transpose(_I64(n_outer), _I64(n_inner), _p(ptr), _p(idx), _p(val),
+                       _INT(val.dtype.itemsize), _INT(base), _p(tptr), _p(tidx), _p(tval))
+    return tptr, tidx[:nnz].copy(), tval[:nnz].copy()
+
+
+def coo_to_compressed(n_outer, n_inner, outer, inner, val, base=1):
+    """fsparse then Dense(SparseList): (outer, inner)-sorted, duplicates summed
+    left to right in input order (fsparse.jl:64-97)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    outer, inner = _i64(outer), _i64(inner)
+    nnz = len(val)
+    cptr = np.empty(n_outer + 1, dtype=np.int64)
+    cidx = np.empty(max(1, nnz), dtype=np.int64)
+    cval = np.empty(max(1, nnz), dtype=val.dtype)
+    f = getattr(lib(), f"or_coo_{s}")
+    f.restype = _I64
+    u = f(_I64(n_outer), _I64(n_inner), _I64(nnz), _p(outer), _p(inner), _p(val), _INT(base),
+          _p(cptr), _p(cidx), _p(cval))
+    return cptr, cidx[:u].copy(), cval[:u].copy()
+
+
+def dropfills(n_outer, ptr, idx, val, fill=0.0, base=1):
+    """Keep the entries that are not isequal to `fill` (copy.jl:73-117)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    ptr, idx = _i64(ptr), _i64(idx)
+    nnz = int(ptr[n_outer] - ptr[0])
+    cptr = np.empty(n_outer + 1, dtype=np.int64)
+    cidx = np.empty(max(1, nnz), dtype=np.int64)
+    cval = np.empty(max(1, nnz), dtype=val.dtype)
+    f = getattr(lib(), f"or_dropfills_{s}")
+    f.restype = _I64
+    ctype = {"f64": ctypes.c_double, "f32": ctypes.c_float, "i32": ctypes.c_int32}[s]
+    u = f(_I64(n_outer), _p(ptr), _p(idx), _p(val), _INT(base), ctype(fill), _p(cptr),
+          _p(cidx), _p(cval))
+    return cptr, cidx[:u].copy(), cval[:u].copy()

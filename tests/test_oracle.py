@@ -218,3 +218,101 @@ def test_min_plus_brute_force():
     rows = np.repeat(np.arange(m), np.diff(cptr))
     assert np.array_equal(cval, R[rows, cidx - 1])           # bit-exact
     assert np.isfinite(R).sum() == cptr[-1] - 1             # pattern = finite entries
+
+
+# ---- format conversion --------------------------------------------------------------
+
+def _full_storage(A):
+    """Every entry of a dense matrix stored, column-major (CSC, 1-based)."""
+    m, n = A.shape
+    ptr = np.arange(0, m * n + 1, m, dtype=np.int64) + 1
+    idx = np.tile(np.arange(m, dtype=np.int64), n) + 1
+    return ptr, idx, A.T.reshape(-1).copy()
+
+
+def test_dropfills_and_coo_reproduce_representation_goldens(goldens):
+    # Dense{SparseList}_representation.txt: the exact ptr/idx/val Finch builds
+    # from each dense array.  Reached two independent ways: dropping the fills
+    # from full storage, and assembling the nonzeros from shuffled COO lists.
+    rng = np.random.default_rng(0)
+    for c in goldens["representation_dense_sparselist"]["cases"]:
+        A = np.array(c["dense"], dtype=np.float64)
+        m, n = A.shape
+        ptr, idx, val = _full_storage(A)
+        dp, di, dv = oracle.dropfills(n, ptr, idx, val, 0.0)
+        assert (dp.tolist(), di.tolist(), dv.tolist()) == (c["ptr"], c["idx"], c["val"]), c["name"]
+        I, J = np.nonzero(A)
+        p = rng.permutation(len(I))
+        cp, ci, cv = oracle.coo_to_compressed(n, m, J[p] + 1, I[p] + 1, A[I[p], J[p]])
+        assert (cp.tolist(), ci.tolist(), cv.tolist()) == (c["ptr"], c["idx"], c["val"]), c["name"]
+
+
+def test_dropfills_isequal_semantics():
+    # copy.jl:98-101 drops only values isequal to the fill: -0.0 and NaN stay
+    # when the fill is 0.0; with a NaN fill the NaNs go.
+    ptr = np.array([1, 4, 6])
+    idx = np.array([1, 2, 3, 1, 3])
+    val = np.array([0.0, -0.0, np.nan, 2.0, 0.0])
+    p, i, v = oracle.dropfills(2, ptr, idx, val, 0.0)
+    assert p.tolist() == [1, 3, 4] and i.tolist() == [2, 3, 1]
+    assert np.signbit(v[0]) and np.isnan(v[1]) and v[2] == 2.0
+    p, i, v = oracle.dropfills(2, ptr, idx, val, np.nan)
+    assert p.tolist() == [1, 3, 5] and i.tolist() == [1, 2, 1, 3]
+
+
+@pytest.mark.parametrize("base", [1, 0])
+def test_transpose_is_dense_transpose(base):
+    # A permutation of the stored entries: the result, densified, is A^T
+    # exactly, fibres are sorted, and transposing twice is the identity.
+    rng = np.random.default_rng(3)
+    A = sp.random(57, 41, density=0.12, random_state=rng, format="csr")
+    A.data = rng.standard_normal(A.nnz)
+    A.sort_indices()
+    ptr, idx = A.indptr.astype(np.int64) + base, A.indices.astype(np.int64) + base
+    tp, ti, tv = oracle.transpose(57, 41, ptr, idx, A.data, base)
+    T = sp.csr_matrix((tv, ti - base, tp - base), shape=(41, 57))
+    assert np.array_equal(T.toarray(), A.toarray().T)
+    for o in range(41):
+        assert np.all(np.diff(ti[tp[o] - base:tp[o + 1] - base]) > 0)
+    bp, bi, bv = oracle.transpose(41, 57, tp, ti, tv, base)
+    assert np.array_equal(bp, ptr) and np.array_equal(bi, idx) and np.array_equal(bv, A.data)
+
+
+def test_transpose_keeps_explicit_zeros_and_empty_fibres():
+    # Written coordinates are stored even when equal to the fill
+    # (scalars.jl:209-218), so a stored 0.0 survives the transpose.
+    ptr = np.array([1, 1, 3, 3, 4])
+    idx = np.array([2, 5, 2])
+    val = np.array([0.0, 7.0, 1.5])
+    tp, ti, tv = oracle.transpose(4, 6, ptr, idx, val)
+    assert tp.tolist() == [1, 1, 3, 3, 3, 4, 4]
+    assert ti.tolist() == [2, 4, 2] and tv.tolist() == [0.0, 1.5, 7.0]
+
+
+def test_coo_duplicates_fold_left_in_input_order():
+    # fsparse.jl:85-90: foldl(+, V[start:stop-1]) over each run of equal
+    # coordinates, in input order.  1e16 + 1 + -1e16 (left fold) is 0.0,
+    # while any other order gives 1.0 or 2.0: the order is pinned.
+    outer = np.array([2, 1, 2, 2, 1])
+    inner = np.array([3, 1, 3, 3, 2])
+    val = np.array([1e16, 5.0, 1.0, -1e16, 4.0])
+    cp, ci, cv = oracle.coo_to_compressed(2, 3, outer, inner, val)
+    assert cp.tolist() == [1, 3, 4] and ci.tolist() == [1, 2, 3]
+    assert cv.tolist() == [5.0, 4.0, 0.0]
+    # Int32 sums wrap like Julia's Int32.
+    cp, ci, cv = oracle.coo_to_compressed(1, 1, np.array([1, 1]), np.array([1, 1]),
+                                          np.array([2**31 - 1, 2], dtype=np.int32))
+    assert cv.tolist() == [-(2**31) + 1]
+
+
+def test_coo_matches_scipy_sum_duplicates():
+    rng = np.random.default_rng(5)
+    m, n, k = 300, 200, 5000
+    I = rng.integers(0, m, k)
+    J = rng.integers(0, n, k)
+    V = rng.integers(-9, 10, k).astype(np.float64)  # small integers: every order is exact
+    cp, ci, cv = oracle.coo_to_compressed(n, m, J + 1, I + 1, V)
+    S = sp.coo_matrix((V, (I, J)), shape=(m, n)).tocsc()
+    S.sum_duplicates()
+    assert np.array_equal(cp - 1, S.indptr) and np.array_equal(ci - 1, S.indices)
+    assert np.array_equal(cv, S.data)
