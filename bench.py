@@ -179,6 +179,7 @@ def main():
     if not args.no_sparse:
         sparse = bench_sparse_sparse(dev, with_cpu=not args.no_cpu)
         sparse["fp64_10k"] = bench_fp64_10k(dev)
+        sparse["format_conversion"] = bench_format(dev, A, ptr, idx, val, with_cpu=not args.no_cpu)
         print("# sparse-sparse done", file=sys.stderr, flush=True)
     batched = None
     if not args.no_batch:
@@ -323,6 +324,48 @@ def bench_sparse_sparse(dev, with_cpu=True):
             oracle.spgemm(sr, rows, n, ap[: rows + 1], ai[:q1], xv[:q1], bp, bi, yv)
             row["cpu_ms_1thread_first_tenth_rows"] = round((time.perf_counter() - t) * 1e3, 1)
         out[name] = row
+    return out
+
+
+def bench_format(dev, A, ptr, idx, val, with_cpu=True):
+    """Format conversion on the 1M x 1M power-law operand (16M stored
+    entries): the device transpose (CSR -> CSC, stable radix sort), COO
+    assembly of the same entries listed in a shuffled order, and dropfills.
+    Algorithmic bytes: the input read once plus the output written once."""
+    import numpy as np
+    import torch
+
+    from finchgpu import ops
+
+    gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
+    nnz = A.nnz
+    out = {"workload": f"1M x 1M CSR, nnz={nnz}, Zipf(1.5) rows, Int32 ptr/idx, FP32 values"}
+    ms = _events_ms(lambda: ops.transpose(A), 10)
+    by = 2 * _csr_bytes(A)
+    out["transpose"] = {"ms": round(ms, 3), "algorithmic_bytes": by, "gbs": gbs(by, ms)}
+    rng = np.random.default_rng(31)
+    perm = torch.from_numpy(rng.permutation(nnz)).to(dev)
+    rows = torch.repeat_interleave(torch.arange(A.n_outer, device=dev, dtype=torch.int32),
+                                   (A.ptr[1:] - A.ptr[:-1]).long())
+    ro = (rows + 1)[perm].contiguous()
+    co = A.idx[perm].contiguous()
+    vo = A.val[perm].contiguous()
+    del rows
+    ms = _wall_ms(lambda: ops.coo_to_compressed(ro, co, vo, A.n_outer, A.n_inner, 1), 5)
+    by = nnz * 12 + _csr_bytes(A)
+    out["coo_to_compressed"] = {"ms": round(ms, 3), "algorithmic_bytes": by, "gbs": gbs(by, ms),
+                                "input": "the same entries as (row, col, val) lists in a "
+                                         "random order"}
+    ms = _wall_ms(lambda: ops.dropfills(A, 0.0), 10)
+    by = 2 * _csr_bytes(A)
+    out["dropfills"] = {"ms": round(ms, 3), "algorithmic_bytes": by, "gbs": gbs(by, ms)}
+    del perm, ro, co, vo
+    if with_cpu:
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        t = time.perf_counter()
+        oracle.transpose(A.n_outer, A.n_inner, ptr, idx, val, 1)
+        out["transpose"]["cpu_ms_1thread"] = round((time.perf_counter() - t) * 1e3, 1)
     return out
 
 

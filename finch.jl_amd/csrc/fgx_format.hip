@@ -22,13 +22,13 @@
 namespace fgx {
 namespace {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortItems = 16;
+constexpr int kSortThreads = 512;
+constexpr int kSortItems = 8;
 constexpr int64_t kSortTile = static_cast<int64_t>(kSortThreads) * kSortItems;
 constexpr int kDigitBits = 8;
 constexpr int kBins = 1 << kDigitBits;
 constexpr int kSortWaves = kSortThreads / kWave;
-static_assert(kSortThreads == kBins, "the digit tables give one digit to each thread");
+static_assert(kSortThreads >= kBins, "the digit tables give one digit to each of the first threads");
 
 // Raw value words: values are moved, never interpreted, by the sort.
 template <int B> struct word_of;
@@ -41,6 +41,23 @@ __device__ __forceinline__ uint32_t rebased(const T* __restrict__ a, int64_t bas
   return static_cast<uint32_t>(static_cast<int64_t>(a[e]) - base);
 }
 
+// Each wave of a sort block owns a contiguous sub-tile of kWaveSpan
+// elements: element e of the tile sits in wave e / kWaveSpan, round
+// (e % kWaveSpan) / 64, lane e % 64.
+constexpr int kWaveSpan = kSortItems * kWave;
+
+// Lanes of the wave whose digit equals this lane's (among lanes with `in`).
+__device__ __forceinline__ uint64_t same_digit(uint32_t d, bool in) {
+  uint64_t same = ballot(in);
+#pragma unroll
+  for (int b = 0; b < kDigitBits; b++) {
+    const bool bit = ((d >> b) & 1u) != 0;
+    const uint64_t m = ballot(bit);
+    same &= bit ? m : ~m;
+  }
+  return same;
+}
+
 // Digit counts of one tile of keys: hist[d * ntiles + tile].
 template <typename TK>
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(const TK* __restrict__ keys,
@@ -48,45 +65,91 @@ __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(const TK* __res
                                                                  int shift, int64_t ntiles,
                                                                  int64_t* __restrict__ hist) {
   __shared__ uint32_t h[kBins];
-  h[threadIdx.x] = 0;
-  __syncthreads();
+  const int tid = threadIdx.x;
+  if (tid < kBins) h[tid] = 0;
   const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
-#pragma unroll 4
+  uint32_t k[kSortItems];
+#pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + threadIdx.x;
-    if (e < n) atomicAdd(&h[(rebased(keys, kbase, e) >> shift) & (kBins - 1)], 1u);
+    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
+    k[i] = e < n ? (rebased(keys, kbase, e) >> shift) & (kBins - 1) : kBins;
   }
   __syncthreads();
-  hist[static_cast<int64_t>(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++)
+    if (k[i] < kBins) atomicAdd(&h[k[i]], 1u);
+  __syncthreads();
+  if (tid < kBins) hist[static_cast<int64_t>(tid) * ntiles + blockIdx.x] = h[tid];
 }
 
-// Stable scatter of one tile.  Element e (key digit d) goes to
-// offs[d * ntiles + tile] plus the number of elements of the tile before e
-// with digit d.  The tile is read in rounds of kSortThreads consecutive
-// elements (element order = round, then thread); in a round, the lanes of a
-// wave that share e's digit are found with kDigitBits ballots, and the waves
-// are ordered through the per-wave digit counts in LDS.  The whole tile is
-// loaded before the first round, so its loads are in flight together.
+// Tile processed by this block.  Blocks go round-robin over the 8 XCDs;
+// numbering the tiles so that each XCD takes a contiguous range keeps the
+// boundary cache lines shared by consecutive tiles' output runs in one L2.
+// A bijection for any grid size.
+__device__ __forceinline__ int64_t xcd_tile() {
+  constexpr int kXcd = 8;
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t q = nb / kXcd, r = nb % kXcd, x = b % kXcd;
+  return x * q + (x < r ? x : r) + b / kXcd;
+}
+
+// Exclusive scan of one value per thread over the block; returns the prefix.
+__device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t* wsum) {
+  const int lane = lane_id();
+  const int w = wave_in_block();
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const uint32_t y = shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == kWave - 1) wsum[w] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int v2 = 0; v2 < w; v2++) before += wsum[v2];
+  return before + x - v;
+}
+
+// Stable scatter of one tile, in three steps.
+//   1. Wave ranks: each wave walks its sub-tile in element order, 64
+//      elements a round; the lanes sharing an element's digit are found with
+//      kDigitBits ballots, and the wave keeps its own running digit counts in
+//      LDS (read by every lane, then advanced by each group's lowest lane: one
+//      instruction stream, so no barrier is needed).
+//   2. Tile ranks: the waves' counts are combined per digit (one barrier) into
+//      each wave's first slot for each digit, after the tile's earlier digits;
+//      every element is parked in LDS at its rank in the tile sorted stably by
+//      digit.
+//   3. Writes: thread t takes the t-th element of the sorted tile to
+//      offs[d * ntiles + tile] + (t - start[d]), so consecutive threads write
+//      consecutive addresses of one digit's run (coalesced stores).
+// The sub-tile is loaded before the first round, so its loads are in flight
+// together.
 template <typename TK, typename TQ, typename TW, typename TQO>
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     const TK* __restrict__ keys, int64_t kbase, const TQ* __restrict__ pay, int64_t pbase,
     const TW* __restrict__ val, int64_t n, int shift, int64_t ntiles,
     const int64_t* __restrict__ offs, uint32_t* __restrict__ keys_out, TQO* __restrict__ pay_out,
     int64_t pout_base, TW* __restrict__ val_out) {
-  __shared__ int64_t run[kBins];
-  __shared__ uint32_t wcnt[kSortWaves][kBins];
+  // Small tables (int32 / uint16 so that three blocks fit a CU's LDS):
+  // positions are below 2^31 and tile counts below 2^16.
+  __shared__ int32_t gbase[kBins];
+  __shared__ uint16_t wcnt[kSortWaves][kBins];
+  __shared__ uint32_t wsum[kSortWaves];
+  __shared__ uint32_t sk[kSortTile];
+  __shared__ uint32_t sq[kSortTile];
+  __shared__ TW sv[kSortTile];
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int w = wave_in_block();
-  run[tid] = offs[static_cast<int64_t>(tid) * ntiles + blockIdx.x];
-#pragma unroll
-  for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0;
-  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
+  const int64_t tile = xcd_tile();
+  const int64_t t0 = tile * kSortTile;
+  const int64_t e0 = t0 + w * kWaveSpan + lane;
   uint32_t k[kSortItems], p[kSortItems];
   TW x[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
+    const int64_t e = e0 + i * kWave;
     if (e < n) {
       k[i] = rebased(keys, kbase, e);
       p[i] = rebased(pay, pbase, e);
@@ -97,38 +160,69 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
       x[i] = TW(0);
     }
   }
+  if (tid < kBins) {
+#pragma unroll
+    for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0;
+  }
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (kWave - lane));
+  __syncthreads();
+  // 1. Ranks within the wave's sub-tile.
+  uint16_t* cnt = wcnt[w];
+  uint32_t r[kSortItems];
+#pragma unroll
+  for (int i = 0; i < kSortItems; i++) {
+    const bool in = e0 + i * kWave < n;
+    const uint32_t d = (k[i] >> shift) & (kBins - 1);
+    const uint64_t same = same_digit(d, in);
+    const uint32_t rank = static_cast<uint32_t>(popc(same & below));
+    const uint32_t c = cnt[d];
+    r[i] = c + rank;
+    __builtin_amdgcn_wave_barrier();
+    if (in && rank == 0) cnt[d] = static_cast<uint16_t>(c + static_cast<uint32_t>(popc(same)));
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // 2. Tile ranks: digit tid's start in the tile, then each wave's first slot.
+  {
+    const bool owner = tid < kBins;
+    const int64_t cell = static_cast<int64_t>(owner ? tid : 0) * ntiles + tile;
+    const int64_t g0 = owner ? offs[cell] : 0;
+    const uint32_t tcnt = owner ? static_cast<uint32_t>(offs[cell + 1] - g0) : 0;
+    const uint32_t start = block_prefix(tcnt, wsum);
+    if (owner) {
+      gbase[tid] = static_cast<int32_t>(g0 - start);
+      uint32_t s0 = start;
+#pragma unroll
+      for (int v = 0; v < kSortWaves; v++) {
+        const uint32_t c = wcnt[v][tid];
+        wcnt[v][tid] = static_cast<uint16_t>(s0);
+        s0 += c;
+      }
+    }
+  }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
-    const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
-    const bool in = e < n;
-    const uint32_t d = (k[i] >> shift) & (kBins - 1);
-    uint64_t same = ballot(in);
-#pragma unroll
-    for (int b = 0; b < kDigitBits; b++) {
-      const bool bit = ((d >> b) & 1u) != 0;
-      const uint64_t m = ballot(bit);
-      same &= bit ? m : ~m;
+    if (e0 + i * kWave < n) {
+      const uint32_t pos = cnt[(k[i] >> shift) & (kBins - 1)] + r[i];
+      sk[pos] = k[i];
+      sq[pos] = p[i];
+      sv[pos] = x[i];
     }
-    const int rank = popc(same & below);
-    if (in && rank == 0) wcnt[w][d] = static_cast<uint32_t>(popc(same));
-    __syncthreads();
-    int64_t pos = run[d] + rank;
-    for (int v = 0; v < w; v++) pos += wcnt[v][d];
-    uint32_t tot = 0;
+  }
+  __syncthreads();
+  // 3. Coalesced writes of the sorted tile.
+  const int64_t tile_n = n - t0 < kSortTile ? n - t0 : kSortTile;
 #pragma unroll
-    for (int v = 0; v < kSortWaves; v++) tot += wcnt[v][tid];
-    __syncthreads();
-    run[tid] += tot;
-#pragma unroll
-    for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0;
-    if (in) {
-      keys_out[pos] = k[i];
-      pay_out[pos] = static_cast<TQO>(static_cast<int64_t>(p[i]) + pout_base);
-      val_out[pos] = x[i];
+  for (int i = 0; i < kSortItems; i++) {
+    const int t = i * kSortThreads + tid;
+    if (t < tile_n) {
+      const uint32_t kk = sk[t];
+      const int64_t pos = static_cast<int64_t>(gbase[(kk >> shift) & (kBins - 1)]) + t;
+      keys_out[pos] = kk;
+      pay_out[pos] = static_cast<TQO>(static_cast<int64_t>(sq[t]) + pout_base);
+      val_out[pos] = sv[t];
     }
-    __syncthreads();
   }
 }
 

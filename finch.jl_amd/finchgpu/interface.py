@@ -17,6 +17,14 @@ program                                     kernel
 ``C[i, j] = A[i, j] * B[i, j]`` (``+``, ``max``)  element-wise coiteration
 ==========================================  ==========================================
 
+When the storage order of an operand disagrees with the program (a CSC
+operand of SpMM, or sparse operands of different orders), the operand is
+transposed on the device first (``fgx_transpose``), as Finch's concordize
+pass inserts a transpose (src/scheduler/optimize.jl:396-452); the transposed
+copy is cached on the tensor.  ``permutedims``, ``fsparse`` and ``dropfills``
+are the device forms of src/interface/copy.jl:64-117 and
+src/interface/fsparse.jl:64-97.
+
 ``compute(lazy(A) * (lazy(X) @ lazy(Y)))`` is the fused SDDMM plan of
 ``compute(lazy(M) .* (lazy(A)*lazy(B)))`` (benchmark/benchmarks.jl:141-168).
 Programs outside this set raise ``UnsupportedError``; nothing falls back to
@@ -75,6 +83,17 @@ def parse(program: str):
 def _compressed(t: Tensor) -> ops.Compressed:
     sl = t.lvl.lvl
     return ops.Compressed(sl.ptr, sl.idx, t.element.val, t.lvl.shape, sl.shape, 1)
+
+
+def _other_order(t: Tensor) -> ops.Compressed:
+    """t's matrix stored along the other dimension (device transpose), cached
+    on the tensor while its buffers stay the same objects."""
+    key = (id(t.lvl.lvl.ptr), id(t.lvl.lvl.idx), id(t.element.val))
+    cached = getattr(t, "_other_order", None)
+    if cached is None or cached[0] != key:
+        cached = (key, ops.transpose(_compressed(t)))
+        t._other_order = cached
+    return cached[1]
 
 
 def _kind(x):
@@ -144,8 +163,13 @@ def einsum(program: str, **tensors):
         i, j = a_idx
         j2, k = b_idx
         if red is not None and j == j2 and oidx == (i, k):
-            if comb == "*" and red == "+" and ka == "csr" and kb == "dense":
-                return _dense_tensor(ops.spmm_csr(_compressed(va), vb))
+            if comb == "*" and red == "+" and ka in ("csr", "csc") and kb == "dense":
+                Ar = _compressed(va) if ka == "csr" else _other_order(va)
+                return _dense_tensor(ops.spmm_csr(Ar, vb))
+            if kb in ("csc", "csr") and ka in ("csc", "csr") and ka != kb:
+                # Bring B to A's storage order.
+                vb = _sparse_tensor(_other_order(vb))
+                kb = ka
             if kb in ("csc", "csr") and ka == kb:
                 semiring = {("*", "+"): "plus_times", ("+", "min"): "min_plus"}.get((comb, red))
                 if semiring is None:
@@ -156,7 +180,9 @@ def einsum(program: str, **tensors):
                 # Column-compressed: C = A*B column by column is (B^T A^T)^T.
                 C = ops.spgemm(semiring, _compressed(vb), _compressed(va))
                 return _sparse_tensor(C, _fill(semiring))
-        if red is None and a_idx == b_idx == oidx and ka == kb and ka in ("csc", "csr"):
+        if red is None and a_idx == b_idx == oidx and ka in ("csc", "csr") and kb in ("csc", "csr"):
+            if ka != kb:
+                vb = _sparse_tensor(_other_order(vb))
             op = {"*": "mul", "+": "add", "max": "max"}[comb]
             C = ops.ewise(op, _compressed(va), _compressed(vb))
             t = _sparse_tensor(C)
@@ -218,6 +244,11 @@ def compute(expr: LazyTensor):
                 out = ops.sddmm_csr(_compressed(va), vx, vy.t())
                 C = _compressed(va).with_values(out)
                 return SwizzleArray(_sparse_tensor(C), (2, 1))
+            if ka == "csc" and kx == ky == "dense":
+                # Column fibres j over rows i: the same row dots with the
+                # factors' roles exchanged, on A's own pattern.
+                out = ops.sddmm_csr(_compressed(va), vy.t(), vx)
+                return _sparse_tensor(_compressed(va).with_values(out))
     names = {"matmul": None, "mul": "*", "add": "+", "max": "max"}
     if expr.op in names and a.op == "leaf" and b.op == "leaf":
         A, B = a.args[0], b.args[0]
@@ -230,3 +261,48 @@ def compute(expr: LazyTensor):
         rhs = "max(A[i, j], B[i, j])" if c == "max" else f"A[i, j] {c} B[i, j]"
         return einsum(f"C[i, j] = {rhs}", A=A, B=B)
     raise L.UnsupportedError(L.EUNSUP, f"plan {expr.op} is not routed to the backend")
+
+
+# ---- format conversion ---------------------------------------------------------------------
+
+def permutedims(t, perm=(2, 1)):
+    """permutedims of a 2-D device matrix (src/interface/copy.jl:64-71).
+
+    A Dense(SparseList) tensor gives the transposed matrix in the same format,
+    built on the device; a swizzled CSR view gives its underlying tensor."""
+    perm = tuple(perm)
+    if perm == (1, 2):
+        return t
+    if perm != (2, 1):
+        raise L.UnsupportedError(L.EUNSUP, f"permutation {perm} of a matrix")
+    k, v = _kind(t)
+    if k == "csc":
+        return _sparse_tensor(ops.transpose(_compressed(v)), v.fill_value)
+    if k == "csr":
+        return v
+    raise L.UnsupportedError(L.EUNSUP, "permutedims of a dense operand is not routed")
+
+
+def fsparse(I: torch.Tensor, J: torch.Tensor, V: torch.Tensor, shape=None) -> Tensor:
+    """fsparse(I, J, V, (m, n)) (src/interface/fsparse.jl:64-97) copied into
+    Tensor(Dense(SparseList(Element(0)))), assembled on the device.
+
+    I and J are 1-based device integer tensors.  Entries are ordered column
+    by column, rows ascending; duplicates are summed left to right in input
+    order."""
+    if shape is None:
+        shape = (int(I.max()), int(J.max())) if I.numel() else (0, 0)
+    m, n = (int(x) for x in shape)
+    C = ops.coo_to_compressed(J.contiguous(), I.contiguous(), V.contiguous(), n, m, 1)
+    return _sparse_tensor(C)
+
+
+def dropfills(t, fill=None):
+    """dropfills(t) (src/interface/copy.jl:73-117): the same format without
+    the stored entries that are isequal to the fill value."""
+    k, v = _kind(t)
+    if k not in ("csc", "csr"):
+        raise L.UnsupportedError(L.EUNSUP, "dropfills of a dense operand is not routed")
+    f = v.fill_value if fill is None else fill
+    out = _sparse_tensor(ops.dropfills(_compressed(v), float(f)), f)
+    return SwizzleArray(out, (2, 1)) if k == "csr" else out

@@ -122,3 +122,79 @@ def test_fused_sddmm_compute(dev):
     C = fi.compute(fi.lazy(swizzle(Mt, 2, 1)) * (fi.lazy(X) @ fi.lazy(Y)))
     ref = S.toarray().astype(np.float64) * (Xn.astype(np.float64) @ Yn.astype(np.float64))
     assert rel_err(C.todense(), ref) <= 1e-5
+
+
+# ---- storage order and format conversion ---------------------------------------------
+
+def test_spmm_with_column_compressed_operand(dev):
+    # @einsum C[i,k] += A[i,j]*B[j,k] with A in Finch's default CSC: the
+    # operand is transposed on the device to row order, then SpMM runs.
+    S = _rand(300, 200, 0.05, 11)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    Bn = np.random.default_rng(12).standard_normal((200, 9))
+    B = transfer(GPU(0), Tensor(Dense(Dense(Element(0.0))), Bn))
+    C = fi.einsum("C[i, k] += A[i, j] * B[j, k]", A=A, B=B)
+    assert C.shape == (300, 9)
+    assert rel_err(C.todense(), S @ Bn) <= 1e-12
+    # The row-ordered copy is cached on the tensor.
+    assert getattr(A, "_other_order", None) is not None
+
+
+@pytest.mark.parametrize("prog,fn", [("C[i, j] = A[i, j] * B[i, j]", lambda a, b: a * b),
+                                     ("C[i, j] = A[i, j] + B[i, j]", lambda a, b: a + b)])
+def test_elementwise_mixed_storage_orders(dev, prog, fn):
+    S, T = _rand(60, 50, 0.1, 13), _rand(60, 50, 0.2, 14)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    Bt = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T.T.tocsc()))
+    C = fi.einsum(prog, A=A, B=swizzle(Bt, 2, 1))
+    assert np.array_equal(C.todense(), fn(S.toarray(), T.toarray()))
+
+
+def test_spgemm_mixed_storage_orders(dev):
+    S, T = _rand(70, 40, 0.1, 15), _rand(40, 55, 0.1, 16)
+    At = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S.T.tocsc()))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T))
+    C = fi.einsum("C[i, k] += A[i, j] * B[j, k]", A=swizzle(At, 2, 1), B=B)
+    assert rel_err(C.todense(), (S @ T).toarray()) <= 1e-12
+
+
+def test_fused_sddmm_on_column_compressed_pattern(dev):
+    S = _rand(80, 70, 0.1, 17, np.float32)
+    M = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.float32(0)))), S))
+    rng = np.random.default_rng(18)
+    Xn = rng.standard_normal((80, 16)).astype(np.float32)
+    Yn = rng.standard_normal((16, 70)).astype(np.float32)
+    X = transfer(GPU(0), Tensor(Dense(Dense(Element(np.float32(0)))), Xn))
+    Y = transfer(GPU(0), Tensor(Dense(Dense(Element(np.float32(0)))), Yn))
+    C = fi.compute(fi.lazy(M) * (fi.lazy(X) @ fi.lazy(Y)))
+    ref = S.toarray().astype(np.float64) * (Xn.astype(np.float64) @ Yn.astype(np.float64))
+    assert C.format == "csc"
+    assert rel_err(C.todense(), ref) <= 1e-5
+
+
+def test_permutedims_fsparse_dropfills(dev, goldens):
+    # permutedims (copy.jl:64-71) of a CSC tensor is the transpose in the
+    # same format; fsparse (fsparse.jl:64-97) of a representation golden's
+    # nonzeros rebuilds Finch's buffers; dropfills removes stored zeros.
+    S = _rand(45, 30, 0.2, 19)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    P = fi.permutedims(A)
+    assert P.shape == (30, 45) and P.format == "csc"
+    assert np.array_equal(P.todense(), S.toarray().T)
+    assert np.array_equal(fi.permutedims(P).todense(), S.toarray())
+    for c in goldens["representation_dense_sparselist"]["cases"]:
+        D = np.array(c["dense"], dtype=np.float64)
+        I, J = np.nonzero(D)
+        p = np.random.default_rng(20).permutation(len(I))
+        dev_i = lambda a: torch.as_tensor(a + 1, dtype=torch.int64).to(dev)
+        T = fi.fsparse(dev_i(I[p]), dev_i(J[p]), torch.as_tensor(D[I[p], J[p]]).to(dev), D.shape)
+        sl = T.lvl.lvl
+        assert sl.ptr.tolist() == c["ptr"] and sl.idx.tolist() == c["idx"], c["name"]
+        assert T.element.val.tolist() == c["val"], c["name"]
+    E = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    E.element.val = E.element.val * (torch.arange(E.element.val.numel(), device=dev) % 3 != 0)
+    F = fi.dropfills(E)
+    kept = E.element.val.cpu().numpy()
+    assert F.countstored() == int(np.count_nonzero(kept != 0.0) + np.count_nonzero(
+        (kept == 0.0) & np.signbit(kept)))
+    assert np.array_equal(F.todense(), E.todense())
