@@ -197,4 +197,55 @@ function spgemm(dev::GPU, A::Tensor, B::Tensor; semiring::Symbol = :plus_times)
     Tensor(DenseLevel(SparseListLevel(ElementLevel{fill}(val), A.lvl.lvl.shape, ptr, idx), B.lvl.shape))
 end
 
+# ---- format conversion on the device --------------------------------------------------------
+# permutedims(A) of a Dense(SparseList) tensor (src/interface/copy.jl:64-71):
+# the same matrix stored along its other dimension; the transpose concordize
+# inserts (src/scheduler/optimize.jl:396-452) when loop and storage order differ.
+function Base.permutedims(dev::GPU, A::Tensor)
+    d = descriptor(A)
+    ptr = GPUVector{eltype(A.lvl.lvl.ptr)}(d.n_inner + 1)
+    idx = GPUVector{eltype(A.lvl.lvl.idx)}(d.nnz)
+    val = GPUVector{eltype(A.lvl.lvl.lvl.val)}(d.nnz)
+    check(ccall((:fgx_transpose, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, d, ptr.ptr, idx.ptr, val.ptr, C_NULL))
+    Tensor(DenseLevel(SparseListLevel(ElementLevel{Finch.fill_value(A)}(val), A.lvl.shape, ptr, idx),
+                      A.lvl.lvl.shape))
+end
+
+# Tensor(Dense(SparseList(Element(0))), fsparse(I, J, V, (m, n))) assembled on
+# the device (src/interface/fsparse.jl:64-97): column fibres, rows sorted,
+# duplicates summed left to right in input order.
+function fsparse_csc(dev::GPU, I::GPUVector{Ti}, J::GPUVector{Ti}, V::GPUVector{Tv},
+                     shape::Tuple{Integer,Integer}) where {Ti,Tv}
+    m, n = shape
+    ptr = GPUVector{Ti}(n + 1)
+    idx = GPUVector{Ti}(length(V))
+    val = GPUVector{Tv}(length(V))
+    nnz = Ref{Int64}(0)
+    check(ccall((:fgx_coo_to_compressed, libfgx), Cint,
+                (Ptr{Cvoid}, Int64, Int64, Int64, Cint, Cint, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Int64}, Ptr{Cvoid}),
+                dev.ctx, n, m, length(V), fgx_dtype(Ti), fgx_dtype(Tv), 1, J.ptr, I.ptr, V.ptr,
+                ptr.ptr, idx.ptr, val.ptr, nnz, C_NULL))
+    resize!(idx, nnz[]); resize!(val, nnz[])       # freeze: shrink to the distinct count
+    Tensor(DenseLevel(SparseListLevel(ElementLevel{zero(Tv)}(val), m, ptr, idx), n))
+end
+
+# dropfills(A) (src/interface/copy.jl:73-117) into the same format.
+function Finch.dropfills(dev::GPU, A::Tensor)
+    d = descriptor(A)
+    ptr = GPUVector{eltype(A.lvl.lvl.ptr)}(d.n_outer + 1)
+    idx = GPUVector{eltype(A.lvl.lvl.idx)}(d.nnz)
+    val = GPUVector{eltype(A.lvl.lvl.lvl.val)}(d.nnz)
+    nnz = Ref{Int64}(0)
+    check(ccall((:fgx_dropfills, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Cdouble, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ref{Int64}, Ptr{Cvoid}),
+                dev.ctx, d, Float64(Finch.fill_value(A)), ptr.ptr, idx.ptr, val.ptr, nnz, C_NULL))
+    resize!(idx, nnz[]); resize!(val, nnz[])
+    Tensor(DenseLevel(SparseListLevel(ElementLevel{Finch.fill_value(A)}(val), A.lvl.lvl.shape, ptr, idx),
+                      A.lvl.shape))
+end
+
 end # module
