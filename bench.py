@@ -461,23 +461,48 @@ def bench_serving(dev, slo_ms, large):
     operands32 = small + big
     operands16 = [A.to_values(torch.bfloat16) for A in operands32]
     inputs = serve.Inputs(operands32, seed=1)
-    qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 85_000, 100_000,
-                120_000, 140_000, 200_000, 300_000]
+    qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 100_000, 140_000,
+                200_000, 300_000]
+
+    def run(ops_, mode, q, seed):
+        n = int(min(20_000, max(2_000, q * 0.25)))
+        reqs = serve.make_trace(operands32, inputs, qps=q, n=n, seed=seed)
+        s = serve.serve(ops_, reqs, mode).summary()
+        return {"offered_qps": q, "req_per_s": round(s["req_per_s"], 1),
+                "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+                "launches": s["launches"], "ok": s["p99_ms"] <= slo_ms and s["rejected"] == 0}
+
+    def passes(ops_, mode, q, points):
+        # A rate passes when two runs (different arrival draws) both keep p99
+        # within the target; the slower of the two is recorded.
+        a = run(ops_, mode, q, 7)
+        points.append(a)
+        if not a["ok"]:
+            return None
+        b = run(ops_, mode, q, 8)
+        points.append(b)
+        if not b["ok"]:
+            return None
+        return min((a, b), key=lambda r: r["req_per_s"]) | {"p99_ms": max(a["p99_ms"], b["p99_ms"])}
 
     def sweep(ops_, mode):
-        best, points = None, []
+        run(ops_, mode, 20_000, 1)          # warm-up (plans, tables, scratch)
+        best, points, lo, hi = None, [], None, None
         for q in qps_grid:
-            n = int(min(20_000, max(2_000, q * 0.25)))
-            reqs = serve.make_trace(operands32, inputs, qps=q, n=n, seed=7)
-            s = serve.serve(ops_, reqs, mode).summary()
-            points.append({"offered_qps": q, "req_per_s": round(s["req_per_s"], 1),
-                           "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
-                           "launches": s["launches"]})
-            if s["p99_ms"] <= slo_ms and s["rejected"] == 0:
-                if best is None or s["req_per_s"] > best["req_per_s"]:
-                    best = points[-1]
-            else:
+            r = passes(ops_, mode, q, points)
+            if r is None:
+                hi = q
                 break
+            best, lo = r, q
+        # Bisect between the last passing and the first failing rate.
+        if lo is not None and hi is not None:
+            for _ in range(4):
+                mid = (lo + hi) // 2
+                r = passes(ops_, mode, mid, points)
+                if r is None:
+                    hi = mid
+                else:
+                    best, lo = r, mid
         return best, points
 
     res = {}

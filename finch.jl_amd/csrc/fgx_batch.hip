@@ -142,6 +142,10 @@ static void free_plan_buffers(fgx_batch_plan* p) {
   if (p->d_carry_val) (void)hipFree(p->d_carry_val);
   if (p->d_tables) (void)hipFree(p->d_tables);
   p->d_tables = nullptr;
+  for (void* b : p->retired_dev) (void)hipFree(b);
+  for (void* b : p->retired_host) (void)hipHostFree(b);
+  p->retired_dev.clear();
+  p->retired_host.clear();
   p->exec = nullptr;
   p->graph = nullptr;
   p->capture_stream = nullptr;
@@ -203,10 +207,10 @@ int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const
 }
 
 template <typename T>
-static int grow_device(T** buf, int64_t* cap, int64_t need) {
+static int grow_device(fgx_batch_plan* p, T** buf, int64_t* cap, int64_t need) {
   if (need <= *cap && *buf) return FGX_OK;
   int64_t want = std::max<int64_t>(need, std::max<int64_t>(2 * *cap, 64));
-  if (*buf) (void)hipFree(*buf);
+  if (*buf) p->retired_dev.push_back(*buf);
   *buf = nullptr;
   *cap = 0;
   if (hipMalloc(reinterpret_cast<void**>(buf), static_cast<size_t>(want) * sizeof(T)) != hipSuccess) {
@@ -223,8 +227,8 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
                         BatchWork* h_work, int64_t nw, const BatchWork* h_fix, int64_t nf,
                         int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
                         hipStream_t s) {
-  FGX_TRY(grow_device(&p->d_carry_row, &p->cap_crow, carry_tiles));
-  FGX_TRY(grow_device(&p->d_carry_val, &p->cap_cval, carry_vals));
+  FGX_TRY(grow_device(p, &p->d_carry_row, &p->cap_crow, carry_tiles));
+  FGX_TRY(grow_device(p, &p->d_carry_val, &p->cap_cval, carry_vals));
   order_by_class(p, h_desc, h_work, nw);
   p->segmented = true;
   // Descriptors in entry order (entry u's at u, with its kind), so a wave
@@ -291,7 +295,7 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   const size_t bytes = o_seg + seg.size() * sizeof(int64_t);
   if (static_cast<int64_t>(bytes) > p->cap_hstage) {
     const size_t want = std::max(bytes, static_cast<size_t>(2 * p->cap_hstage) + 4096);
-    if (p->h_stage) (void)hipHostFree(p->h_stage);
+    if (p->h_stage) p->retired_host.push_back(p->h_stage);
     p->h_stage = nullptr;
     p->cap_hstage = 0;
     if (hipHostMalloc(&p->h_stage, want) != hipSuccess) {
@@ -303,7 +307,7 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   }
   if (static_cast<int64_t>(bytes) > p->cap_stage) {
     const size_t want = std::max(bytes, static_cast<size_t>(2 * p->cap_stage) + 4096);
-    if (p->d_stage) (void)hipFree(p->d_stage);
+    if (p->d_stage) p->retired_dev.push_back(p->d_stage);
     p->d_stage = nullptr;
     p->cap_stage = 0;
     if (hipMalloc(&p->d_stage, want) != hipSuccess) {
@@ -337,6 +341,38 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
 }
 
 void batch_free(fgx_batch_plan* p) { free_plan_buffers(p); }
+
+int batch_reserve(fgx_batch_plan* p, int64_t entries, int64_t carry_tiles, int64_t carry_vals) {
+  FGX_TRY(grow_device(p, &p->d_carry_row, &p->cap_crow, carry_tiles));
+  FGX_TRY(grow_device(p, &p->d_carry_val, &p->cap_cval, carry_vals));
+  // Staging: descriptors, work and fix-up entries and the segment offsets
+  // of `entries` entries (the layout of batch_launch_staged, with slack for
+  // its 16-byte alignment).
+  const size_t bytes = static_cast<size_t>(entries) *
+                           (sizeof(BatchDesc) + 2 * sizeof(BatchWork) + 2 * sizeof(int64_t)) +
+                       4096;
+  if (static_cast<int64_t>(bytes) > p->cap_hstage) {
+    if (p->h_stage) p->retired_host.push_back(p->h_stage);
+    p->h_stage = nullptr;
+    p->cap_hstage = 0;
+    if (hipHostMalloc(&p->h_stage, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(FGX_EOOM, "batch: pinned staging allocation failed");
+    }
+    p->cap_hstage = static_cast<int64_t>(bytes);
+  }
+  if (static_cast<int64_t>(bytes) > p->cap_stage) {
+    if (p->d_stage) p->retired_dev.push_back(p->d_stage);
+    p->d_stage = nullptr;
+    p->cap_stage = 0;
+    if (hipMalloc(&p->d_stage, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return set_error(FGX_EOOM, "batch: staging allocation failed");
+    }
+    p->cap_stage = static_cast<int64_t>(bytes);
+  }
+  return FGX_OK;
+}
 
 int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out,
                     hipStream_t s) {

@@ -13,6 +13,7 @@
 // class's units in a fixed stride, so the launches (and their hipGraph) are
 // the same on every replay and results do not depend on timing.
 #pragma once
+#include <vector>
 #include "fgx_csr_tiles.hpp"
 #include "fgx_internal.hpp"
 
@@ -134,6 +135,10 @@ struct fgx_batch_plan {
   bool has_sddmm = false;
   int64_t cap_desc = 0, cap_work = 0, cap_fix = 0, cap_crow = 0, cap_cval = 0;
   hipStream_t capture_stream = nullptr;
+  // Buffers replaced by a larger one while the plan is in use.  They are
+  // freed with the plan, not at the replacement: hipFree waits for the whole
+  // device, which would stall every other stream of a serving loop.
+  std::vector<void*> retired_dev, retired_host;
 };
 
 namespace fgx {
@@ -151,6 +156,11 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
                         int64_t carry_tiles, int64_t carry_vals, bool has_sddmm, int num_cus,
                         hipStream_t s);
 void batch_free(fgx_batch_plan* p);
+// Sizes a plan's carry and staging buffers up front (for a launch of up to
+// `entries` work entries, `carry_tiles` carries and `carry_vals` carry
+// values), so that later launches allocate nothing: a reallocation frees
+// device memory, and hipFree waits for the device.
+int batch_reserve(fgx_batch_plan* p, int64_t entries, int64_t carry_tiles, int64_t carry_vals);
 // Maximum row length of each compressed operand (one block per operand).
 int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* out, hipStream_t s);
 // Sorts work units by class (stable, so each class keeps the host's order)
