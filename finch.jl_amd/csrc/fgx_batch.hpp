@@ -83,9 +83,15 @@ __host__ __device__ inline int unit_class(const BatchDesc& d) {
   return d.op == BO_SDDMM ? kClsSddmm : (d.k == 1 ? kClsSpmv : group_for_k(d.k));
 }
 
+// Positions per lane run of a batched SpMV (width 1) tile.
+#ifndef FGX_BATCH_SPMV_RUN
+#define FGX_BATCH_SPMV_RUN 8
+#endif
+constexpr int kBatchSpmvRun = FGX_BATCH_SPMV_RUN;
+
 // Positions per TILES tile for a width k.
 __host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
-  if (k == 1) return kWave * 8;
+  if (k == 1) return kWave * kBatchSpmvRun;
   const int g = group_for_k(k);
   return static_cast<int64_t>(kWave / g) * tile_e_for_g(g);
 }

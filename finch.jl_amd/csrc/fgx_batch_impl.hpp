@@ -149,7 +149,7 @@ template <typename TP, typename TI, typename TV>
 __device__ __forceinline__ void tiles_spmv(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
   for (int64_t t = t0; t < t1; t++)
-    spmv_tile<TP, TI, TV, float, 8>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+    spmv_tile<TP, TI, TV, float, kBatchSpmvRun>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
                                     static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.B, d.C,
                                     t, d.vec != 0, d.table, lds, carry_row + d.carry_tile0 + t,
                                     carry_val + d.carry_val0 + t);

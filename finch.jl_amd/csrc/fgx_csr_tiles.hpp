@@ -159,26 +159,41 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
   int64_t first_row = -1;
   TA first_val = 0;
   int64_t r = active ? (a == 0 ? 0 : win.find(a)) : r_hi;
+  // Rows that end inside the lane's run are stored as they end, except empty
+  // rows (same end as the row before): those are zeroed below by the whole
+  // wave, with coalesced stores, instead of one scattered store per lane.
+  int64_t last_end = -1;
   if (active) {
 #pragma unroll
     for (int s = 0; s < V; s++) {
       const int64_t q = a + s;
       if (q < b) {
-        while (r < m && r <= r_win_last && win.end_of(r) <= q) {
+        int64_t e;
+        while (r < m && r <= r_win_last && (e = win.end_of(r)) <= q) {
           if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
-          else y[r] = acc;
+          else if (e != last_end) y[r] = acc;
+          last_end = e;
           acc = 0;
           r++;
         }
         acc += widen<TA>(vv[s]) * xg[s];
       }
     }
-    while (r < m && r <= r_win_last && win.end_of(r) <= b) {
+    int64_t e;
+    while (r < m && r <= r_win_last && (e = win.end_of(r)) <= b) {
       if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
-      else y[r] = acc;
+      else if (e != last_end) y[r] = acc;
+      last_end = e;
       acc = 0;
       r++;
     }
+  }
+  // Empty rows the tile owns: rows ending in (q0, qend] (from position 0 on
+  // for the first tile), i.e. [r_lo, r_hi); row r is empty when it ends where
+  // row r - 1 does (row r_lo of a later tile holds position q0).
+  for (int64_t rr = r_lo + (tile > 0 ? 1 : 0) + lane; rr < r_hi && rr < m; rr += kWave) {
+    const int64_t prev = rr == 0 ? 0 : win.end_of(rr - 1);
+    if (win.end_of(rr) == prev) y[rr] = TA(0);
   }
   // Segmented inclusive scan of the runs' carries; a run with an emit starts
   // a new segment.
