@@ -52,7 +52,10 @@ int dense_scratch(fgx_ctx* ctx, size_t bytes, void** out) {
   return grow_arena(ctx, &ctx->dscratch, &ctx->dscratch_bytes, bytes, out);
 }
 
-constexpr size_t kMaxTables = 32;
+// Enough for a serving-sized operand set (hundreds of operands, a few tile
+// widths each): a table evicted and rebuilt on every call would also free
+// device memory, which waits for the device.
+constexpr size_t kMaxTables = 1024;
 
 void free_tile_tables(fgx_ctx* ctx) {
   if (ctx->tables.empty()) return;

@@ -31,7 +31,10 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int kServeStreams = 4;
+#ifndef FGX_SERVE_STREAMS
+#define FGX_SERVE_STREAMS 4
+#endif
+constexpr int kServeStreams = FGX_SERVE_STREAMS;
 // A request whose algorithmic bytes exceed kLargeRequestBytes is large: it
 // only ever opens a launch (small requests may follow it in that launch); a
 // large request met while a launch is filling keeps its place for the next
