@@ -237,7 +237,8 @@ DEF_EWISE(int32_t, i32)
  * fibers are CSR rows (the same nest over swizzled operands): for row i of A,
  * for stored k (ascending), for stored j of row k of B.
  * Semiring 0: plus-times (w starts at 0; `w = a*b + w`).
- * Semiring 1: min-plus (w starts at +Inf; `w = min(w, a + b)`), the
+ * Semiring 1: min-plus (w starts at +Inf; `w = min(w, a + b)` with Julia's
+ *             `min`, JL_MIN below), the
  *             tropical product with fill Inf (src/symbolic/symbolic.jl:146).
  * Int32 arithmetic wraps, as Julia's Int32 does.
  * ------------------------------------------------------------------------- */
@@ -271,6 +272,20 @@ i64 or_spgemm_symbolic(i64 m, i64 n, const i64* aptr, const i64* aidx, const i64
   return total;
 }
 
+/* Julia's `min` on floats (base/math.jl): NaN wins (returned as the quiet
+ * NaN), -0.0 is below 0.0, otherwise the smaller value.  With these rules
+ * `min` is associative and commutative, so a min-plus fold does not depend
+ * on the order of its products. */
+static inline float JL_MIN_f32(float x, float y) {
+  if (isnan(x) || isnan(y)) return NAN;
+  if (y < x || (y == x && signbit(y) && !signbit(x))) return y;
+  return x;
+}
+static inline double JL_MIN_f64(double x, double y) {
+  if (isnan(x) || isnan(y)) return (double)NAN;
+  if (y < x || (y == x && signbit(y) && !signbit(x))) return y;
+  return x;
+}
 #define DEF_SPGEMM(T, SFX, INF)                                                          \
   void or_spgemm_##SFX(int semiring, i64 m, i64 n, const i64* aptr, const i64* aidx,     \
                        const T* aval, const i64* bptr, const i64* bidx, const T* bval,   \
@@ -290,8 +305,7 @@ i64 or_spgemm_symbolic(i64 m, i64 n, const i64* aptr, const i64* aidx, const i64
           if (semiring == 0) {                                                           \
             w[j] = SPG_MUL_##SFX(a, bval[qb]) + w[j];                                    \
           } else {                                                                       \
-            T s = a + bval[qb];                                                          \
-            w[j] = s < w[j] ? s : w[j];                                                  \
+            w[j] = JL_MIN_##SFX(w[j], a + bval[qb]);                                     \
           }                                                                              \
           if (!tbl[j]) { tbl[j] = 1; srt[fill++] = j; }                                  \
         }                                                                                \

@@ -183,3 +183,24 @@ def test_spgemm_empty(dev):
     e = np.zeros(0, np.int64)
     _gemm_check(dev, "plus_times", ep, e, np.zeros(0, np.float32), 5, 4, np.ones(5, np.int64), e,
                 np.zeros(0, np.float32), 7)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spgemm_min_plus_julia_min(dev, dtype):
+    # Julia's `min`: -0.0 is below 0.0 and NaN wins, in every product order
+    # (wave rows here; the all-bins test covers the block and window bins).
+    B_ptr, B_idx = np.array([1, 2, 3]), np.array([1, 1])
+    cases = [((-0.0, 0.0), (-0.0, 0.0)), ((0.0, -0.0), (0.0, -0.0)),
+             ((1.0, np.nan), (0.0, 0.0)), ((np.nan, 1.0), (0.0, 0.0))]
+    for a, b in cases:
+        A = operand([1, 3], [1, 2], np.array(a, dtype), 1, 2, dev)
+        B = operand(B_ptr, B_idx, np.array(b, dtype), 2, 1, dev)
+        C = ops.spgemm("min_plus", A, B)
+        p, i, v = oracle.spgemm("min_plus", 1, 1, np.array([1, 3]), np.array([1, 2]),
+                                np.array(a, dtype), B_ptr, B_idx, np.array(b, dtype))
+        got = C.val.cpu().numpy()
+        assert C.ptr.cpu().tolist() == p.tolist() and C.idx.cpu().tolist() == i.tolist()
+        if np.isnan(v[0]):
+            assert np.isnan(got[0])
+        else:
+            assert got[0] == v[0] and np.signbit(got[0]) == np.signbit(v[0])

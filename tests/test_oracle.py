@@ -316,3 +316,19 @@ def test_coo_matches_scipy_sum_duplicates():
     S.sum_duplicates()
     assert np.array_equal(cp - 1, S.indptr) and np.array_equal(ci - 1, S.indices)
     assert np.array_equal(cv, S.data)
+
+
+def test_min_plus_uses_julia_min():
+    # `<<min>>=` folds with Julia's `min` (base/math.jl): NaN wins and -0.0 is
+    # below 0.0, so the fold gives the same result in either product order.
+    B_ptr, B_idx = np.array([1, 2, 3]), np.array([1, 1])
+    for order in ((-0.0, 0.0), (0.0, -0.0)):
+        # Products A[0,k] + B[k,0]: -0.0 + -0.0 and 0.0 + 0.0, in both orders.
+        p, i, v = oracle.spgemm("min_plus", 1, 1, np.array([1, 3]), np.array([1, 2]),
+                                np.array(order), B_ptr, B_idx, np.array(order))
+        assert p.tolist() == [1, 2] and i.tolist() == [1]
+        assert v[0] == 0.0 and np.signbit(v[0])
+    for order in ((1.0, np.nan), (np.nan, 1.0)):
+        p, i, v = oracle.spgemm("min_plus", 1, 1, np.array([1, 3]), np.array([1, 2]),
+                                np.array(order), B_ptr, B_idx, np.array([0.0, 0.0]))
+        assert np.isnan(v[0])
