@@ -30,6 +30,7 @@
 namespace fgx {
 
 constexpr int kWaveCap = 256;    // products (and lists) per wave-team row
+constexpr int kWide = 512;       // products per wide wave row (numeric register sort)
 constexpr int kBlockCap = 1536;  // products (and lists) per block-team row
 constexpr int kWindow = 4096;    // columns per pass of the dense fallback
 
@@ -253,6 +254,156 @@ __global__ __launch_bounds__(256) void spgemm_block_kernel(
                                         bval, cnt, cptr, cidx, cval, L);
 }
 
+// ---- wave rows, numeric: register sort ------------------------------------------------
+// A row of at most 64*PER products (PER = 4 or 8) as (key, value) pairs,
+// PER per lane (element e = PER * lane + r): key = column << PB | product
+// index, where the product index p (PB bits) counts the row's products in
+// the reference's order (A position, then B position).  A bitonic sort of
+// the keys in registers (shuffles across lanes, swaps within a lane) leaves
+// each column's products in that order, so folding each run left to right
+// is the reference's update sequence, bit for bit.  The key is 31 bits, so
+// columns must be below 2^(31 - PB).
+template <int PER>
+struct SortShape {
+  static constexpr int kCap = 64 * PER;
+  static constexpr int kPBits = PER == 4 ? 8 : 9;
+  static constexpr int64_t kMaxCols = int64_t(1) << (31 - kPBits);
+};
+
+template <int PER, typename TV>
+struct WaveSortLds {
+  int off[SortShape<PER>::kCap + 1];
+  int lk[SortShape<PER>::kCap];
+  TV la[SortShape<PER>::kCap];
+  uint32_t sk[SortShape<PER>::kCap];
+  TV sv[SortShape<PER>::kCap];
+};
+
+template <int PER, int SR, typename TP, typename TI, typename TV>
+__device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restrict__ aptr,
+                         const TI* __restrict__ aidx, const TV* __restrict__ aval,
+                         const TP* __restrict__ bptr, const TI* __restrict__ bidx,
+                         const TV* __restrict__ bval, const TP* __restrict__ cptr,
+                         TI* __restrict__ cidx, TV* __restrict__ cval, WaveSortLds<PER, TV>& L) {
+  constexpr int N = SortShape<PER>::kCap;
+  constexpr int PB = SortShape<PER>::kPBits;
+  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+  const int nl = static_cast<int>(a1 - a0);
+  // 1. List offsets.
+  int run = 0;
+  for (int c = 0; c < nl; c += kWave) {
+    const int t = c + lane;
+    int len = 0;
+    if (t < nl) {
+      const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
+      len = static_cast<int>(pos_at(bptr, base, k + 1) - pos_at(bptr, base, k));
+      L.lk[t] = k;
+      L.la[t] = aval[a0 + t];
+    }
+    int tot;
+    const int ex = team_scan<64>(len, lane, nullptr, &tot);
+    if (t < nl) L.off[t] = run + ex;
+    run += tot;
+  }
+  const int np = run;
+  if (lane == 0) L.off[nl] = np;
+  team_sync<64>();
+  // 2. This lane's products (padding keys sort last).
+  uint32_t key[PER];
+  TV val[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int p = PER * lane + r;
+    key[r] = 0xffffffffu;
+    val[r] = TV(0);
+    if (p < np) {
+      const int t = list_of(L.off, nl, p);
+      const int64_t q = pos_at(bptr, base, L.lk[t]) + (p - L.off[t]);
+      const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bidx[q]) - base);
+      key[r] = (col << PB) | static_cast<uint32_t>(p);
+      val[r] = sr_mul<TV, SR>(L.la[t], bval[q]);
+    }
+  }
+  // 3. Bitonic sort of the N keys (all distinct), ascending.
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= PER) {
+        const int d = j / PER;
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+          const int e = PER * lane + r;
+          const uint32_t pk = __shfl_xor(key[r], d, 64);
+          const TV pv = __shfl_xor(val[r], d, 64);
+          const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
+          const bool take = keep_min ? (pk < key[r]) : (pk > key[r]);
+          if (take) {
+            key[r] = pk;
+            val[r] = pv;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < PER; r++) {
+          if (r & j) continue;
+          const int r2 = r | j;
+          const int e = PER * lane + r;
+          const bool up = (e & k) == 0;
+          if ((key[r] > key[r2]) == up) {
+            const uint32_t tk = key[r];
+            key[r] = key[r2];
+            key[r2] = tk;
+            const TV tv = val[r];
+            val[r] = val[r2];
+            val[r2] = tv;
+          }
+        }
+      }
+    }
+  }
+  // 4. Fold each column's run in order and write it at its rank.
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    L.sk[PER * lane + r] = key[r];
+    L.sv[PER * lane + r] = val[r];
+  }
+  team_sync<64>();
+  const int64_t o = pos_at(cptr, base, row);
+  int seen = 0;
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    // Rounds of 64 consecutive sorted positions.
+    const int e = r * kWave + lane;
+    const uint32_t col = L.sk[e] >> PB;
+    const bool head = e < np && (e == 0 || (L.sk[e - 1] >> PB) != col);
+    int tot;
+    const int ex = team_scan<64>(head ? 1 : 0, lane, nullptr, &tot);
+    if (head) {
+      TV w = sr_zero<TV, SR>();
+      for (int x = e; x < np && (L.sk[x] >> PB) == col; x++) w = sr_add<TV, SR>(w, L.sv[x]);
+      cidx[o + seen + ex] = static_cast<TI>(static_cast<int64_t>(col) + base);
+      cval[o + seen + ex] = w;
+    }
+    seen += tot;
+  }
+}
+
+// Four wave rows per 256-thread block.
+template <int PER, int SR, typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(256) void spgemm_wave_sort_kernel(
+    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
+    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
+    const TI* __restrict__ bidx, const TV* __restrict__ bval, const TP* __restrict__ cptr,
+    TI* __restrict__ cidx, TV* __restrict__ cval) {
+  __shared__ WaveSortLds<PER, TV> lds[4];
+  const int w = wave_in_block();
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  if (r >= nrows) return;
+  sort_row<PER, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr, bidx, bval, cptr,
+                    cidx, cval, lds[w]);
+}
+
 // ---- hash teams (symbolic pass) ---------------------------------------------------------
 constexpr int kWaveSlots = 512;    // 2 x kWaveCap: at most half full
 constexpr int kBlockSlots = 4096;  // >= 2 x kBlockCap
@@ -442,20 +593,26 @@ __global__ __launch_bounds__(256) void spgemm_window_kernel(
   }
 }
 
-// Row bins: 0 wave team, 1 block team, 2 dense window.  Slots are taken
-// with one atomic per wave and bin (ballot + rank), not one per row: a
-// single counter hit by every row serialises the whole pass.  The order of
-// rows within a bin does not affect any output.
+// Row bins by products: 0 up to 256 (wave), 1 up to 512 (wide wave), 2 up
+// to 1536 (block), 3 beyond (dense window).  Slots are taken with one
+// atomic per bin and 1024-row block: the waves' counts (ballots) are
+// combined in LDS first.  One atomic per row, or per wave, on the four
+// shared counters serialises the pass (one cache line takes every update).
+// The order of rows within a bin does not affect any output.
+constexpr int kBinThreads = 1024;
 template <typename TP, typename TI>
-__global__ __launch_bounds__(256) void spgemm_bin_kernel(int64_t m, int64_t base,
-                                                         const TP* __restrict__ aptr,
-                                                         const TI* __restrict__ aidx,
-                                                         const TP* __restrict__ bptr,
-                                                         int64_t* __restrict__ bin_count,
-                                                         int64_t* __restrict__ bin_rows) {
-  const int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBinThreads) void spgemm_bin_kernel(int64_t m, int64_t base,
+                                                                 const TP* __restrict__ aptr,
+                                                                 const TI* __restrict__ aidx,
+                                                                 const TP* __restrict__ bptr,
+                                                                 int64_t* __restrict__ bin_count,
+                                                                 int64_t* __restrict__ bin_rows) {
+  constexpr int kWaves = kBinThreads / kWave;
+  __shared__ int wcnt[kWaves][4];
+  __shared__ int64_t wbase[kWaves][4];
+  const int64_t row = static_cast<int64_t>(blockIdx.x) * kBinThreads + threadIdx.x;
   const bool valid = row < m;
-  int bin = 3;
+  int bin = 4;
   if (valid) {
     const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
     int64_t prod = 0;
@@ -464,28 +621,39 @@ __global__ __launch_bounds__(256) void spgemm_bin_kernel(int64_t m, int64_t base
       prod += pos_at(bptr, base, k + 1) - pos_at(bptr, base, k);
     }
     const int64_t key = prod > a1 - a0 ? prod : a1 - a0;
-    bin = key <= kWaveCap ? 0 : (key <= kBlockCap ? 1 : 2);
+    bin = key <= kWaveCap ? 0 : (key <= kWide ? 1 : (key <= kBlockCap ? 2 : 3));
   }
   const int lane = lane_id();
+  const int w = wave_in_block();
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t mask[4];
 #pragma unroll
-  for (int b = 0; b < 3; b++) {
-    const uint64_t mask = ballot(bin == b);
-    if (mask == 0) continue;
-    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-    unsigned long long first = 0;
-    if (lane == leader)
-      first = atomicAdd(reinterpret_cast<unsigned long long*>(bin_count + b),
-                        static_cast<unsigned long long>(__popcll(mask)));
-    first = shfl(first, leader);
-    if (bin == b)
-      bin_rows[b * m + static_cast<int64_t>(first) + __popcll(mask & below)] = row;
+  for (int b = 0; b < 4; b++) {
+    mask[b] = ballot(bin == b);
+    if (lane == 0) wcnt[w][b] = __popcll(mask[b]);
   }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int b = threadIdx.x;
+    int tot = 0;
+    for (int v = 0; v < kWaves; v++) tot += wcnt[v][b];
+    int64_t first = 0;
+    if (tot > 0)
+      first = static_cast<int64_t>(atomicAdd(reinterpret_cast<unsigned long long*>(bin_count + b),
+                                             static_cast<unsigned long long>(tot)));
+    for (int v = 0; v < kWaves; v++) {
+      wbase[v][b] = first;
+      first += wcnt[v][b];
+    }
+  }
+  __syncthreads();
+  if (bin < 4)
+    bin_rows[bin * m + wbase[w][bin] + __popcll(mask[bin] & below)] = row;
 }
 
 struct Bins {
-  int64_t* rows;  // 3 * m
-  int64_t count[3];
+  int64_t* rows;  // 4 * m
+  int64_t count[4];
 };
 
 template <typename TP, typename TI>
@@ -496,15 +664,16 @@ static int make_bins(fgx_ctx* ctx, const fgx_compressed* A, const fgx_compressed
   bins->rows = dev + 4;
   FGX_HIP(hipMemsetAsync(bin_count, 0, 4 * sizeof(int64_t), s));
   if (m > 0) {
-    hipLaunchKernelGGL((spgemm_bin_kernel<TP, TI>), dim3(static_cast<unsigned>((m + 255) / 256)),
-                       dim3(256), 0, s, m, static_cast<int64_t>(A->index_base),
+    hipLaunchKernelGGL((spgemm_bin_kernel<TP, TI>),
+                       dim3(static_cast<unsigned>((m + kBinThreads - 1) / kBinThreads)),
+                       dim3(kBinThreads), 0, s, m, static_cast<int64_t>(A->index_base),
                        static_cast<const TP*>(A->ptr), static_cast<const TI*>(A->idx),
                        static_cast<const TP*>(B->ptr), bin_count, bins->rows);
     FGX_TRY(launch_check("spgemm_bin_kernel"));
   }
-  FGX_HIP(hipMemcpyAsync(ctx->host_i64, bin_count, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  FGX_HIP(hipMemcpyAsync(ctx->host_i64, bin_count, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
   FGX_HIP(hipStreamSynchronize(s));
-  for (int b = 0; b < 3; b++) bins->count[b] = ctx->host_i64[b];
+  for (int b = 0; b < 4; b++) bins->count[b] = ctx->host_i64[b];
   return FGX_OK;
 }
 
@@ -530,34 +699,52 @@ static int run_bins(const fgx_compressed* A, const fgx_compressed* B, const Bins
                          bins.rows, bins.count[0], base, aptr, aidx, bptr, bidx, cnt);
       FGX_TRY(launch_check("spgemm_wave_count_kernel"));
     }
-    if (bins.count[1] > 0) {
+    for (int b = 1; b <= 2; b++) {
+      if (bins.count[b] == 0) continue;
       hipLaunchKernelGGL((spgemm_block_count_kernel<TP, TI>),
-                         dim3(static_cast<unsigned>(bins.count[1])), dim3(256), 0, s,
-                         bins.rows + m, bins.count[1], base, aptr, aidx, bptr, bidx, cnt);
+                         dim3(static_cast<unsigned>(bins.count[b])), dim3(256), 0, s,
+                         bins.rows + b * m, bins.count[b], base, aptr, aidx, bptr, bidx, cnt);
       FGX_TRY(launch_check("spgemm_block_count_kernel"));
     }
   } else {
-    // Numeric: the ordered merge.
-    if (bins.count[0] > 0) {
+    // Numeric: rows of up to 256 / 512 products by the register sort (when
+    // the columns fit its key), the rest by the ordered merge.
+    const bool sort4 = B->n_inner <= SortShape<4>::kMaxCols;
+    const bool sort8 = B->n_inner <= SortShape<8>::kMaxCols;
+    if (bins.count[0] > 0 && sort4) {
+      hipLaunchKernelGGL((spgemm_wave_sort_kernel<4, SR, TP, TI, TV>),
+                         dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
+                         bins.rows, bins.count[0], base, aptr, aidx, aval, bptr, bidx, bval, cp,
+                         ci, cv);
+      FGX_TRY(launch_check("spgemm_wave_sort_kernel"));
+    } else if (bins.count[0] > 0) {
       hipLaunchKernelGGL((spgemm_wave_kernel<NUMERIC, SR, TP, TI, TV>),
                          dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
                          bins.rows, bins.count[0], base, aptr, aidx, aval, bptr, bidx, bval, cnt,
                          cp, ci, cv);
       FGX_TRY(launch_check("spgemm_wave_kernel"));
     }
-    if (bins.count[1] > 0) {
-      const size_t lds = sizeof(TeamLds<kBlockCap, TV>);
-      hipLaunchKernelGGL((spgemm_block_kernel<NUMERIC, SR, TP, TI, TV>),
-                         dim3(static_cast<unsigned>(bins.count[1])), dim3(256), lds, s,
+    if (bins.count[1] > 0 && sort8) {
+      hipLaunchKernelGGL((spgemm_wave_sort_kernel<8, SR, TP, TI, TV>),
+                         dim3(static_cast<unsigned>((bins.count[1] + 3) / 4)), dim3(256), 0, s,
                          bins.rows + m, bins.count[1], base, aptr, aidx, aval, bptr, bidx, bval,
-                         cnt, cp, ci, cv);
+                         cp, ci, cv);
+      FGX_TRY(launch_check("spgemm_wave_sort_kernel"));
+    }
+    const size_t lds = sizeof(TeamLds<kBlockCap, TV>);
+    for (int b = sort8 ? 2 : 1; b <= 2; b++) {
+      if (bins.count[b] == 0) continue;
+      hipLaunchKernelGGL((spgemm_block_kernel<NUMERIC, SR, TP, TI, TV>),
+                         dim3(static_cast<unsigned>(bins.count[b])), dim3(256), lds, s,
+                         bins.rows + b * m, bins.count[b], base, aptr, aidx, aval, bptr, bidx,
+                         bval, cnt, cp, ci, cv);
       FGX_TRY(launch_check("spgemm_block_kernel"));
     }
   }
-  if (bins.count[2] > 0) {
+  if (bins.count[3] > 0) {
     hipLaunchKernelGGL((spgemm_window_kernel<NUMERIC, SR, TP, TI, TV>),
-                       dim3(static_cast<unsigned>(bins.count[2])), dim3(256), 0, s,
-                       bins.rows + 2 * m, bins.count[2], base, B->n_inner, aptr, aidx, aval, bptr,
+                       dim3(static_cast<unsigned>(bins.count[3])), dim3(256), 0, s,
+                       bins.rows + 3 * m, bins.count[3], base, B->n_inner, aptr, aidx, aval, bptr,
                        bidx, bval, cnt, cp, ci, cv);
     FGX_TRY(launch_check("spgemm_window_kernel"));
   }
@@ -607,10 +794,10 @@ int fgx_spgemm_symbolic(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
   hipStream_t s = pick_stream(ctx, stream);
   const int64_t m = A->n_outer;
   void* scr;
-  const size_t need = static_cast<size_t>(4 + 3 * m + m + 1) * sizeof(int64_t) + scan_work_bytes(m);
+  const size_t need = static_cast<size_t>(4 + 4 * m + m + 1) * sizeof(int64_t) + scan_work_bytes(m);
   FGX_TRY(scratch(ctx, need, &scr));
   int64_t* binmem = static_cast<int64_t*>(scr);
-  int64_t* cnt = binmem + 4 + 3 * m;
+  int64_t* cnt = binmem + 4 + 4 * m;
   void* work = cnt + m + 1;
   int64_t total = 0;
   FGX_DISPATCH_IDX(ic, {
@@ -640,7 +827,7 @@ int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
   hipStream_t s = pick_stream(ctx, stream);
   const int64_t m = A->n_outer;
   void* scr;
-  FGX_TRY(scratch(ctx, static_cast<size_t>(4 + 3 * m) * sizeof(int64_t), &scr));
+  FGX_TRY(scratch(ctx, static_cast<size_t>(4 + 4 * m) * sizeof(int64_t), &scr));
   int64_t* binmem = static_cast<int64_t*>(scr);
 #define FGX_GEMM_NUM(SR_, TV_)                                                            \
   FGX_DISPATCH_IDX(ic, {                                                                  \
