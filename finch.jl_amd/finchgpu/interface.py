@@ -306,3 +306,33 @@ def dropfills(t, fill=None):
     f = v.fill_value if fill is None else fill
     out = _sparse_tensor(ops.dropfills(_compressed(v), float(f)), f)
     return SwizzleArray(out, (2, 1)) if k == "csr" else out
+
+
+# ---- SpGEMM formulations (docs/examples/spgemm.jl) ------------------------------------------
+
+def _gemm_csc(A, B):
+    ka, va = _kind(A)
+    kb, vb = _kind(B)
+    if ka != "csc" or kb != "csc":
+        raise L.UnsupportedError(L.EUNSUP, "the SpGEMM examples take Dense(SparseList) operands")
+    return einsum("C[i, k] += A[i, j] * B[j, k]", A=A, B=B)
+
+
+def spgemm_inner(A, B):
+    """docs/examples/spgemm.jl:1-25.  Each C[i, j] sums A[i, k]*B[k, j] over
+    the k of both fibres in ascending order, and C stores the (i, j) with at
+    least one product; that is the column-order Gustavson result exactly,
+    which is how it runs here."""
+    return _gemm_csc(A, B)
+
+
+def spgemm_outer(A, B):
+    """docs/examples/spgemm.jl:27-57.  The outer-product loop adds the k-th
+    outer product for k ascending, so each C[i, j] again sums over k in
+    ascending order: the same values and pattern as spgemm_gustavson."""
+    return _gemm_csc(A, B)
+
+
+def spgemm_gustavson(A, B):
+    """docs/examples/spgemm.jl:59-76, on the device (fgx_spgemm_*)."""
+    return _gemm_csc(A, B)

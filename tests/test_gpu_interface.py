@@ -198,3 +198,23 @@ def test_permutedims_fsparse_dropfills(dev, goldens):
     assert F.countstored() == int(np.count_nonzero(kept != 0.0) + np.count_nonzero(
         (kept == 0.0) & np.signbit(kept)))
     assert np.array_equal(F.todense(), E.todense())
+
+
+def test_spgemm_formulations_match_the_examples_test(dev):
+    # test/suites/examples_tests.jl:103-119: 32x32 Int matrices at p = 0.1;
+    # inner, Gustavson and outer products all equal A*B.
+    rng = np.random.default_rng(21)
+    S = sp.random(32, 32, density=0.1, random_state=rng, format="csc")
+    T = sp.random(32, 32, density=0.1, random_state=rng, format="csc")
+    S.data = rng.integers(-1000, 1000, S.nnz).astype(np.float64)
+    T.data = rng.integers(-1000, 1000, T.nnz).astype(np.float64)
+    Si, Ti = S.astype(np.int32), T.astype(np.int32)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.int32(0)))), Si))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(np.int32(0)))), Ti))
+    ref = (S @ T).toarray()
+    outs = [fn(A, B) for fn in (fi.spgemm_inner, fi.spgemm_gustavson, fi.spgemm_outer)]
+    for C in outs:
+        assert np.array_equal(C.todense().astype(np.float64), ref)
+    sl0 = outs[0].lvl.lvl
+    for C in outs[1:]:
+        assert torch.equal(C.lvl.lvl.ptr, sl0.ptr) and torch.equal(C.lvl.lvl.idx, sl0.idx)
