@@ -46,30 +46,92 @@ __device__ __forceinline__ uint64_t group_mask() {
   return (G == 64) ? ~0ull : (((1ull << G) - 1) << (lane & ~(G - 1)));
 }
 
+// Coordinates staged in LDS per side per group.  A fibre pair that fits is
+// loaded once, coalesced, and searched in LDS; a longer pair is searched in
+// global memory (same results, more round trips).
+template <int G>
+__host__ __device__ constexpr int ew_cap() { return 8 * G; }
+
+template <typename TI>
+__device__ __forceinline__ int lds_lower_bound(const TI* s, int lo, int hi, int64_t key) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(s[mid]) < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Either side of a fibre pair: its coordinates in global memory and, when
+// staged, the same coordinates in LDS (s != nullptr).
+template <typename TI>
+struct EwSide {
+  const TI* g;  // idx + fibre start
+  const TI* s;  // LDS copy, or nullptr
+  int64_t n;
+  __device__ __forceinline__ int64_t at(int64_t i) const {
+    return static_cast<int64_t>(s ? s[i] : g[i]);
+  }
+  // First i in [0, n) with coordinate >= key (n if none).
+  __device__ __forceinline__ int64_t lower(int64_t key) const {
+    return s ? static_cast<int64_t>(lds_lower_bound(s, 0, static_cast<int>(n), key))
+             : lower_bound(g, 0, n, key);
+  }
+};
+
+// Loads the group's two fibres into its LDS slots when both fit; the group's
+// lanes share a wave, so a wave-scope fence orders the stores and the reads.
+template <int G, typename TI>
+__device__ __forceinline__ bool ew_stage(const TI* __restrict__ aidx, int64_t a0, int64_t la,
+                                         const TI* __restrict__ bidx, int64_t b0, int64_t lb,
+                                         TI* sa, TI* sb, int gl) {
+  constexpr int CAP = ew_cap<G>();
+  const bool staged = la <= CAP && lb <= CAP;
+  if (staged) {
+    for (int64_t i = gl; i < la; i += G) sa[i] = ld_stream(aidx + a0 + i);
+    for (int64_t i = gl; i < lb; i += G) sb[i] = ld_stream(bidx + b0 + i);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return staged;
+}
+
 template <int G, typename TP, typename TI>
 __global__ __launch_bounds__(256) void ewise_count_kernel(
     int op, int64_t m, int64_t base, const TP* __restrict__ aptr, const TI* __restrict__ aidx,
     const TP* __restrict__ bptr, const TI* __restrict__ bidx, int64_t* __restrict__ cnt) {
+  constexpr int CAP = ew_cap<G>();
+  __shared__ TI s_a[256 / G][CAP];
+  __shared__ TI s_b[256 / G][CAP];
   const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
   if (row >= m) return;
+  const int grp = threadIdx.x / G;
   const int gl = lane_id() & (G - 1);
   const uint64_t gm = group_mask<G>();
   const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
   const int64_t b0 = pos_at(bptr, base, row), b1 = pos_at(bptr, base, row + 1);
+  const int64_t la = a1 - a0, lb = b1 - b0;
+  const bool staged = ew_stage<G>(aidx, a0, la, bidx, b0, lb, s_a[grp], s_b[grp], gl);
+  EwSide<TI> A{aidx + a0, staged ? s_a[grp] : nullptr, la};
+  EwSide<TI> B{bidx + b0, staged ? s_b[grp] : nullptr, lb};
+  // The common coordinates: each coordinate of the shorter fibre looks for
+  // its partner in the longer one.
+  const EwSide<TI>& S = la <= lb ? A : B;
+  const EwSide<TI>& L = la <= lb ? B : A;
   int64_t found = 0;
-  if (b1 > b0) {
-    for (int64_t c = a0; c < a1; c += G) {
-      const int64_t p = c + gl;
+  if (L.n > 0) {
+    for (int64_t c = 0; c < S.n; c += G) {
+      const int64_t i = c + gl;
       bool f = false;
-      if (p < a1) {
-        const int64_t key = aidx[p];
-        const int64_t lb = lower_bound(bidx, b0, b1, key);
-        f = lb < b1 && static_cast<int64_t>(bidx[lb]) == key;
+      if (i < S.n) {
+        const int64_t key = S.at(i);
+        const int64_t p = L.lower(key);
+        f = p < L.n && L.at(p) == key;
       }
       found += popc(ballot(f) & gm);
     }
   }
-  if (gl == 0) cnt[row] = (op == FGX_OP_MUL) ? found : (a1 - a0) + (b1 - b0) - found;
+  if (gl == 0) cnt[row] = (op == FGX_OP_MUL) ? found : la + lb - found;
 }
 
 template <int G, typename TP, typename TI, typename TV>
@@ -78,8 +140,12 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
     const TV* __restrict__ aval, const TP* __restrict__ bptr, const TI* __restrict__ bidx,
     const TV* __restrict__ bval, const TP* __restrict__ cptr, TI* __restrict__ cidx,
     TV* __restrict__ cval) {
+  constexpr int CAP = ew_cap<G>();
+  __shared__ TI s_a[256 / G][CAP];
+  __shared__ TI s_b[256 / G][CAP];
   const int64_t row = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
   if (row >= m) return;
+  const int grp = threadIdx.x / G;
   const int lane = lane_id();
   const int gl = lane & (G - 1);
   const uint64_t gm = group_mask<G>();
@@ -87,30 +153,36 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
   const int64_t b0 = pos_at(bptr, base, row), b1 = pos_at(bptr, base, row + 1);
   const int64_t c0 = pos_at(cptr, base, row);
+  const int64_t la = a1 - a0, lb = b1 - b0;
   const bool uni = op != FGX_OP_MUL;
+  if (!uni && (la == 0 || lb == 0)) return;
+  const bool staged = ew_stage<G>(aidx, a0, la, bidx, b0, lb, s_a[grp], s_b[grp], gl);
+  EwSide<TI> A{aidx + a0, staged ? s_a[grp] : nullptr, la};
+  EwSide<TI> B{bidx + b0, staged ? s_b[grp] : nullptr, lb};
   // A side: every stored a (union) or every a with a partner (intersection).
   int64_t dup = 0;
-  for (int64_t c = a0; c < a1; c += G) {
-    const int64_t p = c + gl;
+  for (int64_t c = 0; c < la; c += G) {
+    const int64_t i = c + gl;
     bool f = false;
-    int64_t lb = b0;
+    int64_t rb = 0;
     int64_t key = 0;
-    if (p < a1) {
-      key = aidx[p];
-      lb = lower_bound(bidx, b0, b1, key);
-      f = lb < b1 && static_cast<int64_t>(bidx[lb]) == key;
+    if (i < la) {
+      key = A.at(i);
+      rb = B.lower(key);
+      f = rb < lb && B.at(rb) == key;
     }
     const uint64_t fb = ballot(f) & gm;
     const int64_t before = dup + popc(fb & lt);
-    if (p < a1) {
+    if (i < la) {
       if (uni) {
-        const int64_t pos = c0 + (p - a0) + (lb - b0) - before;
+        const int64_t pos = c0 + i + rb - before;
         cidx[pos] = static_cast<TI>(key);
-        cval[pos] = f ? ew_both<TV>(op, aval[p], bval[lb]) : ew_only_a<TV>(op, aval[p]);
+        cval[pos] = f ? ew_both<TV>(op, aval[a0 + i], bval[b0 + rb])
+                      : ew_only_a<TV>(op, aval[a0 + i]);
       } else if (f) {
         const int64_t pos = c0 + before;
         cidx[pos] = static_cast<TI>(key);
-        cval[pos] = ew_both<TV>(op, aval[p], bval[lb]);
+        cval[pos] = ew_both<TV>(op, aval[a0 + i], bval[b0 + rb]);
       }
     }
     dup += popc(fb);
@@ -118,22 +190,22 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   if (!uni) return;
   // B side: the b without a partner in A.
   dup = 0;
-  for (int64_t c = b0; c < b1; c += G) {
-    const int64_t q = c + gl;
+  for (int64_t c = 0; c < lb; c += G) {
+    const int64_t i = c + gl;
     bool f = false;
-    int64_t la = a0;
+    int64_t ra = 0;
     int64_t key = 0;
-    if (q < b1) {
-      key = bidx[q];
-      la = lower_bound(aidx, a0, a1, key);
-      f = la < a1 && static_cast<int64_t>(aidx[la]) == key;
+    if (i < lb) {
+      key = B.at(i);
+      ra = A.lower(key);
+      f = ra < la && A.at(ra) == key;
     }
     const uint64_t fb = ballot(f) & gm;
     const int64_t before = dup + popc(fb & lt);
-    if (q < b1 && !f) {
-      const int64_t pos = c0 + (q - b0) + (la - a0) - before;
+    if (i < lb && !f) {
+      const int64_t pos = c0 + i + ra - before;
       cidx[pos] = static_cast<TI>(key);
-      cval[pos] = ew_only_b<TV>(op, bval[q]);
+      cval[pos] = ew_only_b<TV>(op, bval[b0 + i]);
     }
     dup += popc(fb);
   }
@@ -141,7 +213,7 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
 
 static int pick_group(int64_t m, int64_t nnz_a, int64_t nnz_b) {
   const double avg = m > 0 ? static_cast<double>(nnz_a + nnz_b) / static_cast<double>(m) : 0.0;
-  if (avg <= 24.0) return 8;
+  if (avg <= 48.0) return 8;
   if (avg <= 96.0) return 16;
   return 64;
 }

@@ -270,13 +270,20 @@ struct SortShape {
   static constexpr int64_t kMaxCols = int64_t(1) << (31 - kPBits);
 };
 
-template <int PER, typename TV>
-struct WaveSortLds {
-  int off[SortShape<PER>::kCap + 1];
-  int lk[SortShape<PER>::kCap];
-  TV la[SortShape<PER>::kCap];
-  uint32_t sk[SortShape<PER>::kCap];
-  TV sv[SortShape<PER>::kCap];
+// The product lists (steps 1-2) and the sorted keys (step 4) are never live
+// together, so they share the wave's LDS: half the footprint, twice the
+// resident waves for the 512-product rows.
+template <int PER, typename TV, typename TP>
+union WaveSortLds {
+  struct {
+    int off[SortShape<PER>::kCap + 1];
+    TP bs[SortShape<PER>::kCap];  // 0-based start of each list's B row
+    TV la[SortShape<PER>::kCap];
+  } in;
+  struct {
+    uint32_t sk[SortShape<PER>::kCap];
+    TV sv[SortShape<PER>::kCap];
+  } out;
 };
 
 template <int PER, int SR, typename TP, typename TI, typename TV>
@@ -284,7 +291,7 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
                          const TI* __restrict__ aidx, const TV* __restrict__ aval,
                          const TP* __restrict__ bptr, const TI* __restrict__ bidx,
                          const TV* __restrict__ bval, const TP* __restrict__ cptr,
-                         TI* __restrict__ cidx, TV* __restrict__ cval, WaveSortLds<PER, TV>& L) {
+                         TI* __restrict__ cidx, TV* __restrict__ cval, WaveSortLds<PER, TV, TP>& L) {
   constexpr int N = SortShape<PER>::kCap;
   constexpr int PB = SortShape<PER>::kPBits;
   const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
@@ -296,17 +303,18 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
     int len = 0;
     if (t < nl) {
       const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
-      len = static_cast<int>(pos_at(bptr, base, k + 1) - pos_at(bptr, base, k));
-      L.lk[t] = k;
-      L.la[t] = aval[a0 + t];
+      const int64_t s0 = pos_at(bptr, base, k);
+      len = static_cast<int>(pos_at(bptr, base, k + 1) - s0);
+      L.in.bs[t] = static_cast<TP>(s0);
+      L.in.la[t] = aval[a0 + t];
     }
     int tot;
     const int ex = team_scan<64>(len, lane, nullptr, &tot);
-    if (t < nl) L.off[t] = run + ex;
+    if (t < nl) L.in.off[t] = run + ex;
     run += tot;
   }
   const int np = run;
-  if (lane == 0) L.off[nl] = np;
+  if (lane == 0) L.in.off[nl] = np;
   team_sync<64>();
   // 2. This lane's products (padding keys sort last).
   uint32_t key[PER];
@@ -317,11 +325,11 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
     key[r] = 0xffffffffu;
     val[r] = TV(0);
     if (p < np) {
-      const int t = list_of(L.off, nl, p);
-      const int64_t q = pos_at(bptr, base, L.lk[t]) + (p - L.off[t]);
+      const int t = list_of(L.in.off, nl, p);
+      const int64_t q = static_cast<int64_t>(L.in.bs[t]) + (p - L.in.off[t]);
       const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bidx[q]) - base);
       key[r] = (col << PB) | static_cast<uint32_t>(p);
-      val[r] = sr_mul<TV, SR>(L.la[t], bval[q]);
+      val[r] = sr_mul<TV, SR>(L.in.la[t], bval[q]);
     }
   }
   // 3. Bitonic sort of the N keys (all distinct), ascending.
@@ -362,11 +370,14 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
       }
     }
   }
-  // 4. Fold each column's run in order and write it at its rank.
+  // 4. Fold each column's run in order and write it at its rank.  The
+  // sorted keys overwrite the product lists: every lane's reads of those
+  // are complete (their values fed the sort) before the wave's stores.
+  team_sync<64>();
 #pragma unroll
   for (int r = 0; r < PER; r++) {
-    L.sk[PER * lane + r] = key[r];
-    L.sv[PER * lane + r] = val[r];
+    L.out.sk[PER * lane + r] = key[r];
+    L.out.sv[PER * lane + r] = val[r];
   }
   team_sync<64>();
   const int64_t o = pos_at(cptr, base, row);
@@ -375,13 +386,13 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
   for (int r = 0; r < PER; r++) {
     // Rounds of 64 consecutive sorted positions.
     const int e = r * kWave + lane;
-    const uint32_t col = L.sk[e] >> PB;
-    const bool head = e < np && (e == 0 || (L.sk[e - 1] >> PB) != col);
+    const uint32_t col = L.out.sk[e] >> PB;
+    const bool head = e < np && (e == 0 || (L.out.sk[e - 1] >> PB) != col);
     int tot;
     const int ex = team_scan<64>(head ? 1 : 0, lane, nullptr, &tot);
     if (head) {
       TV w = sr_zero<TV, SR>();
-      for (int x = e; x < np && (L.sk[x] >> PB) == col; x++) w = sr_add<TV, SR>(w, L.sv[x]);
+      for (int x = e; x < np && (L.out.sk[x] >> PB) == col; x++) w = sr_add<TV, SR>(w, L.out.sv[x]);
       cidx[o + seen + ex] = static_cast<TI>(static_cast<int64_t>(col) + base);
       cval[o + seen + ex] = w;
     }
@@ -396,7 +407,7 @@ __global__ __launch_bounds__(256) void spgemm_wave_sort_kernel(
     const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
     const TI* __restrict__ bidx, const TV* __restrict__ bval, const TP* __restrict__ cptr,
     TI* __restrict__ cidx, TV* __restrict__ cval) {
-  __shared__ WaveSortLds<PER, TV> lds[4];
+  __shared__ WaveSortLds<PER, TV, TP> lds[4];
   const int w = wave_in_block();
   const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (r >= nrows) return;
@@ -416,11 +427,11 @@ __device__ __forceinline__ int hash_slot(int key) {
 }
 
 // Per-team LDS of a counted row: the table and the row's lists.
-template <int CAP, int SLOTS>
+template <int CAP, int SLOTS, typename TP>
 struct HashLds {
   int key[SLOTS];
   int off[CAP + 1];
-  int lk[CAP];
+  TP bs[CAP];  // 0-based start of each list's B row
   int tmp[8];
 };
 
@@ -432,7 +443,7 @@ template <int NT, int CAP, int SLOTS, typename TP, typename TI>
 __device__ void hash_count_row(int64_t row, int tt, int64_t base, const TP* __restrict__ aptr,
                                const TI* __restrict__ aidx, const TP* __restrict__ bptr,
                                const TI* __restrict__ bidx, int64_t* __restrict__ cnt,
-                               HashLds<CAP, SLOTS>& L) {
+                               HashLds<CAP, SLOTS, TP>& L) {
   for (int i = tt; i < SLOTS; i += NT) L.key[i] = kEmptyKey;
   const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
   const int nl = static_cast<int>(a1 - a0);
@@ -442,8 +453,9 @@ __device__ void hash_count_row(int64_t row, int tt, int64_t base, const TP* __re
     int len = 0;
     if (t < nl) {
       const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
-      len = static_cast<int>(pos_at(bptr, base, k + 1) - pos_at(bptr, base, k));
-      L.lk[t] = k;
+      const int64_t s0 = pos_at(bptr, base, k);
+      len = static_cast<int>(pos_at(bptr, base, k + 1) - s0);
+      L.bs[t] = static_cast<TP>(s0);
     }
     int tot;
     const int ex = team_scan<NT>(len, tt, L.tmp, &tot);
@@ -456,7 +468,7 @@ __device__ void hash_count_row(int64_t row, int tt, int64_t base, const TP* __re
   int fresh = 0;
   for (int p = tt; p < np; p += NT) {
     const int t = list_of(L.off, nl, p);
-    const int64_t q = pos_at(bptr, base, L.lk[t]) + (p - L.off[t]);
+    const int64_t q = static_cast<int64_t>(L.bs[t]) + (p - L.off[t]);
     const int key = static_cast<int>(static_cast<int64_t>(bidx[q]) - base);
     int slot = hash_slot<SLOTS>(key);
     for (;;) {
@@ -473,18 +485,21 @@ __device__ void hash_count_row(int64_t row, int tt, int64_t base, const TP* __re
   if (tt == 0) cnt[row] = distinct;
 }
 
-// Wave hash teams: 4 rows per 256-thread block.
-template <typename TP, typename TI>
+// Wave hash teams: 4 rows per 256-thread block; CAP = 256 products with 512
+// slots, or 512 products with 1024 slots (one wave per row instead of a
+// 256-thread block for the rows of 257..512 products).
+template <int CAP, int SLOTS, typename TP, typename TI>
 __global__ __launch_bounds__(256) void spgemm_wave_count_kernel(
     const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
     const TI* __restrict__ aidx, const TP* __restrict__ bptr, const TI* __restrict__ bidx,
     int64_t* __restrict__ cnt) {
-  __shared__ HashLds<kWaveCap, kWaveSlots> lds[4];
+  static_assert(SLOTS >= 2 * CAP, "hash table at most half full");
+  __shared__ HashLds<CAP, SLOTS, TP> lds[4];
   const int w = wave_in_block();
   const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
   if (r >= nrows) return;
-  hash_count_row<64, kWaveCap, kWaveSlots>(rows[r], threadIdx.x & 63, base, aptr, aidx, bptr, bidx,
-                                           cnt, lds[w]);
+  hash_count_row<64, CAP, SLOTS>(rows[r], threadIdx.x & 63, base, aptr, aidx, bptr, bidx, cnt,
+                                 lds[w]);
 }
 
 template <typename TP, typename TI>
@@ -492,7 +507,7 @@ __global__ __launch_bounds__(256) void spgemm_block_count_kernel(
     const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
     const TI* __restrict__ aidx, const TP* __restrict__ bptr, const TI* __restrict__ bidx,
     int64_t* __restrict__ cnt) {
-  __shared__ HashLds<kBlockCap, kBlockSlots> L;
+  __shared__ HashLds<kBlockCap, kBlockSlots, TP> L;
   const int64_t r = blockIdx.x;
   if (r >= nrows) return;
   hash_count_row<256, kBlockCap, kBlockSlots>(rows[r], threadIdx.x, base, aptr, aidx, bptr, bidx,
@@ -694,16 +709,21 @@ static int run_bins(const fgx_compressed* A, const fgx_compressed* B, const Bins
   if constexpr (!NUMERIC) {
     // Symbolic: distinct-column counts from the LDS hash teams.
     if (bins.count[0] > 0) {
-      hipLaunchKernelGGL((spgemm_wave_count_kernel<TP, TI>),
+      hipLaunchKernelGGL((spgemm_wave_count_kernel<kWaveCap, kWaveSlots, TP, TI>),
                          dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
                          bins.rows, bins.count[0], base, aptr, aidx, bptr, bidx, cnt);
       FGX_TRY(launch_check("spgemm_wave_count_kernel"));
     }
-    for (int b = 1; b <= 2; b++) {
-      if (bins.count[b] == 0) continue;
+    if (bins.count[1] > 0) {
+      hipLaunchKernelGGL((spgemm_wave_count_kernel<kWide, 2 * kWide, TP, TI>),
+                         dim3(static_cast<unsigned>((bins.count[1] + 3) / 4)), dim3(256), 0, s,
+                         bins.rows + m, bins.count[1], base, aptr, aidx, bptr, bidx, cnt);
+      FGX_TRY(launch_check("spgemm_wave_count_kernel"));
+    }
+    if (bins.count[2] > 0) {
       hipLaunchKernelGGL((spgemm_block_count_kernel<TP, TI>),
-                         dim3(static_cast<unsigned>(bins.count[b])), dim3(256), 0, s,
-                         bins.rows + b * m, bins.count[b], base, aptr, aidx, bptr, bidx, cnt);
+                         dim3(static_cast<unsigned>(bins.count[2])), dim3(256), 0, s,
+                         bins.rows + 2 * m, bins.count[2], base, aptr, aidx, bptr, bidx, cnt);
       FGX_TRY(launch_check("spgemm_block_count_kernel"));
     }
   } else {

@@ -88,6 +88,44 @@ def test_ewise_edges(dev):
             assert np.array_equal(cval, rval)
 
 
+def _rows_csr(lengths, n, rng, dtype):
+    """1-based CSR with the given row lengths, distinct sorted columns."""
+    ptr = np.concatenate([[1], 1 + np.cumsum(lengths)]).astype(np.int64)
+    idx = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) + 1 if l else
+                          np.zeros(0, np.int64) for l in lengths]).astype(np.int64)
+    val = rng.standard_normal(idx.size).astype(dtype)
+    return ptr, idx, val
+
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_ewise_long_fibres_beside_short(dev, itype):
+    # Short rows pick 8-lane groups whose LDS slots hold 64 coordinates per
+    # side; a few rows of hundreds of entries exceed them and are searched in
+    # global memory.  Both kinds must give the oracle's output bit for bit.
+    rng = np.random.default_rng(11)
+    m, n = 400, 6000
+    la = rng.poisson(3, m)
+    lb = rng.poisson(5, m)
+    la[[7, 100, 399]] = [500, 64, 65]
+    lb[[7, 100, 250]] = [900, 65, 64]
+    (ap, ai, av) = _rows_csr(la, n, rng, np.float32)
+    (bp, bi, bv) = _rows_csr(lb, n, rng, np.float32)
+    # Row 7 of B shares 100 coordinates with row 7 of A, so the long rows'
+    # intersection is not empty.
+    a7 = ai[ap[7] - 1: ap[8] - 1]
+    others = np.setdiff1d(np.arange(1, n + 1), a7)
+    bi[bp[7] - 1: bp[8] - 1] = np.sort(np.concatenate(
+        [a7[:100], rng.choice(others, size=lb[7] - 100, replace=False)]))
+    A = operand(ap, ai, av, m, n, dev, itype)
+    B = operand(bp, bi, bv, m, n, dev, itype)
+    for op in ("mul", "add", "max"):
+        cptr, cidx, cval = host(ops.ewise(op, A, B))
+        rptr, ridx, rval = oracle.ewise(op, ap, ai, av, bp, bi, bv)
+        assert np.array_equal(cptr, rptr) and np.array_equal(cidx, ridx)
+        assert np.array_equal(cval, rval)
+    assert oracle.ewise("mul", ap, ai, av, bp, bi, bv)[1].size >= 100
+
+
 def test_ewise_200k_poisson(dev):
     # 200k x 200k, Poisson(8) and Poisson(24) rows, FP32 values.
     m = n = 200_000
