@@ -274,16 +274,15 @@ struct SortShape {
 // together, so they share the wave's LDS: half the footprint, twice the
 // resident waves for the 512-product rows.
 template <int PER, typename TV, typename TP>
-union WaveSortLds {
-  struct {
-    int off[SortShape<PER>::kCap + 1];
-    TP bs[SortShape<PER>::kCap];  // 0-based start of each list's B row
-    TV la[SortShape<PER>::kCap];
-  } in;
-  struct {
-    uint32_t sk[SortShape<PER>::kCap];
-    TV sv[SortShape<PER>::kCap];
-  } out;
+struct WaveSortLds {
+  union {
+    struct {
+      int off[SortShape<PER>::kCap + 1];
+      TP bs[SortShape<PER>::kCap];  // 0-based start of each list's B row
+    } in;
+    uint32_t sk[SortShape<PER>::kCap];  // sorted keys
+  } u;
+  TV pv[SortShape<PER>::kCap];  // product values by product index
 };
 
 template <int PER, int SR, typename TP, typename TI, typename TV>
@@ -305,31 +304,30 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
       const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
       const int64_t s0 = pos_at(bptr, base, k);
       len = static_cast<int>(pos_at(bptr, base, k + 1) - s0);
-      L.in.bs[t] = static_cast<TP>(s0);
-      L.in.la[t] = aval[a0 + t];
+      L.u.in.bs[t] = static_cast<TP>(s0);
     }
     int tot;
     const int ex = team_scan<64>(len, lane, nullptr, &tot);
-    if (t < nl) L.in.off[t] = run + ex;
+    if (t < nl) L.u.in.off[t] = run + ex;
     run += tot;
   }
   const int np = run;
-  if (lane == 0) L.in.off[nl] = np;
+  if (lane == 0) L.u.in.off[nl] = np;
   team_sync<64>();
-  // 2. This lane's products (padding keys sort last).
+  // 2. This lane's products (padding keys sort last).  Values stay in LDS
+  // under their product index, which the key carries: the sort moves keys
+  // only.
   uint32_t key[PER];
-  TV val[PER];
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     const int p = PER * lane + r;
     key[r] = 0xffffffffu;
-    val[r] = TV(0);
     if (p < np) {
-      const int t = list_of(L.in.off, nl, p);
-      const int64_t q = static_cast<int64_t>(L.in.bs[t]) + (p - L.in.off[t]);
+      const int t = list_of(L.u.in.off, nl, p);
+      const int64_t q = static_cast<int64_t>(L.u.in.bs[t]) + (p - L.u.in.off[t]);
       const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bidx[q]) - base);
       key[r] = (col << PB) | static_cast<uint32_t>(p);
-      val[r] = sr_mul<TV, SR>(L.in.la[t], bval[q]);
+      L.pv[p] = sr_mul<TV, SR>(aval[a0 + t], bval[q]);
     }
   }
   // 3. Bitonic sort of the N keys (all distinct), ascending.
@@ -343,13 +341,9 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
         for (int r = 0; r < PER; r++) {
           const int e = PER * lane + r;
           const uint32_t pk = __shfl_xor(key[r], d, 64);
-          const TV pv = __shfl_xor(val[r], d, 64);
           const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
           const bool take = keep_min ? (pk < key[r]) : (pk > key[r]);
-          if (take) {
-            key[r] = pk;
-            val[r] = pv;
-          }
+          key[r] = take ? pk : key[r];
         }
       } else {
 #pragma unroll
@@ -362,9 +356,6 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
             const uint32_t tk = key[r];
             key[r] = key[r2];
             key[r2] = tk;
-            const TV tv = val[r];
-            val[r] = val[r2];
-            val[r2] = tv;
           }
         }
       }
@@ -375,24 +366,23 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
   // are complete (their values fed the sort) before the wave's stores.
   team_sync<64>();
 #pragma unroll
-  for (int r = 0; r < PER; r++) {
-    L.out.sk[PER * lane + r] = key[r];
-    L.out.sv[PER * lane + r] = val[r];
-  }
+  for (int r = 0; r < PER; r++) L.u.sk[PER * lane + r] = key[r];
   team_sync<64>();
+  constexpr uint32_t kPMask = (1u << PB) - 1;
   const int64_t o = pos_at(cptr, base, row);
   int seen = 0;
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     // Rounds of 64 consecutive sorted positions.
     const int e = r * kWave + lane;
-    const uint32_t col = L.out.sk[e] >> PB;
-    const bool head = e < np && (e == 0 || (L.out.sk[e - 1] >> PB) != col);
+    const uint32_t col = L.u.sk[e] >> PB;
+    const bool head = e < np && (e == 0 || (L.u.sk[e - 1] >> PB) != col);
     int tot;
     const int ex = team_scan<64>(head ? 1 : 0, lane, nullptr, &tot);
     if (head) {
       TV w = sr_zero<TV, SR>();
-      for (int x = e; x < np && (L.out.sk[x] >> PB) == col; x++) w = sr_add<TV, SR>(w, L.out.sv[x]);
+      for (int x = e; x < np && (L.u.sk[x] >> PB) == col; x++)
+        w = sr_add<TV, SR>(w, L.pv[L.u.sk[x] & kPMask]);
       cidx[o + seen + ex] = static_cast<TI>(static_cast<int64_t>(col) + base);
       cval[o + seen + ex] = w;
     }
