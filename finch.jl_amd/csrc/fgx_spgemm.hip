@@ -342,8 +342,9 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
           const int e = PER * lane + r;
           const uint32_t pk = __shfl_xor(key[r], d, 64);
           const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
-          const bool take = keep_min ? (pk < key[r]) : (pk > key[r]);
-          key[r] = take ? pk : key[r];
+          const uint32_t lo = pk < key[r] ? pk : key[r];
+          const uint32_t hi = pk < key[r] ? key[r] : pk;
+          key[r] = keep_min ? lo : hi;
         }
       } else {
 #pragma unroll
@@ -352,11 +353,11 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
           const int r2 = r | j;
           const int e = PER * lane + r;
           const bool up = (e & k) == 0;
-          if ((key[r] > key[r2]) == up) {
-            const uint32_t tk = key[r];
-            key[r] = key[r2];
-            key[r2] = tk;
-          }
+          // Branch-free compare-exchange (ascending where up).
+          const uint32_t lo = key[r] < key[r2] ? key[r] : key[r2];
+          const uint32_t hi = key[r] < key[r2] ? key[r2] : key[r];
+          key[r] = up ? lo : hi;
+          key[r2] = up ? hi : lo;
         }
       }
     }
