@@ -285,6 +285,22 @@ struct WaveSortLds {
   TV pv[SortShape<PER>::kCap];  // product values by product index
 };
 
+// Value of lane (lane ^ d) for the bitonic network.  Distances 1 and 2 are
+// quad permutations and 8 is a rotation by 8 within each 16-lane row (the
+// same as xor 8 there), all DPP register moves; the others go through the
+// LDS crossbar.  d is a constant once the network is unrolled.
+__device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int d) {
+  switch (d) {
+    case 1: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false));
+    case 2: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false));
+    case 8: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x128, 0xF, 0xF, false));
+    default: return __shfl_xor(v, d, 64);
+  }
+}
+
 template <int PER, int SR, typename TP, typename TI, typename TV>
 __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restrict__ aptr,
                          const TI* __restrict__ aidx, const TV* __restrict__ aval,
@@ -340,7 +356,7 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
 #pragma unroll
         for (int r = 0; r < PER; r++) {
           const int e = PER * lane + r;
-          const uint32_t pk = __shfl_xor(key[r], d, 64);
+          const uint32_t pk = xor_lanes(key[r], d);
           const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
           const uint32_t lo = pk < key[r] ? pk : key[r];
           const uint32_t hi = pk < key[r] ? key[r] : pk;
