@@ -287,10 +287,18 @@ struct WaveSortLds {
 
 // Value of lane (lane ^ d) for the bitonic network.  Distances 1 and 2 are
 // quad permutations and 8 is a rotation by 8 within each 16-lane row (the
-// same as xor 8 there), all DPP register moves; the others go through the
-// LDS crossbar.  d is a constant once the network is unrolled.
+// same as xor 8 there), all DPP register moves.  Distance 4 takes two row
+// rotations (row_ror:n hands lane i the value of lane (i - n) mod 16): lanes
+// with bit 2 clear want lane i + 4 (rotation 12), the others lane i - 4
+// (rotation 4).  16 and 32 go through the LDS crossbar.  d is a constant
+// once the network is unrolled.
 __device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int d) {
   switch (d) {
+    case 4: {
+      const int up = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x12C, 0xF, 0xF, false);
+      const int dn = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x124, 0xF, 0xF, false);
+      return static_cast<uint32_t>((lane_id() & 4) ? dn : up);
+    }
     case 1: return static_cast<uint32_t>(
         __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false));
     case 2: return static_cast<uint32_t>(
