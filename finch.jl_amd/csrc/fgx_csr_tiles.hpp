@@ -480,6 +480,9 @@ __device__ __forceinline__ void fixup_tile(const int64_t* __restrict__ carry_row
     for (int col = lane; col < width; col += kWave) {
       if (kofs + col >= k) continue;
       TA s = 0;
+      // Unrolled so that eight loads are in flight; the adds stay in tile
+      // order.
+#pragma unroll 8
       for (int64_t i = 0; i < len; i++) s += carry_val[(t + i) * width + col];
       out[r * ld + kofs + col] += s;
     }

@@ -156,6 +156,19 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   const int64_t la = a1 - a0, lb = b1 - b0;
   const bool uni = op != FGX_OP_MUL;
   if (!uni && (la == 0 || lb == 0)) return;
+  // Union: the values of the group's first NPRE steps on each side are
+  // loaded together with the coordinates (one round trip); each step then
+  // loads the value NPRE steps ahead, so the stores never wait on a fresh
+  // load.  Intersection reads only the values of matched coordinates.
+  constexpr int NPRE = 4;
+  TV pa[NPRE], pb[NPRE];
+#pragma unroll
+  for (int j = 0; j < NPRE; j++) {
+    const int64_t ia = static_cast<int64_t>(j) * G + gl;
+    pa[j] = (uni && ia < la) ? aval[a0 + ia] : TV(0);
+    const int64_t ib = static_cast<int64_t>(j) * G + gl;
+    pb[j] = (uni && ib < lb) ? bval[b0 + ib] : TV(0);
+  }
   const bool staged = ew_stage<G>(aidx, a0, la, bidx, b0, lb, s_a[grp], s_b[grp], gl);
   EwSide<TI> A{aidx + a0, staged ? s_a[grp] : nullptr, la};
   EwSide<TI> B{bidx + b0, staged ? s_b[grp] : nullptr, lb};
@@ -163,6 +176,13 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   int64_t dup = 0;
   for (int64_t c = 0; c < la; c += G) {
     const int64_t i = c + gl;
+    const TV av = pa[0];
+#pragma unroll
+    for (int j = 0; j + 1 < NPRE; j++) pa[j] = pa[j + 1];
+    {
+      const int64_t ia = i + static_cast<int64_t>(NPRE) * G;
+      pa[NPRE - 1] = (uni && ia < la) ? aval[a0 + ia] : TV(0);
+    }
     bool f = false;
     int64_t rb = 0;
     int64_t key = 0;
@@ -177,8 +197,7 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
       if (uni) {
         const int64_t pos = c0 + i + rb - before;
         cidx[pos] = static_cast<TI>(key);
-        cval[pos] = f ? ew_both<TV>(op, aval[a0 + i], bval[b0 + rb])
-                      : ew_only_a<TV>(op, aval[a0 + i]);
+        cval[pos] = f ? ew_both<TV>(op, av, bval[b0 + rb]) : ew_only_a<TV>(op, av);
       } else if (f) {
         const int64_t pos = c0 + before;
         cidx[pos] = static_cast<TI>(key);
@@ -192,6 +211,13 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   dup = 0;
   for (int64_t c = 0; c < lb; c += G) {
     const int64_t i = c + gl;
+    const TV bv = pb[0];
+#pragma unroll
+    for (int j = 0; j + 1 < NPRE; j++) pb[j] = pb[j + 1];
+    {
+      const int64_t ib = i + static_cast<int64_t>(NPRE) * G;
+      pb[NPRE - 1] = ib < lb ? bval[b0 + ib] : TV(0);
+    }
     bool f = false;
     int64_t ra = 0;
     int64_t key = 0;
@@ -205,7 +231,7 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
     if (i < lb && !f) {
       const int64_t pos = c0 + i + ra - before;
       cidx[pos] = static_cast<TI>(key);
-      cval[pos] = ew_only_b<TV>(op, bval[b0 + i]);
+      cval[pos] = ew_only_b<TV>(op, bv);
     }
     dup += popc(fb);
   }
