@@ -215,7 +215,7 @@ def main():
                      "traffic": measured_traffic(),
                      "traffic_source": "profiles/spmm_csr_powerlaw_1m_k64.md (rocprofv3 "
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
-                     "kernel": "spmm_csr_kernel + fixup_group_kernel (one fgx_spmm_csr call)",
+                     "kernel": "spmm_csr_kernel + fixup_chunk_kernel (one fgx_spmm_csr call)",
                      "algorithmic_bytes": by_spmm, "launch_ms": round(spmm_ev_ms, 5)},
         "spmv": {"gbs": round(gbs(by_spmv, spmv_ms), 2), "ms": round(spmv_ms, 5),
                  "event_ms": round(spmv_ev_ms, 5), "algorithmic_bytes": by_spmv,

@@ -9,18 +9,33 @@
 
 namespace fgx {
 
-// Carries of TILES units, finished in tile order.  Persistent form
-// (seg_off == nullptr): one wave per unit, over the unit's tiles.  Segmented
-// form: entries are whole requests' tile ranges, one wave per tile.
-__device__ __forceinline__ void fixup_item_tile(const BatchDesc& d, int64_t t,
-                                                const int64_t* __restrict__ carry_row,
-                                                const float* __restrict__ carry_val) {
-  const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
+// Carries of TILES units, finished by fixup_chunk (a chunk of 64 tiles per
+// wave for SpMV items, 4 for SpMM items).  Persistent form (seg_off ==
+// nullptr): one wave per unit, over the unit's chunks.  Segmented form:
+// entries are whole requests' tile ranges, one wave per chunk.
+__device__ __forceinline__ int item_fix_width(const BatchDesc& d) {
+  return d.k == 1 ? 1 : group_for_k(d.k) * 4;
+}
+
+__device__ __forceinline__ void fixup_item_chunk(const BatchDesc& d, int64_t t0, int64_t t1,
+                                                 const int64_t* __restrict__ carry_row,
+                                                 const float* __restrict__ carry_val) {
+  const int width = item_fix_width(d);
   const int64_t W = tile_width_for_k(d.k);
   const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
   const int64_t ld = d.k == 1 ? 1 : d.ldc;
-  fixup_tile<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t, width, d.C,
-                    ld, 0, d.k);
+  fixup_chunk<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t0, t1, width,
+                     d.C, ld, 0, d.k);
+}
+
+// Chunk `c` of a segmented fix-up entry covering tiles [begin, end).
+__device__ __forceinline__ void fixup_entry_chunk(const BatchDesc& d, int64_t begin, int64_t end,
+                                                  int64_t c, const int64_t* __restrict__ carry_row,
+                                                  const float* __restrict__ carry_val) {
+  const int64_t ct = fixup_chunk_tiles(item_fix_width(d));
+  const int64_t t0 = begin + c * ct;
+  const int64_t t1 = t0 + ct < end ? t0 + ct : end;
+  fixup_item_chunk(d, t0, t1, carry_row, carry_val);
 }
 
 __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
@@ -29,24 +44,30 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
     const float* __restrict__ carry_val, SegInline inl) {
   const int w = wave_in_block();
   if (seg_off) {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
-    if (t >= total) return;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
+    if (c >= total) return;  // wave-uniform
     if (inl.n > 0) {  // few entries: offsets and descriptors are arguments
       int64_t u = 0;
-      for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= t ? 1 : 0;
-      fixup_item_tile(inl.desc[u], t - inl.seg[u], carry_row, carry_val);
+      for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= c ? 1 : 0;
+      // Inline fix-up entries cover whole requests: tiles [0, ntiles).
+      const BatchDesc& d = inl.desc[u];
+      const int64_t W = tile_width_for_k(d.k);
+      const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
+      fixup_entry_chunk(d, 0, nt, c - inl.seg[u], carry_row, carry_val);
       return;
     }
-    const int64_t u = find_segment(seg_off, n_fix, t);
+    const int64_t u = find_segment(seg_off, n_fix, c);
     const BatchWork wk = fix[u];
-    fixup_item_tile(desc[wk.item], wk.begin + (t - seg_off[u]), carry_row, carry_val);
+    fixup_entry_chunk(desc[wk.item], wk.begin, wk.end, c - seg_off[u], carry_row, carry_val);
     return;
   }
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w; u < n_fix; u += nw) {
     const BatchWork wk = fix[u];
     const BatchDesc d = desc[wk.item];
-    for (int64_t t = wk.begin; t < wk.end; t++) fixup_item_tile(d, t, carry_row, carry_val);
+    const int64_t ct = fixup_chunk_tiles(item_fix_width(d));
+    for (int64_t t = wk.begin; t < wk.end; t += ct)
+      fixup_item_chunk(d, t, t + ct < wk.end ? t + ct : wk.end, carry_row, carry_val);
   }
 }
 
@@ -279,7 +300,10 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
   int64_t facc = 0;
   for (int64_t u = 0; u < nf; u++) {
     seg.push_back(facc);
-    facc += h_fix[u].end - h_fix[u].begin;
+    // One wave per chunk of the entry's tiles.
+    const BatchDesc& fd = h_desc[h_fix[u].item];
+    const int64_t ct = fixup_chunk_tiles(fd.k == 1 ? 1 : group_for_k(fd.k) * 4);
+    facc += (h_fix[u].end - h_fix[u].begin + ct - 1) / ct;
   }
   seg.push_back(facc);
   p->fix_total = facc;

@@ -489,6 +489,96 @@ __device__ __forceinline__ void fixup_tile(const int64_t* __restrict__ carry_row
   }
 }
 
+// Carry fix-up over a chunk of tiles [t0, t1) of one operand, by the whole
+// wave (every lane reaches each ballot).
+//   Width 1 (SpMV): one lane per tile, up to 64 tiles.  A run of one tile,
+//   the common case (a row split over two tiles), is completed by its lane
+//   with one add.  Longer runs (a row spanning several tiles; about 8 per 64
+//   tiles on a Zipf(1.5) matrix) are taken eight at a time by 8-lane
+//   subgroups: each finds its run's length 8 tiles per step and sums the
+//   carries lane-strided, then by a fixed xor tree.
+//   Width > 1 (SpMM): 16 lanes per tile, up to 4 tiles; a one-tile run is
+//   added by its 16 lanes (columns strided by 16), longer runs are taken by
+//   the whole wave through fixup_tile.
+// Each run is summed in a fixed order, so results do not depend on timing.
+constexpr int kFixupGroup = 16;
+__host__ __device__ inline int fixup_chunk_tiles(int width) {
+  return width == 1 ? kWave : kWave / kFixupGroup;
+}
+
+template <typename TA>
+__device__ __forceinline__ void fixup_chunk(const int64_t* __restrict__ carry_row,
+                                            const TA* __restrict__ carry_val, int64_t ntiles,
+                                            int64_t t0, int64_t t1, int width,
+                                            TA* __restrict__ out, int64_t ld, int64_t kofs,
+                                            int64_t k) {
+  const int lane = lane_id();
+  if (t1 > ntiles) t1 = ntiles;
+  if (width == 1) {
+    const int64_t t = t0 + lane;
+    bool longrun = false;
+    if (t < t1) {
+      const int64_t r = carry_row[t];
+      if (r >= 0) {
+        if (t + 1 < ntiles && carry_row[t + 1] == -(r + 2)) longrun = true;
+        else if (kofs < k) out[r * ld + kofs] += carry_val[t];
+      }
+    }
+    uint64_t runs = ballot(longrun);
+    const int sub = lane >> 3, sl = lane & 7;
+    while (runs) {
+      // Subgroup `sub` takes the sub-th remaining run, if there is one.
+      uint64_t m = runs;
+      for (int i = 0; i < sub && m; i++) m &= m - 1;
+      const bool have = m != 0;
+      const int l = have ? __ffsll(static_cast<unsigned long long>(m)) - 1 : 0;
+      for (int i = 0; i < 8 && runs; i++) runs &= runs - 1;
+      if (have) {
+        const int64_t ts = t0 + l;
+        const int64_t r = carry_row[ts];
+        const int64_t cont = -(r + 2);
+        int64_t len = 1;
+        for (int64_t c = ts + 1;; c += 8) {
+          const int64_t t2 = c + sl;
+          const bool in = t2 < ntiles && carry_row[t2] == cont;
+          const uint32_t msk = static_cast<uint32_t>(ballot(in) >> (sub * 8)) & 0xffu;
+          const int run = msk == 0xffu ? 8 : __builtin_ctz(~msk);
+          len += run;
+          if (run < 8) break;
+        }
+        TA s = 0;
+        for (int64_t i = sl; i < len; i += 8) s += carry_val[ts + i];
+        s += __shfl_xor(s, 4, kWave);
+        s += __shfl_xor(s, 2, kWave);
+        s += __shfl_xor(s, 1, kWave);
+        if (sl == 0 && kofs < k) out[r * ld + kofs] += s;
+      }
+    }
+  } else {
+    constexpr int GL = kFixupGroup;
+    const int gl = lane % GL;
+    const int64_t t = t0 + lane / GL;
+    bool longrun = false;
+    if (t < t1) {
+      const int64_t r = carry_row[t];
+      if (r >= 0) {
+        if (t + 1 < ntiles && carry_row[t + 1] == -(r + 2)) {
+          longrun = true;
+        } else {
+          for (int col = gl; col < width; col += GL)
+            if (kofs + col < k) out[r * ld + kofs + col] += carry_val[t * width + col];
+        }
+      }
+    }
+    uint64_t runs = ballot(longrun && gl == 0);
+    while (runs) {
+      const int l = __ffsll(static_cast<unsigned long long>(runs)) - 1;
+      runs &= runs - 1;
+      fixup_tile<TA>(carry_row, carry_val, ntiles, t0 + l / GL, width, out, ld, kofs, k);
+    }
+  }
+}
+
 // Builds the tile table: table[t] = smallest r with pos(r+1) > t*W (m if
 // none), for t in [0, ntiles]; table[ntiles] uses position nnz.
 template <typename TP>

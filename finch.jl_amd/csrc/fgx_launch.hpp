@@ -6,109 +6,28 @@
 
 namespace fgx {
 
-// Width 1 (SpMV): one lane per tile.  A run of one tile, the common case (a
-// row split over two tiles), is completed by its lane with one add.  Longer
-// runs (a row spanning several tiles; about 8 per 64 tiles on a Zipf(1.5)
-// matrix) are taken eight at a time by 8-lane subgroups: each finds its run's
-// length 8 tiles per step and sums the carries lane-strided, then by a fixed
-// xor tree, so the order is fixed.  Every lane reaches each ballot.
+// One chunk of fixup_chunk_tiles(width) tiles per wave.
 template <typename TA>
-__global__ __launch_bounds__(256) void fixup_lane_kernel(const int64_t* __restrict__ carry_row,
-                                                         const TA* __restrict__ carry_val,
-                                                         int64_t ntiles, TA* __restrict__ out,
-                                                         int64_t ld, int64_t kofs, int64_t k) {
-  const int lane = lane_id();
-  const int64_t wave0 = static_cast<int64_t>(blockIdx.x) * 256 + (threadIdx.x & ~63);
-  const int64_t t = wave0 + lane;
-  bool longrun = false;
-  if (t < ntiles) {
-    const int64_t r = carry_row[t];
-    if (r >= 0) {
-      if (t + 1 < ntiles && carry_row[t + 1] == -(r + 2)) longrun = true;
-      else if (kofs < k) out[r * ld + kofs] += carry_val[t];
-    }
-  }
-  uint64_t runs = ballot(longrun);
-  const int sub = lane >> 3, sl = lane & 7;
-  while (runs) {
-    // Subgroup `sub` takes the sub-th remaining run, if there is one.
-    uint64_t m = runs;
-    for (int i = 0; i < sub && m; i++) m &= m - 1;
-    const bool have = m != 0;
-    const int l = have ? __ffsll(static_cast<unsigned long long>(m)) - 1 : 0;
-    for (int i = 0; i < 8 && runs; i++) runs &= runs - 1;
-    if (have) {
-      const int64_t t0 = wave0 + l;
-      const int64_t r = carry_row[t0];
-      const int64_t cont = -(r + 2);
-      int64_t len = 1;
-      for (int64_t c = t0 + 1;; c += 8) {
-        const int64_t t2 = c + sl;
-        const bool in = t2 < ntiles && carry_row[t2] == cont;
-        const uint32_t msk = static_cast<uint32_t>(ballot(in) >> (sub * 8)) & 0xffu;
-        const int run = msk == 0xffu ? 8 : __builtin_ctz(~msk);
-        len += run;
-        if (run < 8) break;
-      }
-      TA s = 0;
-      for (int64_t i = sl; i < len; i += 8) s += carry_val[t0 + i];
-      s += __shfl_xor(s, 4, kWave);
-      s += __shfl_xor(s, 2, kWave);
-      s += __shfl_xor(s, 1, kWave);
-      if (sl == 0 && kofs < k) out[r * ld + kofs] += s;
-    }
-  }
-}
-
-// Width > 1 (SpMM): 16 lanes per tile, 4 tiles per wave.  A one-tile run is
-// added by its 16 lanes (columns strided by 16); longer runs are taken by the
-// whole wave through fixup_tile.
-constexpr int kFixupGroup = 16;
-template <typename TA>
-__global__ __launch_bounds__(256) void fixup_group_kernel(const int64_t* __restrict__ carry_row,
+__global__ __launch_bounds__(256) void fixup_chunk_kernel(const int64_t* __restrict__ carry_row,
                                                           const TA* __restrict__ carry_val,
                                                           int64_t ntiles, int width,
                                                           TA* __restrict__ out, int64_t ld,
                                                           int64_t kofs, int64_t k) {
-  constexpr int GL = kFixupGroup;
-  const int lane = lane_id();
-  const int gl = lane % GL;
-  const int64_t wave_t0 =
-      (static_cast<int64_t>(blockIdx.x) * 256 + (threadIdx.x & ~63)) / GL;
-  const int64_t t = wave_t0 + lane / GL;
-  bool longrun = false;
-  if (t < ntiles) {
-    const int64_t r = carry_row[t];
-    if (r >= 0) {
-      if (t + 1 < ntiles && carry_row[t + 1] == -(r + 2)) {
-        longrun = true;
-      } else {
-        for (int col = gl; col < width; col += GL)
-          if (kofs + col < k) out[r * ld + kofs + col] += carry_val[t * width + col];
-      }
-    }
-  }
-  uint64_t runs = ballot(longrun && gl == 0);
-  while (runs) {
-    const int l = __ffsll(static_cast<unsigned long long>(runs)) - 1;
-    runs &= runs - 1;
-    fixup_tile<TA>(carry_row, carry_val, ntiles, wave_t0 + l / GL, width, out, ld, kofs, k);
-  }
+  const int64_t chunk = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave_in_block();
+  const int64_t t0 = chunk * fixup_chunk_tiles(width);
+  if (t0 >= ntiles) return;  // wave-uniform
+  fixup_chunk<TA>(carry_row, carry_val, ntiles, t0, t0 + fixup_chunk_tiles(width), width, out,
+                  ld, kofs, k);
 }
 
 template <typename TA>
 static inline int launch_fixup(hipStream_t s, const int64_t* crow, const TA* cval, int64_t ntiles,
                                int width, TA* out, int64_t ld, int64_t kofs, int64_t k) {
-  if (width == 1) {
-    const int64_t blocks = (ntiles + 255) / 256;
-    hipLaunchKernelGGL((fixup_lane_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
-                       s, crow, cval, ntiles, out, ld, kofs, k);
-    return launch_check("fixup_lane_kernel");
-  }
-  const int64_t blocks = (ntiles * kFixupGroup + 255) / 256;
-  hipLaunchKernelGGL((fixup_group_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+  const int64_t chunks = (ntiles + fixup_chunk_tiles(width) - 1) / fixup_chunk_tiles(width);
+  const int64_t blocks = (chunks + kWavesPerBlock - 1) / kWavesPerBlock;
+  hipLaunchKernelGGL((fixup_chunk_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                      s, crow, cval, ntiles, width, out, ld, kofs, k);
-  return launch_check("fixup_group_kernel");
+  return launch_check("fixup_chunk_kernel");
 }
 
 // dst[r * ldd + c] = src[r + c * lds] for r < rows, c < cols: a column-major
