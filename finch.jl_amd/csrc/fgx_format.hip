@@ -226,42 +226,46 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   }
 }
 
-// fib[q] = the fibre that owns stored position q.  Each thread finds the
-// fibre of its first position by binary search over ptr and walks forward
-// over kExpandRun consecutive positions (8 measured faster than 32: more
-// threads in flight outweigh the extra searches).
+// fib[q] = the fibre that owns stored position q.  A wave takes 512
+// consecutive positions (8 per lane): the fibres owning its first and last
+// position are found by a cooperative 32-ary search (four dependent loads
+// for a million fibres), their ends are loaded into LDS, and each lane finds
+// its first position's fibre there and walks forward.
 constexpr int kExpandRun = 8;
+constexpr int kExpandTile = kWave * kExpandRun;
+constexpr int kExpandWin = 512;
 template <typename TP>
 __global__ __launch_bounds__(256) void expand_fibres_kernel(const TP* __restrict__ ptr,
                                                             int64_t base, int64_t n_outer,
                                                             int64_t nnz, uint32_t* __restrict__ fib) {
-  const int64_t q0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * kExpandRun;
-  if (q0 >= nnz) return;
-  int64_t lo = 0, hi = n_outer - 1;
-  while (lo < hi) {
-    const int64_t mid = lo + ((hi - lo) >> 1);
-    if (pos_at(ptr, base, mid + 1) <= q0) lo = mid + 1; else hi = mid;
-  }
-  int64_t o = lo, end = pos_at(ptr, base, o + 1);
+  __shared__ int64_t lds[4][kExpandWin];
+  const int w = wave_in_block();
+  const int64_t q0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * kExpandTile;
+  if (q0 >= nnz) return;  // wave-uniform
+  const int64_t qlast = (q0 + kExpandTile < nnz ? q0 + kExpandTile : nnz) - 1;
+  const int lane = lane_id();
+  const int64_t r = group_find_fiber<32>(ptr, base, 0, n_outer, lane < 32 ? q0 : qlast);
+  const int64_t r_lo = shfl(r, 0), r_hi = shfl(r, 32);
+  const FiberWindow<TP> win = load_window(ptr, base, r_lo, r_hi - r_lo + 1, lds[w], kExpandWin);
+  const int64_t a = q0 + static_cast<int64_t>(lane) * kExpandRun;
+  if (a > qlast) return;
+  int64_t o = win.find(a);
   typedef unsigned v4u __attribute__((ext_vector_type(4)));
-  const bool whole = q0 + kExpandRun <= nnz;
+  const bool whole = a + kExpandRun <= nnz;
 #pragma unroll
   for (int c = 0; c < kExpandRun / 4; c++) {
     uint32_t f[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int64_t q = q0 + c * 4 + i;
-      while (q >= end && o + 1 < n_outer) {
-        o++;
-        end = pos_at(ptr, base, o + 1);
-      }
+      const int64_t q = a + c * 4 + i;
+      while (o < r_hi && win.end_of(o) <= q) o++;
       f[i] = static_cast<uint32_t>(o);
     }
     if (whole) {
-      *reinterpret_cast<v4u*>(fib + q0 + c * 4) = v4u{f[0], f[1], f[2], f[3]};
+      *reinterpret_cast<v4u*>(fib + a + c * 4) = v4u{f[0], f[1], f[2], f[3]};
     } else {
       for (int i = 0; i < 4; i++)
-        if (q0 + c * 4 + i < nnz) fib[q0 + c * 4 + i] = f[i];
+        if (a + c * 4 + i < nnz) fib[a + c * 4 + i] = f[i];
     }
   }
 }
@@ -269,17 +273,38 @@ __global__ __launch_bounds__(256) void expand_fibres_kernel(const TP* __restrict
 // ptr[j] = base + (first position p with keys[p] >= j) for j in [0, nk],
 // from keys sorted ascending: position p writes the fibres that start at p
 // (those after keys[p-1] up to keys[p]).  Keys are clamped to nk so that no
-// write leaves ptr.
+// write leaves ptr.  Each thread takes 4 consecutive positions with one
+// 16-byte load (the key before them is one more load, mostly an L1 hit).
+template <typename TP>
+__device__ __forceinline__ void ptr_at_position(const uint32_t* __restrict__ keys, int64_t n,
+                                                int64_t nk, int64_t base, int64_t p,
+                                                int64_t prev, int64_t cur, TP* __restrict__ ptr) {
+  const int64_t lo = p == 0 ? 0 : std::min<int64_t>(prev, nk) + 1;
+  const int64_t hi = p == n ? nk : std::min<int64_t>(cur, nk);
+  for (int64_t j = lo; j <= hi; j++) ptr[j] = static_cast<TP>(base + p);
+}
+
 template <typename TP>
 __global__ __launch_bounds__(256) void ptr_from_keys_kernel(const uint32_t* __restrict__ keys,
                                                             int64_t n, int64_t nk, int64_t base,
                                                             TP* __restrict__ ptr) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p <= n;
-       p += stride) {
-    const int64_t lo = p == 0 ? 0 : std::min<int64_t>(keys[p - 1], nk) + 1;
-    const int64_t hi = p == n ? nk : std::min<int64_t>(keys[p], nk);
-    for (int64_t j = lo; j <= hi; j++) ptr[j] = static_cast<TP>(base + p);
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x * 4;
+  for (int64_t p0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; p0 <= n;
+       p0 += stride) {
+    int64_t k[5];
+    k[0] = p0 == 0 ? 0 : keys[p0 - 1];
+    if (p0 + 4 <= n && (reinterpret_cast<uintptr_t>(keys) & 15) == 0) {
+      const v4u q = *reinterpret_cast<const v4u*>(keys + p0);
+      k[1] = q.x; k[2] = q.y; k[3] = q.z; k[4] = q.w;
+    } else {
+      for (int i = 0; i < 4; i++) k[i + 1] = p0 + i < n ? keys[p0 + i] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int64_t p = p0 + i;
+      if (p <= n) ptr_at_position<TP>(keys, n, nk, base, p, k[i], k[i + 1], ptr);
+    }
   }
 }
 
@@ -479,8 +504,8 @@ int radix_sort(const TK* keys, int64_t kbase, const TQ* pay, int64_t pbase, cons
 template <typename TP>
 int write_ptr_from_keys(const uint32_t* keys, int64_t n, int64_t nk, int64_t base, TP* ptr,
                         hipStream_t s) {
-  hipLaunchKernelGGL((ptr_from_keys_kernel<TP>), dim3(grid_for(n + 1, 256)), dim3(256), 0, s, keys,
-                     n, nk, base, ptr);
+  hipLaunchKernelGGL((ptr_from_keys_kernel<TP>), dim3(grid_for(n + 1, 256 * 4, 16384)), dim3(256),
+                     0, s, keys, n, nk, base, ptr);
   return launch_check("ptr_from_keys_kernel");
 }
 
@@ -499,7 +524,7 @@ int transpose_typed(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_
   uint32_t* kfin = c.take<uint32_t>(n);
   SortBufs<TW> S;
   S.carve(c, n);
-  hipLaunchKernelGGL((expand_fibres_kernel<TP>), dim3(grid_for(n, 256 * kExpandRun, INT32_MAX)),
+  hipLaunchKernelGGL((expand_fibres_kernel<TP>), dim3(grid_for(n, 4 * kExpandTile, INT32_MAX)),
                      dim3(256), 0, s, static_cast<const TP*>(A->ptr), base, A->n_outer, n, fib);
   FGX_TRY(launch_check("expand_fibres_kernel"));
   FGX_TRY((radix_sort<TI, uint32_t, TW, TI>(static_cast<const TI*>(A->idx), base, fib, 0,
