@@ -75,6 +75,23 @@ end
 
 Finch.get_num_tasks(::GPU) = 1
 
+"""Release the context (its stream, scratch and plan cache); `dev` is unusable afterwards."""
+close!(dev::GPU) = check(ccall((:fgx_ctx_destroy, libfgx), Cint, (Ptr{Cvoid},), dev.ctx))
+
+"""The context's own non-blocking stream (a hipStream_t), for callers that want one."""
+stream(dev::GPU) = ccall((:fgx_ctx_stream, libfgx), Ptr{Cvoid}, (Ptr{Cvoid},), dev.ctx)
+
+"""Wait for the work queued on `stream` (C_NULL: the HIP null stream)."""
+sync(dev::GPU, stream::Ptr{Cvoid} = C_NULL) =
+    check(ccall((:fgx_sync, libfgx), Cint, (Ptr{Cvoid}, Ptr{Cvoid}), dev.ctx, stream))
+
+"""ABI version of the loaded library (`FGX_ABI_VERSION` in include/finchgpu.h)."""
+abi_version() = ccall((:fgx_version, libfgx), Cint, ())
+
+"""Drop the context's cached per-operand plans (tile tables): needed only after
+an operand's `ptr` was freed and its address reused, to release memory early."""
+clear_plans!(dev::GPU) = check(ccall((:fgx_plan_cache_clear, libfgx), Cint, (Ptr{Cvoid},), dev.ctx))
+
 """A device vector: the storage of a level buffer on the GPU (hipMalloc'd)."""
 mutable struct GPUVector{T} <: AbstractVector{T}
     ptr::Ptr{T}
@@ -246,6 +263,105 @@ function Finch.dropfills(dev::GPU, A::Tensor)
     resize!(idx, nnz[]); resize!(val, nnz[])
     Tensor(DenseLevel(SparseListLevel(ElementLevel{Finch.fill_value(A)}(val), A.lvl.lvl.shape, ptr, idx),
                       A.lvl.shape))
+end
+
+# ---- value-precision form -------------------------------------------------------------
+# The BF16/F16 storage form of an operand's values (FP32 accumulation in every
+# kernel).  Finch has no 16-bit float element in this path; the converted
+# buffer backs an ElementLevel whose values the kernels widen on load.
+function convert_values!(dev::GPU, dst::GPUVector, dst_type::Int32, src::GPUVector, src_type::Int32)
+    check(ccall((:fgx_convert, libfgx), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Cvoid}, Cint, Int64, Ptr{Cvoid}),
+                dev.ctx, src.ptr, src_type, dst.ptr, dst_type, length(src), C_NULL))
+    dst
+end
+
+# ---- batched form ---------------------------------------------------------------------
+struct FgxBatchItem           # struct fgx_batch_item
+    A::FgxCompressed
+    n_rhs::Int64
+    B::Ptr{Cvoid}
+    C::Ptr{Cvoid}
+end
+
+const FGX_BATCH_FORCE_ROWSPLIT, FGX_BATCH_FORCE_NNZSPLIT, FGX_BATCH_NO_GRAPH = 1, 2, 4
+
+"""Many independent `C = A*B` (CSR A, row-major F32 B/C with `n_rhs` columns)
+in one persistent launch, replayed from a hipGraph.  The items' buffers must
+outlive the plan."""
+mutable struct BatchPlan
+    dev::GPU
+    handle::Ptr{Cvoid}
+    items::Vector{FgxBatchItem}
+end
+
+function BatchPlan(dev::GPU, items::Vector{FgxBatchItem}; flags::Integer = 0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:fgx_batch_plan_create, libfgx), Cint,
+                (Ptr{Cvoid}, Ptr{FgxBatchItem}, Int64, Cint, Ref{Ptr{Cvoid}}),
+                dev.ctx, items, length(items), flags, h))
+    plan = BatchPlan(dev, h[], items)
+    finalizer(p -> ccall((:fgx_batch_plan_destroy, libfgx), Cint, (Ptr{Cvoid},), p.handle), plan)
+    plan
+end
+
+run!(plan::BatchPlan) = check(ccall((:fgx_batch_plan_run, libfgx), Cint,
+                                    (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                                    plan.dev.ctx, plan.handle, C_NULL))
+
+"""[work items, row-split items, nnz-split items, grid, nnz, algorithmic bytes per run]."""
+function stats(plan::BatchPlan)
+    st = zeros(Int64, 6)
+    check(ccall((:fgx_batch_plan_stats, libfgx), Cint, (Ptr{Cvoid}, Ptr{Int64}), plan.handle, st))
+    st
+end
+
+"""The plain path: one SpMM (SpMV for n_rhs == 1) call per item."""
+run_loop!(dev::GPU, items::Vector{FgxBatchItem}) =
+    check(ccall((:fgx_batch_run_loop, libfgx), Cint,
+                (Ptr{Cvoid}, Ptr{FgxBatchItem}, Int64, Ptr{Cvoid}),
+                dev.ctx, items, length(items), C_NULL))
+
+# ---- serving loop ---------------------------------------------------------------------
+const FGX_REQ_SPMV, FGX_REQ_SPMM, FGX_REQ_SDDMM = Int32(0), Int32(1), Int32(2)
+
+struct FgxRequest             # struct fgx_request
+    kind::Int32
+    operand::Int32
+    k::Int64
+    arrival_s::Float64
+    in::Ptr{Cvoid}
+    in2::Ptr{Cvoid}
+    out::Ptr{Cvoid}
+end
+
+struct FgxServeConfig         # struct fgx_serve_config
+    hbm_budget_bytes::Int64
+    max_slots::Int32
+    max_batch::Int32
+    mode::Int32               # 0: continuous batching, 1: plain per-call path
+    reserved::Int32
+end
+
+struct FgxRequestResult       # struct fgx_request_result
+    admit_s::Float64
+    done_s::Float64
+    status::Int32
+    batch::Int32
+end
+
+"""Replay an open-loop request trace against device-resident operands
+(continuous batching with HBM admission); returns one result per request."""
+function serve(dev::GPU, operands::Vector{FgxCompressed}, reqs::Vector{FgxRequest};
+               hbm_budget_bytes::Integer = 200 * 10^9, max_slots::Integer = 4096,
+               max_batch::Integer = 512, plain::Bool = false)
+    cfg = FgxServeConfig(hbm_budget_bytes, max_slots, max_batch, plain ? 1 : 0, 0)
+    res = Vector{FgxRequestResult}(undef, length(reqs))
+    check(ccall((:fgx_serve_run, libfgx), Cint,
+                (Ptr{Cvoid}, Ptr{FgxCompressed}, Int64, Ptr{FgxRequest}, Int64,
+                 Ref{FgxServeConfig}, Ptr{FgxRequestResult}),
+                dev.ctx, operands, length(operands), reqs, length(reqs), cfg, res))
+    res
 end
 
 end # module

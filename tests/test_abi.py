@@ -33,6 +33,15 @@ def test_library_exports_every_declared_symbol():
     assert sorted(_lib.SIGNATURES) == declared_symbols()
 
 
+def test_julia_binding_calls_every_declared_symbol():
+    # The Julia glue a Finch maintainer would load binds the same C ABI: each
+    # declared entry point appears in one of its ccall targets.
+    text = open(os.path.join(REPO, "finch.jl_amd", "julia", "FinchGPU.jl")).read()
+    called = set(re.findall(r"ccall\(\(:(fgx_[a-z0-9_]+)", text))
+    missing = [s for s in declared_symbols() if s not in called]
+    assert not missing, missing
+
+
 def test_version_and_error_paths_without_device():
     lib = _lib.load()
     assert lib.fgx_version() == 1
