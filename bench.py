@@ -193,6 +193,11 @@ def main():
         served = bench_serving(dev, args.serve_slo_ms, [A])
         print("# serving done", file=sys.stderr, flush=True)
 
+    traffic = measured_traffic()
+    # The rate at which the call moves its measured HBM traffic (the B-row
+    # gathers that the algorithmic count credits once): how close the kernel
+    # runs to the bandwidth limit, beside the algorithmic fraction.
+    traffic_gbs = traffic / (spmm_ev_ms * 1e-3) / 1e9 if traffic else None
     line = {
         "metric": "SpMM/SpMV HBM GB/s (% of peak); served req/s at p99, batched vs plain",
         "value": round(value, 2),
@@ -212,7 +217,9 @@ def main():
         "pct_of_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(),
+                     "traffic": traffic,
+                     "traffic_gbs": round(traffic_gbs, 1) if traffic_gbs else None,
+                     "traffic_frac": round(traffic_gbs / HBM_PEAK_GBS, 4) if traffic_gbs else None,
                      "traffic_source": "profiles/spmm_csr_powerlaw_1m_k64.md (rocprofv3 "
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel": "spmm_csr_kernel + fixup_chunk_kernel (one fgx_spmm_csr call)",
