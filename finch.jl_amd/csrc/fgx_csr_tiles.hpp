@@ -96,62 +96,33 @@ __device__ __forceinline__ int64_t encode_carry(bool carries, bool continues, in
   return carries ? (continues ? -(r + 2) : r) : -1;
 }
 
-// ---------------------------------------------------------------------------
-// SpMV tile: lane-level runs of V positions.  Writes y for every row the tile
-// finalises and (carry_row, carry_val) of the row continuing into the next
-// tile (carry_row = -1 when no positions of this tile belong to it).
-template <typename TP, typename TI, typename TV, typename TA, int V>
-__device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
-                                          const TV* __restrict__ val, int64_t base, int64_t m,
-                                          int64_t nnz, const TA* __restrict__ x,
-                                          TA* __restrict__ y, int64_t tile, bool vec,
-                                          const int64_t* __restrict__ table, int64_t* lds_win,
-                                          int64_t* carry_row_out, TA* carry_val_out) {
-  constexpr int W = kWave * V;
-  const int lane = lane_id();
-  const int64_t q0 = tile * W;
-  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
-  const int64_t a = q0 + static_cast<int64_t>(lane) * V;
-  const int64_t b = (a + V < qend) ? a + V : qend;
-  const bool active = a < qend || (a == 0 && m > 0);
+// A fibre window known to be cached in LDS: win[t] = pos(r0 + t + 1).
+struct LdsWindow {
+  const int64_t* win;
+  int64_t r0;
+  int64_t n;
+  __device__ __forceinline__ int64_t end_of(int64_t r) const { return win[r - r0]; }
+  __device__ __forceinline__ int64_t find(int64_t target) const {
+    int lo = 0, hi = static_cast<int>(n);
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (win[mid] <= target) lo = mid + 1; else hi = mid;
+    }
+    return r0 + lo;
+  }
+};
 
-  // Stream the run's values and coordinates first: these loads depend only
-  // on the tile index, so they are in flight while the tile's fibres are
-  // located (the window set-up ends in a fence that would otherwise hold
-  // them back).
-  TV vv[V];
-  TI jj[V];
-  if (active && b - a == V) {
-    ld_run<TV, V>(val + a, vec, vv);
-    ld_run<TI, V>(idx + a, vec, jj);
-  } else {
-#pragma unroll
-    for (int s = 0; s < V; s++) {
-      const bool in = a + s < b;
-      vv[s] = in ? ld_stream(val + a + s) : TV{};
-      jj[s] = in ? ld_stream(idx + a + s) : TI{};
-    }
-  }
-  // Gather x (independent loads, issued together), then locate the tile's
-  // fibres while the gathers are in flight.  The gathers are unconditional:
-  // positions past the run read x[0] (valid whenever the tile holds a
-  // position) and are never accumulated.  A load under a per-position branch
-  // makes the compiler wait for every earlier load before issuing the next.
-  TA xg[V];
-  if (qend > q0) {
-#pragma unroll
-    for (int s = 0; s < V; s++) {
-      const bool in = a + s < b;
-      const int64_t j = in ? static_cast<int64_t>(jj[s]) - base : 0;
-      xg[s] = x[j];
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < V; s++) xg[s] = TA(0);
-  }
-  int64_t r_lo, r_hi, pos_lo;
-  FiberWindow<TP> win =
-      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
+// The walk of an SpMV tile over its fibre window (LdsWindow or FiberWindow):
+// finalise the rows that end in each lane's run, zero the tile's empty rows,
+// complete held-back rows by a segmented scan, and leave the carry.
+template <typename TA, int V, typename Win, typename TV>
+__device__ __forceinline__ void spmv_tile_walk(const Win win, int64_t m, int64_t q0, int64_t qend,
+                                               int64_t a, int64_t b, bool active,
+                                               const TV (&vv)[V], const TA (&xg)[V],
+                                               TA* __restrict__ y, int64_t tile, int64_t r_lo,
+                                               int64_t r_hi, int64_t pos_lo,
+                                               int64_t* carry_row_out, TA* carry_val_out) {
+  const int lane = lane_id();
   const int64_t r_win_last = r_hi < m ? r_hi : m - 1;
 
   TA acc = 0;
@@ -225,6 +196,71 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
   }
 }
 
+// ---------------------------------------------------------------------------
+// SpMV tile: lane-level runs of V positions.  Writes y for every row the tile
+// finalises and (carry_row, carry_val) of the row continuing into the next
+// tile (carry_row = -1 when no positions of this tile belong to it).
+template <typename TP, typename TI, typename TV, typename TA, int V>
+__device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
+                                          const TV* __restrict__ val, int64_t base, int64_t m,
+                                          int64_t nnz, const TA* __restrict__ x,
+                                          TA* __restrict__ y, int64_t tile, bool vec,
+                                          const int64_t* __restrict__ table, int64_t* lds_win,
+                                          int64_t* carry_row_out, TA* carry_val_out) {
+  constexpr int W = kWave * V;
+  const int lane = lane_id();
+  const int64_t q0 = tile * W;
+  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  const int64_t a = q0 + static_cast<int64_t>(lane) * V;
+  const int64_t b = (a + V < qend) ? a + V : qend;
+  const bool active = a < qend || (a == 0 && m > 0);
+
+  // Stream the run's values and coordinates first: these loads depend only
+  // on the tile index, so they are in flight while the tile's fibres are
+  // located (the window set-up ends in a fence that would otherwise hold
+  // them back).
+  TV vv[V];
+  TI jj[V];
+  if (active && b - a == V) {
+    ld_run<TV, V>(val + a, vec, vv);
+    ld_run<TI, V>(idx + a, vec, jj);
+  } else {
+#pragma unroll
+    for (int s = 0; s < V; s++) {
+      const bool in = a + s < b;
+      vv[s] = in ? ld_stream(val + a + s) : TV{};
+      jj[s] = in ? ld_stream(idx + a + s) : TI{};
+    }
+  }
+  // Gather x (independent loads, issued together), then locate the tile's
+  // fibres while the gathers are in flight.  The gathers are unconditional:
+  // positions past the run read x[0] (valid whenever the tile holds a
+  // position) and are never accumulated.  A load under a per-position branch
+  // makes the compiler wait for every earlier load before issuing the next.
+  TA xg[V];
+  if (qend > q0) {
+#pragma unroll
+    for (int s = 0; s < V; s++) {
+      const bool in = a + s < b;
+      const int64_t j = in ? static_cast<int64_t>(jj[s]) - base : 0;
+      xg[s] = x[j];
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < V; s++) xg[s] = TA(0);
+  }
+  int64_t r_lo, r_hi, pos_lo;
+  FiberWindow<TP> win =
+      tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
+  // The window is almost always cached in LDS; that case runs with an
+  // LDS-only accessor, so the walk's loops carry no global fallback.
+  if (win.cached)
+    spmv_tile_walk<TA, V>(LdsWindow{win.win, win.r0, win.n}, m, q0, qend, a, b, active, vv, xg,
+                          y, tile, r_lo, r_hi, pos_lo, carry_row_out, carry_val_out);
+  else
+    spmv_tile_walk<TA, V>(win, m, q0, qend, a, b, active, vv, xg, y, tile, r_lo, r_hi, pos_lo,
+                          carry_row_out, carry_val_out);
+}
 // ---------------------------------------------------------------------------
 // SpMM tile: groups of G lanes, each lane owning KV consecutive dense columns
 // of the [kofs, kofs + G*KV) slice; each group walks E consecutive positions.
