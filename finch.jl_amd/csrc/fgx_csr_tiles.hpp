@@ -134,30 +134,31 @@ __device__ __forceinline__ void spmv_tile_walk(const Win win, int64_t m, int64_t
   // rows (same end as the row before): those are zeroed below by the whole
   // wave, with coalesced stores, instead of one scattered store per lane.
   int64_t last_end = -1;
+  // Finalise row r, which ends at e: the first row a run finalises is held
+  // back for the scan, later ones are stored unless empty.  (r <= r_win_last
+  // implies r < m.)
+  auto finish = [&](int64_t e) {
+    const bool store = has_emit && e != last_end;
+    first_row = has_emit ? first_row : r;
+    first_val = has_emit ? first_val : acc;
+    has_emit = true;
+    if (store) y[r] = acc;
+    last_end = e;
+    acc = 0;
+    r++;
+  };
   if (active) {
 #pragma unroll
     for (int s = 0; s < V; s++) {
       const int64_t q = a + s;
       if (q < b) {
         int64_t e;
-        while (r < m && r <= r_win_last && (e = win.end_of(r)) <= q) {
-          if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
-          else if (e != last_end) y[r] = acc;
-          last_end = e;
-          acc = 0;
-          r++;
-        }
+        while (r <= r_win_last && (e = win.end_of(r)) <= q) finish(e);
         acc += widen<TA>(vv[s]) * xg[s];
       }
     }
     int64_t e;
-    while (r < m && r <= r_win_last && (e = win.end_of(r)) <= b) {
-      if (!has_emit) { has_emit = true; first_row = r; first_val = acc; }
-      else if (e != last_end) y[r] = acc;
-      last_end = e;
-      acc = 0;
-      r++;
-    }
+    while (r <= r_win_last && (e = win.end_of(r)) <= b) finish(e);
   }
   // Empty rows the tile owns: rows ending in (q0, qend] (from position 0 on
   // for the first tile), i.e. [r_lo, r_hi); row r is empty when it ends where
