@@ -46,11 +46,11 @@ __device__ __forceinline__ uint32_t rebased(const T* __restrict__ a, int64_t bas
 // (e % kWaveSpan) / 64, lane e % 64.
 constexpr int kWaveSpan = kSortItems * kWave;
 
-// Lanes of the wave whose digit equals this lane's (among lanes with `in`).
-__device__ __forceinline__ uint64_t same_digit(uint32_t d, bool in) {
+// Lanes of the wave whose digit (dbits wide) equals this lane's (among lanes
+// with `in`).
+__device__ __forceinline__ uint64_t same_digit(uint32_t d, bool in, int dbits) {
   uint64_t same = ballot(in);
-#pragma unroll
-  for (int b = 0; b < kDigitBits; b++) {
+  for (int b = 0; b < dbits; b++) {
     const bool bit = ((d >> b) & 1u) != 0;
     const uint64_t m = ballot(bit);
     same &= bit ? m : ~m;
@@ -58,28 +58,32 @@ __device__ __forceinline__ uint64_t same_digit(uint32_t d, bool in) {
   return same;
 }
 
-// Digit counts of one tile of keys: hist[d * ntiles + tile].
+// Digit counts of one tile of keys: hist[d * ntiles + tile], for digits of
+// dbits (<= kDigitBits) bits starting at bit `shift`.
 template <typename TK>
 __global__ __launch_bounds__(kSortThreads) void sort_hist_kernel(const TK* __restrict__ keys,
                                                                  int64_t kbase, int64_t n,
-                                                                 int shift, int64_t ntiles,
+                                                                 int shift, int dbits,
+                                                                 int64_t ntiles,
                                                                  int64_t* __restrict__ hist) {
   __shared__ uint32_t h[kBins];
   const int tid = threadIdx.x;
+  const int nbins = 1 << dbits;
+  const uint32_t dmask = static_cast<uint32_t>(nbins - 1);
   if (tid < kBins) h[tid] = 0;
   const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortTile;
   uint32_t k[kSortItems];
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     const int64_t e = t0 + static_cast<int64_t>(i) * kSortThreads + tid;
-    k[i] = e < n ? (rebased(keys, kbase, e) >> shift) & (kBins - 1) : kBins;
+    k[i] = e < n ? (rebased(keys, kbase, e) >> shift) & dmask : kBins;
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kSortItems; i++)
     if (k[i] < kBins) atomicAdd(&h[k[i]], 1u);
   __syncthreads();
-  if (tid < kBins) hist[static_cast<int64_t>(tid) * ntiles + blockIdx.x] = h[tid];
+  if (tid < nbins) hist[static_cast<int64_t>(tid) * ntiles + blockIdx.x] = h[tid];
 }
 
 // Tile processed by this block.  Blocks go round-robin over the 8 XCDs;
@@ -113,7 +117,7 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t* wsum) {
 // Stable scatter of one tile, in three steps.
 //   1. Wave ranks: each wave walks its sub-tile in element order, 64
 //      elements a round; the lanes sharing an element's digit are found with
-//      kDigitBits ballots, and the wave keeps its own running digit counts in
+//      dbits ballots, and the wave keeps its own running digit counts in
 //      LDS (read by every lane, then advanced by each group's lowest lane: one
 //      instruction stream, so no barrier is needed).
 //   2. Tile ranks: the waves' counts are combined per digit (one barrier) into
@@ -128,7 +132,7 @@ __device__ __forceinline__ uint32_t block_prefix(uint32_t v, uint32_t* wsum) {
 template <typename TK, typename TQ, typename TW, typename TQO>
 __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     const TK* __restrict__ keys, int64_t kbase, const TQ* __restrict__ pay, int64_t pbase,
-    const TW* __restrict__ val, int64_t n, int shift, int64_t ntiles,
+    const TW* __restrict__ val, int64_t n, int shift, int dbits, int64_t ntiles,
     const int64_t* __restrict__ offs, uint32_t* __restrict__ keys_out, TQO* __restrict__ pay_out,
     int64_t pout_base, TW* __restrict__ val_out) {
   // Small tables (int32 / uint16 so that three blocks fit a CU's LDS):
@@ -142,6 +146,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   const int tid = threadIdx.x;
   const int lane = lane_id();
   const int w = wave_in_block();
+  const int nbins = 1 << dbits;
+  const uint32_t dmask = static_cast<uint32_t>(nbins - 1);
   const int64_t tile = xcd_tile();
   const int64_t t0 = tile * kSortTile;
   const int64_t e0 = t0 + w * kWaveSpan + lane;
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     const bool in = e0 + i * kWave < n;
-    const uint32_t d = (k[i] >> shift) & (kBins - 1);
-    const uint64_t same = same_digit(d, in);
+    const uint32_t d = (k[i] >> shift) & dmask;
+    const uint64_t same = same_digit(d, in, dbits);
     const uint32_t rank = static_cast<uint32_t>(popc(same & below));
     const uint32_t c = cnt[d];
     r[i] = c + rank;
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
   __syncthreads();
   // 2. Tile ranks: digit tid's start in the tile, then each wave's first slot.
   {
-    const bool owner = tid < kBins;
+    const bool owner = tid < nbins;
     const int64_t cell = static_cast<int64_t>(owner ? tid : 0) * ntiles + tile;
     const int64_t g0 = owner ? offs[cell] : 0;
     const uint32_t tcnt = owner ? static_cast<uint32_t>(offs[cell + 1] - g0) : 0;
@@ -204,7 +210,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
 #pragma unroll
   for (int i = 0; i < kSortItems; i++) {
     if (e0 + i * kWave < n) {
-      const uint32_t pos = cnt[(k[i] >> shift) & (kBins - 1)] + r[i];
+      const uint32_t pos = cnt[(k[i] >> shift) & dmask] + r[i];
       sk[pos] = k[i];
       sq[pos] = p[i];
       sv[pos] = x[i];
@@ -218,7 +224,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_scatter_kernel(
     const int t = i * kSortThreads + tid;
     if (t < tile_n) {
       const uint32_t kk = sk[t];
-      const int64_t pos = static_cast<int64_t>(gbase[(kk >> shift) & (kBins - 1)]) + t;
+      const int64_t pos = static_cast<int64_t>(gbase[(kk >> shift) & dmask]) + t;
       keys_out[pos] = kk;
       pay_out[pos] = static_cast<TQO>(static_cast<int64_t>(sq[t]) + pout_base);
       val_out[pos] = sv[t];
@@ -459,20 +465,21 @@ struct SortBufs {
   }
 };
 
-// One radix pass on digit `shift`: counts, scan, stable scatter.
+// One radix pass on the dbits-wide digit at bit `shift`: counts, scan,
+// stable scatter.
 template <typename TK, typename TQ, typename TW, typename TQO>
 int sort_pass(const TK* keys, int64_t kbase, const TQ* pay, int64_t pbase, const TW* val, int64_t n,
-              int shift, SortBufs<TW>& S, uint32_t* kout, TQO* pout, int64_t pout_base, TW* vout,
-              hipStream_t s) {
+              int shift, int dbits, SortBufs<TW>& S, uint32_t* kout, TQO* pout, int64_t pout_base,
+              TW* vout, hipStream_t s) {
   const int64_t ntiles = (n + kSortTile - 1) / kSortTile;
   hipLaunchKernelGGL((sort_hist_kernel<TK>), dim3(static_cast<unsigned>(ntiles)),
-                     dim3(kSortThreads), 0, s, keys, kbase, n, shift, ntiles, S.hist);
+                     dim3(kSortThreads), 0, s, keys, kbase, n, shift, dbits, ntiles, S.hist);
   FGX_TRY(launch_check("sort_hist_kernel"));
-  FGX_TRY(scan_counts_to_ptr<int64_t>(S.hist, static_cast<int64_t>(kBins) * ntiles, 0, S.offs,
-                                      S.work, nullptr, nullptr, s));
+  FGX_TRY(scan_counts_to_ptr<int64_t>(S.hist, (int64_t(1) << dbits) * ntiles, 0, S.offs, S.work,
+                                      nullptr, nullptr, s));
   hipLaunchKernelGGL((sort_scatter_kernel<TK, TQ, TW, TQO>), dim3(static_cast<unsigned>(ntiles)),
-                     dim3(kSortThreads), 0, s, keys, kbase, pay, pbase, val, n, shift, ntiles,
-                     S.offs, kout, pout, pout_base, vout);
+                     dim3(kSortThreads), 0, s, keys, kbase, pay, pbase, val, n, shift, dbits,
+                     ntiles, S.offs, kout, pout, pout_base, vout);
   return launch_check("sort_scatter_kernel");
 }
 
@@ -483,21 +490,26 @@ template <typename TK, typename TQ, typename TW, typename TQO>
 int radix_sort(const TK* keys, int64_t kbase, const TQ* pay, int64_t pbase, const TW* val,
                int64_t n, int bits, SortBufs<TW>& S, uint32_t* kout, TQO* pout, int64_t pout_base,
                TW* vout, hipStream_t s) {
+  // As few passes as 8-bit digits need, with the bits spread evenly over
+  // them: a 20-bit key takes 7+7+6 rather than 8+8+4, so each digit's run in
+  // a 4096-element tile averages 32 elements (a full 128-byte line of keys)
+  // instead of 16.
   const int passes = std::max(1, (bits + kDigitBits - 1) / kDigitBits);
+  const int dbits = std::max(1, (bits + passes - 1) / passes);
   if (passes == 1)
-    return sort_pass<TK, TQ, TW, TQO>(keys, kbase, pay, pbase, val, n, 0, S, kout, pout,
+    return sort_pass<TK, TQ, TW, TQO>(keys, kbase, pay, pbase, val, n, 0, dbits, S, kout, pout,
                                       pout_base, vout, s);
-  FGX_TRY((sort_pass<TK, TQ, TW, uint32_t>(keys, kbase, pay, pbase, val, n, 0, S, S.k[0], S.q[0],
-                                           0, S.v[0], s)));
+  FGX_TRY((sort_pass<TK, TQ, TW, uint32_t>(keys, kbase, pay, pbase, val, n, 0, dbits, S, S.k[0],
+                                           S.q[0], 0, S.v[0], s)));
   int cur = 0;
   for (int pass = 1; pass < passes - 1; pass++) {
     FGX_TRY((sort_pass<uint32_t, uint32_t, TW, uint32_t>(S.k[cur], 0, S.q[cur], 0, S.v[cur], n,
-                                                          pass * kDigitBits, S, S.k[cur ^ 1],
+                                                          pass * dbits, dbits, S, S.k[cur ^ 1],
                                                           S.q[cur ^ 1], 0, S.v[cur ^ 1], s)));
     cur ^= 1;
   }
   return sort_pass<uint32_t, uint32_t, TW, TQO>(S.k[cur], 0, S.q[cur], 0, S.v[cur], n,
-                                                (passes - 1) * kDigitBits, S, kout, pout,
+                                                (passes - 1) * dbits, dbits, S, kout, pout,
                                                 pout_base, vout, s);
 }
 
