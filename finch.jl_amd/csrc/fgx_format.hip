@@ -374,34 +374,36 @@ __device__ __forceinline__ bool is_fill(float v, float f) {
 }
 __device__ __forceinline__ bool is_fill(int32_t v, int32_t f) { return v == f; }
 
+// One byte per entry: 1 if it is kept (not the fill).
 template <typename TV>
 __global__ __launch_bounds__(256) void keep_flags_kernel(const TV* __restrict__ val, int64_t n,
-                                                         TV fill, int64_t* __restrict__ keep) {
+                                                         TV fill, uint8_t* __restrict__ keep) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
     keep[q] = is_fill(val[q], fill) ? 0 : 1;
 }
 
-// c_ptr[o] = base + (kept entries before fibre o).
-template <typename TP>
+// c_ptr[o] = base + (kept entries before fibre o).  kept[q] = kept entries
+// before position q (TK: int32 when the entry count allows, else int64).
+template <typename TP, typename TK>
 __global__ __launch_bounds__(256) void kept_ptr_kernel(const TP* __restrict__ ptr, int64_t base,
                                                        int64_t n_outer,
-                                                       const int64_t* __restrict__ kept,
+                                                       const TK* __restrict__ kept,
                                                        TP* __restrict__ c_ptr) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t o = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; o <= n_outer;
        o += stride)
-    c_ptr[o] = static_cast<TP>(base + kept[pos_at(ptr, base, o)]);
+    c_ptr[o] = static_cast<TP>(base + static_cast<int64_t>(kept[pos_at(ptr, base, o)]));
 }
 
-template <typename TI, typename TV>
+template <typename TI, typename TV, typename TK>
 __global__ __launch_bounds__(256) void compact_kernel(const TI* __restrict__ idx,
                                                       const TV* __restrict__ val, int64_t n,
-                                                      const int64_t* __restrict__ kept,
+                                                      const TK* __restrict__ kept,
                                                       TI* __restrict__ c_idx, TV* __restrict__ c_val) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
-    const int64_t u = kept[q];
+    const TK u = kept[q];
     if (kept[q + 1] != u) {
       c_idx[u] = idx[q];
       c_val[u] = val[q];
@@ -615,28 +617,36 @@ int dropfills_typed(fgx_ctx* ctx, const fgx_compressed* A, TV fill, void* c_ptr,
     *c_nnz = 0;
     return write_ptr_from_keys<TP>(nullptr, 0, A->n_outer, base, static_cast<TP*>(c_ptr), s);
   }
-  void* scr;
-  FGX_TRY(scratch(ctx, (2 * static_cast<size_t>(n) + 1) * 8 + 1024 + scan_work_bytes(n), &scr));
-  Carver c{static_cast<char*>(scr)};
-  int64_t* keep = c.take<int64_t>(n);
-  int64_t* kept = c.take<int64_t>(n + 1);
-  void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
-  const TV* val = static_cast<const TV*>(A->val);
-  hipLaunchKernelGGL((keep_flags_kernel<TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, val, n, fill,
-                     keep);
-  FGX_TRY(launch_check("keep_flags_kernel"));
-  int64_t u = 0;
-  FGX_TRY(scan_counts_to_ptr<int64_t>(keep, n, 0, kept, work, ctx->host_i64, &u, s));
-  hipLaunchKernelGGL((kept_ptr_kernel<TP>), dim3(grid_for(A->n_outer + 1, 256)), dim3(256), 0, s,
-                     static_cast<const TP*>(A->ptr), base, A->n_outer, kept,
-                     static_cast<TP*>(c_ptr));
-  FGX_TRY(launch_check("kept_ptr_kernel"));
-  hipLaunchKernelGGL((compact_kernel<TI, TV>), dim3(grid_for(n, 256)), dim3(256), 0, s,
-                     static_cast<const TI*>(A->idx), val, n, kept, static_cast<TI*>(c_idx),
-                     static_cast<TV*>(c_val));
-  FGX_TRY(launch_check("compact_kernel"));
-  *c_nnz = u;
-  return FGX_OK;
+  // Keep flags are one byte each; the prefix of kept entries is int32 when
+  // it fits (fewer bytes through the scan and the compaction), else int64.
+  auto run = [&](auto kept_tag) -> int {
+    using TK = decltype(kept_tag);
+    void* scr;
+    FGX_TRY(scratch(ctx, static_cast<size_t>(n) + 256 + (static_cast<size_t>(n) + 1) * sizeof(TK) +
+                             256 + scan_work_bytes(n) + 256,
+                    &scr));
+    Carver c{static_cast<char*>(scr)};
+    uint8_t* keep = c.take<uint8_t>(n);
+    TK* kept = c.take<TK>(n + 1);
+    void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
+    const TV* val = static_cast<const TV*>(A->val);
+    hipLaunchKernelGGL((keep_flags_kernel<TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, val, n,
+                       fill, keep);
+    FGX_TRY(launch_check("keep_flags_kernel"));
+    int64_t u = 0;
+    FGX_TRY((scan_counts_to_ptr<TK, uint8_t>(keep, n, 0, kept, work, ctx->host_i64, &u, s)));
+    hipLaunchKernelGGL((kept_ptr_kernel<TP, TK>), dim3(grid_for(A->n_outer + 1, 256)), dim3(256),
+                       0, s, static_cast<const TP*>(A->ptr), base, A->n_outer, kept,
+                       static_cast<TP*>(c_ptr));
+    FGX_TRY(launch_check("kept_ptr_kernel"));
+    hipLaunchKernelGGL((compact_kernel<TI, TV, TK>), dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       static_cast<const TI*>(A->idx), val, n, kept, static_cast<TI*>(c_idx),
+                       static_cast<TV*>(c_val));
+    FGX_TRY(launch_check("compact_kernel"));
+    *c_nnz = u;
+    return FGX_OK;
+  };
+  return n < INT32_MAX ? run(int32_t{}) : run(int64_t{});
 }
 
 }  // namespace

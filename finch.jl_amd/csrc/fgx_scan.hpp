@@ -44,7 +44,8 @@ __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* lds,
   return r;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_block_sums(const int64_t* __restrict__ cnt,
+template <typename TC>
+__global__ __launch_bounds__(kScanThreads) void scan_block_sums(const TC* __restrict__ cnt,
                                                                 int64_t n,
                                                                 int64_t* __restrict__ sums) {
   __shared__ int64_t lds[17];
@@ -76,8 +77,8 @@ __global__ __launch_bounds__(kScanThreads) void scan_sums(int64_t* __restrict__ 
 
 }  // namespace
 
-template <typename TP>
-__global__ __launch_bounds__(kScanThreads) void scan_write_ptr(const int64_t* __restrict__ cnt,
+template <typename TP, typename TC>
+__global__ __launch_bounds__(kScanThreads) void scan_write_ptr(const TC* __restrict__ cnt,
                                                                int64_t n,
                                                                const int64_t* __restrict__ sums,
                                                                int64_t base, TP* __restrict__ ptr) {
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_write_ptr(const int64_t* __
   int64_t s = 0;
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
-    v[i] = (q0 + i < n) ? cnt[q0 + i] : 0;
+    v[i] = (q0 + i < n) ? static_cast<int64_t>(cnt[q0 + i]) : 0;
     s += v[i];
   }
   int64_t tot;
@@ -109,19 +110,20 @@ inline size_t scan_work_bytes(int64_t n) {
   return static_cast<size_t>(nb + 1) * sizeof(int64_t);
 }
 
-template <typename TP>
-int scan_counts_to_ptr(const int64_t* cnt, int64_t n, int64_t base, TP* ptr, void* work,
+// Counts may be any integer type (e.g. one-byte flags); sums are int64.
+template <typename TP, typename TC = int64_t>
+int scan_counts_to_ptr(const TC* cnt, int64_t n, int64_t base, TP* ptr, void* work,
                        int64_t* pinned, int64_t* total_host, hipStream_t s) {
   const int64_t nb = n > 0 ? (n + kScanTile - 1) / kScanTile : 1;
   int64_t* sums = static_cast<int64_t*>(work);
   int64_t* grand = sums + nb;
-  hipLaunchKernelGGL(scan_block_sums, dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0, s,
-                     cnt, n, sums);
+  hipLaunchKernelGGL((scan_block_sums<TC>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0,
+                     s, cnt, n, sums);
   FGX_TRY(launch_check("scan_block_sums"));
   hipLaunchKernelGGL(scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums, nb, grand);
   FGX_TRY(launch_check("scan_sums"));
-  hipLaunchKernelGGL((scan_write_ptr<TP>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads), 0,
-                     s, cnt, n, sums, base, ptr);
+  hipLaunchKernelGGL((scan_write_ptr<TP, TC>), dim3(static_cast<unsigned>(nb)), dim3(kScanThreads),
+                     0, s, cnt, n, sums, base, ptr);
   FGX_TRY(launch_check("scan_write_ptr"));
   if (total_host) {
     FGX_HIP(hipMemcpyAsync(pinned, grand, sizeof(int64_t), hipMemcpyDeviceToHost, s));
