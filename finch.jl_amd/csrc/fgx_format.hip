@@ -375,12 +375,23 @@ __device__ __forceinline__ bool is_fill(float v, float f) {
 __device__ __forceinline__ bool is_fill(int32_t v, int32_t f) { return v == f; }
 
 // One byte per entry: 1 if it is kept (not the fill).
+// Four entries per thread, stored as one 32-bit word.
 template <typename TV>
 __global__ __launch_bounds__(256) void keep_flags_kernel(const TV* __restrict__ val, int64_t n,
                                                          TV fill, uint8_t* __restrict__ keep) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
-    keep[q] = is_fill(val[q], fill) ? 0 : 1;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x * 4;
+  const bool aligned = (reinterpret_cast<uintptr_t>(keep) & 3) == 0;
+  for (int64_t q0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; q0 < n;
+       q0 += stride) {
+    if (q0 + 4 <= n && aligned) {
+      uint32_t w = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) w |= (is_fill(val[q0 + i], fill) ? 0u : 1u) << (8 * i);
+      *reinterpret_cast<uint32_t*>(keep + q0) = w;
+    } else {
+      for (int64_t q = q0; q < n && q < q0 + 4; q++) keep[q] = is_fill(val[q], fill) ? 0 : 1;
+    }
+  }
 }
 
 // c_ptr[o] = base + (kept entries before fibre o).  kept[q] = kept entries
@@ -630,8 +641,8 @@ int dropfills_typed(fgx_ctx* ctx, const fgx_compressed* A, TV fill, void* c_ptr,
     TK* kept = c.take<TK>(n + 1);
     void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
     const TV* val = static_cast<const TV*>(A->val);
-    hipLaunchKernelGGL((keep_flags_kernel<TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, val, n,
-                       fill, keep);
+    hipLaunchKernelGGL((keep_flags_kernel<TV>), dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, val,
+                       n, fill, keep);
     FGX_TRY(launch_check("keep_flags_kernel"));
     int64_t u = 0;
     FGX_TRY((scan_counts_to_ptr<TK, uint8_t>(keep, n, 0, kept, work, ctx->host_i64, &u, s)));

@@ -14,6 +14,78 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
+// The kScanItems consecutive counts cnt[q0 .. q0 + kScanItems) (zero past n),
+// with vector loads when the run is whole and aligned.
+template <typename TC>
+__device__ __forceinline__ void load_counts(const TC* __restrict__ cnt, int64_t q0, int64_t n,
+                                            int64_t (&v)[kScanItems]) {
+  static_assert(kScanItems == 8, "vector loads below take 8 counts");
+  const TC* p = cnt + q0;
+  constexpr uintptr_t kAlign = sizeof(TC) == 1 ? 7 : 15;  // 8-byte loads for byte counts
+  const bool whole = q0 + kScanItems <= n && (reinterpret_cast<uintptr_t>(p) & kAlign) == 0;
+  if constexpr (sizeof(TC) == 1) {
+    if (whole) {
+      const uint64_t u = *reinterpret_cast<const uint64_t*>(p);
+#pragma unroll
+      for (int i = 0; i < 8; i++) v[i] = static_cast<int64_t>(static_cast<TC>(u >> (8 * i)));
+      return;
+    }
+  } else if constexpr (sizeof(TC) == 4) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    if (whole) {
+      const v4i x = reinterpret_cast<const v4i*>(p)[0], y = reinterpret_cast<const v4i*>(p)[1];
+      v[0] = static_cast<TC>(x.x); v[1] = static_cast<TC>(x.y);
+      v[2] = static_cast<TC>(x.z); v[3] = static_cast<TC>(x.w);
+      v[4] = static_cast<TC>(y.x); v[5] = static_cast<TC>(y.y);
+      v[6] = static_cast<TC>(y.z); v[7] = static_cast<TC>(y.w);
+      return;
+    }
+  } else if constexpr (sizeof(TC) == 8) {
+    typedef long long v2l __attribute__((ext_vector_type(2)));
+    if (whole) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const v2l x = reinterpret_cast<const v2l*>(p)[c];
+        v[2 * c] = static_cast<int64_t>(x.x);
+        v[2 * c + 1] = static_cast<int64_t>(x.y);
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) v[i] = (q0 + i < n) ? static_cast<int64_t>(cnt[q0 + i]) : 0;
+}
+
+// Stores the kScanItems values o[] at dst, as vectors when the run is whole
+// and 16-byte aligned.
+template <typename TP>
+__device__ __forceinline__ void store_run(TP* __restrict__ dst, int valid,
+                                          const TP (&o)[kScanItems]) {
+  const bool whole = valid == kScanItems && (reinterpret_cast<uintptr_t>(dst) & 15) == 0;
+  if constexpr (sizeof(TP) == 4) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    if (whole) {
+      reinterpret_cast<v4i*>(dst)[0] = v4i{static_cast<int>(o[0]), static_cast<int>(o[1]),
+                                           static_cast<int>(o[2]), static_cast<int>(o[3])};
+      reinterpret_cast<v4i*>(dst)[1] = v4i{static_cast<int>(o[4]), static_cast<int>(o[5]),
+                                           static_cast<int>(o[6]), static_cast<int>(o[7])};
+      return;
+    }
+  } else if constexpr (sizeof(TP) == 8) {
+    typedef long long v2l __attribute__((ext_vector_type(2)));
+    if (whole) {
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+        reinterpret_cast<v2l*>(dst)[c] =
+            v2l{static_cast<long long>(o[2 * c]), static_cast<long long>(o[2 * c + 1])};
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++)
+    if (i < valid) dst[i] = o[i];
+}
+
 namespace {  // kernels local to each translation unit
 
 __device__ __forceinline__ int64_t block_exclusive_scan(int64_t v, int64_t* lds, int64_t* total) {
@@ -50,11 +122,13 @@ __global__ __launch_bounds__(kScanThreads) void scan_block_sums(const TC* __rest
                                                                 int64_t* __restrict__ sums) {
   __shared__ int64_t lds[17];
   const int64_t b0 = static_cast<int64_t>(blockIdx.x) * kScanTile;
+  // Each thread sums kScanItems consecutive counts (the same runs as
+  // scan_write_ptr; a block sum does not depend on the order).
+  int64_t v[kScanItems];
+  load_counts(cnt, b0 + static_cast<int64_t>(threadIdx.x) * kScanItems, n, v);
   int64_t s = 0;
-  for (int i = 0; i < kScanItems; i++) {
-    const int64_t q = b0 + static_cast<int64_t>(i) * kScanThreads + threadIdx.x;
-    if (q < n) s += cnt[q];
-  }
+#pragma unroll
+  for (int i = 0; i < kScanItems; i++) s += v[i];
   int64_t tot;
   block_exclusive_scan(s, lds, &tot);
   if (threadIdx.x == 0) sums[blockIdx.x] = tot;
@@ -87,20 +161,24 @@ __global__ __launch_bounds__(kScanThreads) void scan_write_ptr(const TC* __restr
   // Each thread owns kScanItems consecutive entries.
   const int64_t q0 = b0 + static_cast<int64_t>(threadIdx.x) * kScanItems;
   int64_t v[kScanItems];
+  load_counts(cnt, q0, n, v);
   int64_t s = 0;
 #pragma unroll
-  for (int i = 0; i < kScanItems; i++) {
-    v[i] = (q0 + i < n) ? static_cast<int64_t>(cnt[q0 + i]) : 0;
-    s += v[i];
-  }
+  for (int i = 0; i < kScanItems; i++) s += v[i];
   int64_t tot;
   int64_t run = block_exclusive_scan(s, lds, &tot) + sums[blockIdx.x];
+  // ptr[q] = base + (counts before q) for q in [q0, q0 + kScanItems): the
+  // thread's slots start at q0, so whole aligned runs go out as vectors; the
+  // thread holding the last count also writes ptr[n].
+  TP o[kScanItems];
 #pragma unroll
   for (int i = 0; i < kScanItems; i++) {
+    o[i] = static_cast<TP>(base + run);
     run += v[i];
-    if (q0 + i < n) ptr[q0 + i + 1] = static_cast<TP>(base + run);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) ptr[0] = static_cast<TP>(base);
+  if (q0 < n) store_run(ptr + q0, static_cast<int>(n - q0 < kScanItems ? n - q0 : kScanItems), o);
+  if (q0 < n && q0 + kScanItems >= n) ptr[n] = static_cast<TP>(base + run);
+  if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) ptr[0] = static_cast<TP>(base);
 }
 
 // Writes ptr (n+1 entries) from cnt (n entries) and returns the total in
