@@ -329,12 +329,27 @@ __global__ __launch_bounds__(256) void coord_check_kernel(const TI* __restrict__
 }
 
 // head[p] = 1 where sorted entry p starts a run of equal (outer, inner).
+// head[p] = 1 where a run of equal (outer, inner) coordinates starts: one
+// byte per entry, four entries per thread stored as one 32-bit word.
 __global__ __launch_bounds__(256) void run_heads_kernel(const uint32_t* __restrict__ ko,
                                                         const uint32_t* __restrict__ ki, int64_t n,
-                                                        int64_t* __restrict__ head) {
-  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
-    head[p] = (p == 0 || ko[p] != ko[p - 1] || ki[p] != ki[p - 1]) ? 1 : 0;
+                                                        uint8_t* __restrict__ head) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x * 4;
+  const bool aligned = (reinterpret_cast<uintptr_t>(head) & 3) == 0;
+  for (int64_t p0 = (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) * 4; p0 < n;
+       p0 += stride) {
+    uint32_t w = 0;
+    for (int i = 0; i < 4 && p0 + i < n; i++) {
+      const int64_t p = p0 + i;
+      const bool h = p == 0 || ko[p] != ko[p - 1] || ki[p] != ki[p - 1];
+      w |= (h ? 1u : 0u) << (8 * i);
+    }
+    if (p0 + 4 <= n && aligned) {
+      *reinterpret_cast<uint32_t*>(head + p0) = w;
+    } else {
+      for (int i = 0; i < 4 && p0 + i < n; i++) head[p0 + i] = static_cast<uint8_t>((w >> (8 * i)) & 1u);
+    }
+  }
 }
 
 template <typename T>
@@ -349,11 +364,11 @@ __device__ __forceinline__ int32_t combine_add<int32_t>(int32_t a, int32_t b) {
 template <typename TI, typename TV>
 __global__ __launch_bounds__(256) void combine_runs_kernel(
     const uint32_t* __restrict__ ko, const uint32_t* __restrict__ ki, const TV* __restrict__ v,
-    const int64_t* __restrict__ uidx, int64_t n, int64_t base, uint32_t* __restrict__ ou,
+    const int32_t* __restrict__ uidx, int64_t n, int64_t base, uint32_t* __restrict__ ou,
     TI* __restrict__ c_idx, TV* __restrict__ c_val) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride) {
-    const int64_t u = uidx[p];
+    const int32_t u = uidx[p];
     if (uidx[p + 1] == u) continue;  // not a head
     TV acc = v[p];
     for (int64_t s = p + 1; s < n && uidx[s + 1] == uidx[s]; s++) acc = combine_add(acc, v[s]);
@@ -582,8 +597,9 @@ int coo_typed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t n, int64_t
   uint32_t* ki = c.take<uint32_t>(n);  // inner, in (outer, inner) order
   TV* v2 = c.take<TV>(n);
   uint32_t* ou = c.take<uint32_t>(n);  // outer of each distinct coordinate
-  int64_t* head = c.take<int64_t>(n);
-  int64_t* uidx = c.take<int64_t>(n + 1);
+  // One-byte run heads and an int32 run index (nnz is below 2^31 here).
+  uint8_t* head = c.take<uint8_t>(n);
+  int32_t* uidx = c.take<int32_t>(n + 1);
   void* work = c.take<char>(static_cast<int64_t>(scan_work_bytes(n)));
   int* flag = c.take<int>(1);
   SortBufs<TV> S;
@@ -607,10 +623,11 @@ int coo_typed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t n, int64_t
                                             k1, q1, 0, v1, s)));
   FGX_TRY((radix_sort<uint32_t, uint32_t, TV, uint32_t>(q1, 0, k1, 0, v1, n, key_bits(n_outer), S,
                                                          ko, ki, 0, v2, s)));
-  hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, ko, ki, n, head);
+  hipLaunchKernelGGL(run_heads_kernel, dim3(grid_for(n, 256 * 4)), dim3(256), 0, s, ko, ki, n,
+                     head);
   FGX_TRY(launch_check("run_heads_kernel"));
   int64_t u = 0;
-  FGX_TRY(scan_counts_to_ptr<int64_t>(head, n, 0, uidx, work, ctx->host_i64, &u, s));
+  FGX_TRY((scan_counts_to_ptr<int32_t, uint8_t>(head, n, 0, uidx, work, ctx->host_i64, &u, s)));
   hipLaunchKernelGGL((combine_runs_kernel<TI, TV>), dim3(grid_for(n, 256)), dim3(256), 0, s, ko, ki,
                      v2, uidx, n, base, ou, c_idx, c_val);
   FGX_TRY(launch_check("combine_runs_kernel"));
