@@ -242,3 +242,35 @@ def test_spgemm_min_plus_julia_min(dev, dtype):
             assert np.isnan(got[0])
         else:
             assert got[0] == v[0] and np.signbit(got[0]) == np.signbit(v[0])
+
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_symbolic_ptr_into_unaligned_buffer(dev, itype):
+    # The symbolic passes write ptr with 16-byte vector stores where the
+    # caller's buffer allows; a ptr that starts one element into an allocation
+    # (a view, as Julia's or torch's slicing can hand over) takes the scalar
+    # path.  Both must give the same ptr and nnz.
+    from finchgpu import _lib as L
+    from finchgpu.ops import _ptr, _stream, context
+    import ctypes
+    m = n = 3000
+    (ap, ai, av), (bp, bi, bv) = _pair(9, m, n, 5.0, 9.0, np.float32)
+    A = operand(ap, ai, av, m, n, dev, itype)
+    B = operand(bp, bi, bv, m, n, dev, itype)
+    lib = L.load()
+    ctx = context(dev).handle
+    da, db = A.desc(), B.desc()
+    for call, args in (("fgx_ewise_symbolic", (L.OP_ADD,)), ("fgx_spgemm_symbolic", (L.PLUS_TIMES,))):
+        got = []
+        for offset in (0, 1):
+            big = torch.full((m + 1 + offset,), -7, dtype=itype, device=dev)
+            cptr = big[offset:]
+            nnz = ctypes.c_int64()
+            L.check(getattr(lib, call)(ctx, *args, ctypes.byref(da), ctypes.byref(db), _ptr(cptr),
+                                       ctypes.byref(nnz), _stream(dev)), call)
+            torch.cuda.synchronize()
+            if offset:
+                assert int(big[0]) == -7                  # nothing written before the view
+            got.append((cptr.cpu().numpy().copy(), nnz.value))
+        assert np.array_equal(got[0][0], got[1][0]) and got[0][1] == got[1][1]
+        assert got[0][0][0] == 1 and got[0][0][-1] == 1 + got[0][1]
