@@ -274,3 +274,37 @@ def test_symbolic_ptr_into_unaligned_buffer(dev, itype):
             got.append((cptr.cpu().numpy().copy(), nnz.value))
         assert np.array_equal(got[0][0], got[1][0]) and got[0][1] == got[1][1]
         assert got[0][0][0] == 1 and got[0][0][-1] == 1 + got[0][1]
+
+
+def _bin_edges(seed, dtype, values):
+    """A rows built on B rows of known lengths (1, 2, 0 and 4 entries) so that
+    their product counts sit on every bin edge: 256 / 257 (wave sort of 4
+    keys per lane / of 8), 512 / 513 (of 8 / block merge), 1536 / 1537 (block
+    merge / dense window), and rows whose lists point at empty B rows, where
+    the list count (300, 512, 513) picks the bin with no products at all."""
+    rng = np.random.default_rng(seed)
+    n = 20_000
+    blen = np.concatenate([np.full(1000, 1), np.full(1000, 2), np.zeros(1000, int), np.full(1000, 4)])
+    l = blen.size
+    bp = np.concatenate([[0], np.cumsum(blen)]) + 1
+    bi = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in blen if k]) + 1
+    one, two, empty, four = (np.arange(1000) + 1000 * g for g in range(4))
+    pick = lambda pool, k: list(rng.choice(pool, k, replace=False))
+    rows = [pick(one, 256), pick(one, 257), pick(two, 256), pick(two, 256) + pick(one, 1),
+            pick(empty, 300), pick(empty, 512), pick(empty, 513), pick(four, 384),
+            pick(four, 384) + pick(one, 1), pick(empty, 200) + pick(one, 256), [], pick(two, 3)]
+    ap = np.concatenate([[0], np.cumsum([len(r) for r in rows])]) + 1
+    ai = np.concatenate([np.sort(np.array(r, dtype=np.int64)) for r in rows if r]) + 1
+    draw = (lambda k: rng.integers(1, 10, k)) if values == "int19" else rng.standard_normal
+    return (ap, ai, draw(ai.size).astype(dtype), len(rows), l, bp, bi,
+            draw(bi.size).astype(dtype), n)
+
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_spgemm_bin_edges_exact(dev, itype):
+    ap, ai, av, m, l, bp, bi, bv, n = _bin_edges(5, np.int32, "int19")
+    prods = [int(np.diff(bp)[ai[ap[r] - 1: ap[r + 1] - 1] - 1].sum()) for r in range(m)]
+    assert prods[:4] == [256, 257, 512, 513] and prods[7:9] == [1536, 1537]
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n, itype=itype)
+    ap, ai, av, m, l, bp, bi, bv, n = _bin_edges(6, np.float32, "randn")
+    _gemm_check(dev, "min_plus", ap, ai, av, m, l, bp, bi, bv, n, itype=itype)
