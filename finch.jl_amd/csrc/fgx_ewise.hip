@@ -155,7 +155,10 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
   const int64_t c0 = pos_at(cptr, base, row);
   const int64_t la = a1 - a0, lb = b1 - b0;
   const bool uni = op != FGX_OP_MUL;
-  if (!uni && (la == 0 || lb == 0)) return;
+  // Intersection: the symbolic pass's ptr already says which rows have no
+  // common coordinate (almost all, for sparse random patterns); they write
+  // nothing and skip the staging and the searches.
+  if (!uni && (la == 0 || lb == 0 || pos_at(cptr, base, row + 1) == c0)) return;
   // Union: the values of the group's first NPRE steps on each side are
   // loaded together with the coordinates (one round trip); each step then
   // loads the value NPRE steps ahead, so the stores never wait on a fresh
