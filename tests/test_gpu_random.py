@@ -5,7 +5,7 @@ dense-operand case 5,000 to 50,000), a row-length
 mix (Poisson rows, empty rows and a few long rows that span many tiles), an
 index width (Int32 / Int64), an index base (0 / 1) and a value type, and runs
 SpMV (CSR and CSC readings), SpMM at an odd and a wide width, SDDMM,
-coiteration, SpGEMM, transpose and dropfills on the same buffers.  The bars
+coiteration, SpGEMM, transpose, COO assembly and dropfills on the same buffers.  The bars
 are the ones the backend states: FP64 relative 1e-12, FP32 normwise 1e-5,
 and bit-exact structure, integer, min-plus, coiteration and conversion
 results.
@@ -120,6 +120,21 @@ def test_sparse_output_kernels(dev, seed):
     dp, di, dvv = oracle.dropfills(m, ptr, idx, dv, 0.0, base=base)
     assert np.array_equal(D.ptr.cpu().numpy(), dp) and np.array_equal(D.idx.cpu().numpy(), di), case
     assert np.array_equal(D.val.cpu().numpy(), dvv), case
+    # COO assembly of the same entries plus ~10 % duplicates, shuffled: sorted
+    # fibres, duplicates folded left to right in input order, bit-exact.
+    rows = np.repeat(np.arange(m, dtype=np.int64), np.diff(ptr)) + base
+    dup = rng.random(idx.size) < 0.1
+    co = np.concatenate([rows, rows[dup]])
+    ci = np.concatenate([idx, idx[dup]])
+    cv = np.concatenate([val, rng.standard_normal(int(dup.sum())).astype(vdtype)])
+    perm = rng.permutation(co.size)
+    co, ci, cv = co[perm], ci[perm], cv[perm]
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(dt).to(dev)
+    K = ops.coo_to_compressed(t(co, itype), t(ci, itype), t(cv, torch.from_numpy(cv).dtype), m, n,
+                              base)
+    kp, ki, kv = oracle.coo_to_compressed(m, n, co, ci, cv, base=base)
+    assert np.array_equal(K.ptr.cpu().numpy(), kp) and np.array_equal(K.idx.cpu().numpy(), ki), case
+    assert np.array_equal(K.val.cpu().numpy(), kv), case
     # SpGEMM A (m x n) * B' (n x n2): Int32 plus-times and min-plus, bit-exact.
     n2 = int(rng.integers(1, 3000))
     gp, gi = random_csr(rng, n, n2, base)
