@@ -96,24 +96,44 @@ __device__ __forceinline__ int64_t encode_carry(bool carries, bool continues, in
   return carries ? (continues ? -(r + 2) : r) : -1;
 }
 
-// A fibre window known to be cached in LDS: win[t] = pos(r0 + t + 1).
-struct LdsWindow {
-  const int64_t* win;
-  int64_t r0;
-  int64_t n;
-  __device__ __forceinline__ int64_t end_of(int64_t r) const { return win[r - r0]; }
-  __device__ __forceinline__ int64_t find(int64_t target) const {
-    int lo = 0, hi = static_cast<int>(n);
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (win[mid] <= target) lo = mid + 1; else hi = mid;
+// The end of an SpMV tile walk: a segmented inclusive scan of the runs'
+// carries (a run that finalised a row starts a new segment) completes each
+// run's held-back first row; the last lane leaves the tile's carry.
+// `start_hi` = pos(r_hi) on the last lane, -1 when r_hi == r_lo.
+template <typename TA>
+__device__ __forceinline__ void spmv_runs_scan(TA acc, bool has_emit, int64_t first_row,
+                                               TA first_val, TA* __restrict__ y, int64_t m,
+                                               int64_t q0, int64_t qend, int64_t tile,
+                                               int64_t r_lo, int64_t r_hi, int64_t pos_lo,
+                                               int64_t start_hi, int64_t* carry_row_out,
+                                               TA* carry_val_out) {
+  const int lane = lane_id();
+  TA v = acc;
+  bool f = has_emit;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    TA v_up = shfl_up(v, d);
+    int f_up = shfl_up(static_cast<int>(f), d);
+    if (lane >= d) {
+      if (!f) v = v_up + v;
+      f = f || f_up;
     }
-    return r0 + lo;
   }
-};
+  TA carry_in = shfl_up(v, 1);
+  if (lane == 0) carry_in = 0;
+  if (has_emit) y[first_row] = first_val + carry_in;
 
-// The walk of an SpMV tile over its fibre window (LdsWindow or FiberWindow):
-// finalise the rows that end in each lane's run, zero the tile's empty rows,
+  if (lane == kWave - 1) {
+    // Does the continuing row r_hi hold positions of this tile?
+    const bool carries = r_hi < m && qend > q0 && start_hi < qend;
+    const bool continues = tile > 0 && r_lo == r_hi && pos_lo < q0;
+    *carry_row_out = encode_carry(carries, continues, r_hi);
+    *carry_val_out = carries ? v : TA(0);
+  }
+}
+
+// The walk of an SpMV tile over a fibre window too wide for LDS (read from
+// ptr): finalise the rows that end in each lane's run, zero the tile's empty rows,
 // complete held-back rows by a segmented scan, and leave the carry.
 template <typename TA, int V, typename Win, typename TV>
 __device__ __forceinline__ void spmv_tile_walk(const Win win, int64_t m, int64_t q0, int64_t qend,
@@ -167,34 +187,120 @@ __device__ __forceinline__ void spmv_tile_walk(const Win win, int64_t m, int64_t
     const int64_t prev = rr == 0 ? 0 : win.end_of(rr - 1);
     if (win.end_of(rr) == prev) y[rr] = TA(0);
   }
-  // Segmented inclusive scan of the runs' carries; a run with an emit starts
-  // a new segment.
-  TA v = acc;
-  bool f = has_emit;
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    TA v_up = shfl_up(v, d);
-    int f_up = shfl_up(static_cast<int>(f), d);
-    if (lane >= d) {
-      if (!f) v = v_up + v;
-      f = f || f_up;
-    }
-  }
-  TA carry_in = shfl_up(v, 1);
-  if (lane == 0) carry_in = 0;
-  if (has_emit) y[first_row] = first_val + carry_in;
+  // pos(r_hi): where the continuing row starts (-1 when the tile lies in it).
+  const int64_t start_hi =
+      (lane == kWave - 1 && r_hi < m && qend > q0 && r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
+  spmv_runs_scan(acc, has_emit, first_row, first_val, y, m, q0, qend, tile, r_lo, r_hi, pos_lo,
+                 start_hi, carry_row_out, carry_val_out);
+}
 
-  if (lane == kWave - 1) {
-    // Does the continuing row r_hi hold positions of this tile?
-    bool carries = false;
-    if (r_hi < m && qend > q0) {
-      const int64_t start_hi = (r_hi > r_lo) ? win.end_of(r_hi - 1) : -1;
-      carries = start_hi < qend;
+// The non-empty fibres of an LDS-cached window, packed in place over it:
+// pk[c] = (end - q0) << 32 | (r - r0) for the c-th non-empty fibre r, ends
+// clamped to W + 1 (they are only compared with positions of the tile).
+// Empty fibres hold no position, so a run's walk never visits them.  The
+// entries keep the window's type, so every access to the buffer (including
+// the next tile's window, in the batched loop) is ordered as may-alias.
+struct PackedRows {
+  const int64_t* pk;
+  int nc;
+  int64_t r0;
+  __device__ __forceinline__ int rel_end(int c) const { return static_cast<int>(pk[c] >> 32); }
+  __device__ __forceinline__ int64_t row(int c) const { return r0 + (pk[c] & 0xFFFFFFFF); }
+  // First c with rel_end(c) > t (nc if none).
+  __device__ __forceinline__ int find(int t) const {
+    int lo = 0, hi = nc;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (rel_end(mid) <= t) lo = mid + 1; else hi = mid;
     }
-    const bool continues = tile > 0 && r_lo == r_hi && pos_lo < q0;
-    *carry_row_out = encode_carry(carries, continues, r_hi);
-    *carry_val_out = carries ? v : TA(0);
+    return lo;
   }
+};
+
+// Packs the window `lds` (n <= kWinCap ends of fibres r0 .. r0+n-1) of the
+// tile starting at q0 into PackedRows, and zeroes y over the empty fibres r0+t
+// with t in [t_zero0, t_zero1) (the empty rows the tile owns).  pos_lo =
+// pos(r0).  Returns in *start_hi the end of window entry t_hi (-1 if t_hi < 0),
+// read before the packing overwrites it.
+template <int W, typename TA>
+__device__ __forceinline__ PackedRows pack_window(int64_t* lds, int64_t n, int64_t r0, int64_t q0,
+                                                  int64_t pos_lo, int64_t t_zero0,
+                                                  int64_t t_zero1, TA* __restrict__ y,
+                                                  int64_t t_hi, int64_t* start_hi) {
+  constexpr int R = kWinCap / kWave;
+  const int lane = lane_id();
+  const uint64_t below = (uint64_t{1} << lane) - 1;
+  int64_t pkv[R];
+  int slot[R];
+  int nc = 0;
+  // Read every entry (and its predecessor) before any packed write.
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    slot[i] = -1;
+    pkv[i] = 0;
+    if (i * kWave < n) {
+      const int t = i * kWave + lane;
+      const bool in = t < n;
+      const int64_t e = in ? lds[t] : 0;
+      const int64_t p = in ? (t == 0 ? pos_lo : lds[t - 1]) : 0;
+      const bool keep = in && e > p;
+      if (in && !keep && t >= t_zero0 && t < t_zero1) y[r0 + t] = TA(0);
+      const uint64_t bal = ballot(keep);
+      const int64_t rel = e - q0 < W + 1 ? e - q0 : W + 1;
+      pkv[i] = static_cast<int64_t>((static_cast<uint64_t>(rel) << 32) | static_cast<uint32_t>(t));
+      slot[i] = keep ? nc + popc(bal & below) : -1;
+      nc += popc(bal);
+    }
+  }
+  *start_hi = t_hi >= 0 ? lds[t_hi] : -1;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int i = 0; i < R; i++)
+    if (slot[i] >= 0) lds[slot[i]] = pkv[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return PackedRows{lds, nc, r0};
+}
+
+// The walk of an SpMV tile over its packed non-empty fibres: each lane
+// finalises the rows that end in its run (a, b] (in tile-relative positions),
+// and every row it visits holds positions, so each visit stores a result.
+template <typename TA, int V, typename TV>
+__device__ __forceinline__ void spmv_tile_walk_packed(
+    const PackedRows rows, int64_t m, int64_t q0, int64_t qend, int64_t a, int64_t b, bool active,
+    const TV (&vv)[V], const TA (&xg)[V], TA* __restrict__ y, int64_t tile, int64_t r_lo,
+    int64_t r_hi, int64_t pos_lo, int64_t start_hi, int64_t* carry_row_out, TA* carry_val_out) {
+  TA acc = 0;
+  bool has_emit = false;
+  int64_t first_row = -1;
+  TA first_val = 0;
+  const int ra = static_cast<int>(a - q0), rb = static_cast<int>(b - q0);
+  int c = active ? rows.find(ra) : rows.nc;
+  // Finalise row c: the first row a run finalises is held back for the scan.
+  auto finish = [&]() {
+    const int64_t r = rows.row(c);
+    if (has_emit) y[r] = acc;
+    first_row = has_emit ? first_row : r;
+    first_val = has_emit ? first_val : acc;
+    has_emit = true;
+    acc = 0;
+    c++;
+  };
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < V; s++) {
+      if (ra + s < rb) {
+        while (c < rows.nc && rows.rel_end(c) <= ra + s) finish();
+        acc += widen<TA>(vv[s]) * xg[s];
+      }
+    }
+    while (c < rows.nc && rows.rel_end(c) <= rb) finish();
+  }
+  spmv_runs_scan(acc, has_emit, first_row, first_val, y, m, q0, qend, tile, r_lo, r_hi, pos_lo,
+                 start_hi, carry_row_out, carry_val_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -253,12 +359,20 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
   int64_t r_lo, r_hi, pos_lo;
   FiberWindow<TP> win =
       tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo);
-  // The window is almost always cached in LDS; that case runs with an
-  // LDS-only accessor, so the walk's loops carry no global fallback.
-  if (win.cached)
-    spmv_tile_walk<TA, V>(LdsWindow{win.win, win.r0, win.n}, m, q0, qend, a, b, active, vv, xg,
-                          y, tile, r_lo, r_hi, pos_lo, carry_row_out, carry_val_out);
-  else
+  // The window is almost always cached in LDS.  That case packs it to the
+  // non-empty rows (zeroing the empty ones the tile owns on the way), so a
+  // run's walk steps only over rows that hold positions: on power-law
+  // matrices most rows are empty, and a lane crossing a stretch of them would
+  // hold its wave back one LDS read per row.
+  if (win.cached) {
+    const int64_t t_hi = (r_hi < m && qend > q0 && r_hi > r_lo) ? r_hi - 1 - r_lo : -1;
+    const int64_t t_zero1 = (r_hi < m ? r_hi : m) - r_lo;
+    int64_t start_hi;
+    const PackedRows rows = pack_window<W, TA>(lds_win, win.n, r_lo, q0, pos_lo,
+                                               tile > 0 ? 1 : 0, t_zero1, y, t_hi, &start_hi);
+    spmv_tile_walk_packed<TA, V>(rows, m, q0, qend, a, b, active, vv, xg, y, tile, r_lo, r_hi,
+                                 pos_lo, start_hi, carry_row_out, carry_val_out);
+  } else
     spmv_tile_walk<TA, V>(win, m, q0, qend, a, b, active, vv, xg, y, tile, r_lo, r_hi, pos_lo,
                           carry_row_out, carry_val_out);
 }

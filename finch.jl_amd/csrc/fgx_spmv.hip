@@ -10,8 +10,7 @@ namespace fgx {
 
 // Positions per lane run for SpMV (FGX_SPMV_RUN for 4-byte and smaller
 // values; half of it for FP64).  On the 1M-row Zipf(1.5) matrix (16M
-// positions, FP32): 2 -> 132 us, 4 -> 116.5 us, 8 -> 123.7 us, 12 -> 131 us,
-// 16 -> 142 us.
+// positions, FP32), per call: 2 -> 118.1 us, 4 -> 107.4 us, 8 -> 112.8 us.
 #ifndef FGX_SPMV_RUN
 #define FGX_SPMV_RUN 4
 #endif
