@@ -227,32 +227,38 @@ __device__ __forceinline__ PackedRows pack_window(int64_t* lds, int64_t n, int64
                                                   int64_t pos_lo, int64_t t_zero0,
                                                   int64_t t_zero1, TA* __restrict__ y,
                                                   int64_t t_hi, int64_t* start_hi) {
+  constexpr int R = kWinCap / kWave;
   const int lane = lane_id();
   const uint64_t below = (uint64_t{1} << lane) - 1;
-  *start_hi = t_hi >= 0 ? lds[t_hi] : -1;
-  // One round per 64 entries.  A round's packed writes go to slots below its
-  // own last entry (and are ordered after its reads by the ballot they depend
-  // on), so later rounds still read unpacked entries; the entry before each
-  // round's first is carried in a register, not re-read.
+  int64_t pkv[R];
+  int slot[R];
   int nc = 0;
-  int64_t prev_last = pos_lo;
-  for (int64_t t0 = 0; t0 < n; t0 += kWave) {
-    const int64_t t = t0 + lane;
-    const bool in = t < n;
-    const int64_t e = in ? lds[t] : 0;
-    int64_t p = shfl_up(e, 1);
-    if (lane == 0) p = prev_last;
-    prev_last = shfl(e, kWave - 1);
-    const bool keep = in && e > p;
-    if (in && !keep && t >= t_zero0 && t < t_zero1) y[r0 + t] = TA(0);
-    const uint64_t bal = ballot(keep);
-    if (keep) {
+  // Read every entry (and its predecessor) before any packed write.
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    slot[i] = -1;
+    pkv[i] = 0;
+    if (i * kWave < n) {
+      const int t = i * kWave + lane;
+      const bool in = t < n;
+      const int64_t e = in ? lds[t] : 0;
+      const int64_t p = in ? (t == 0 ? pos_lo : lds[t - 1]) : 0;
+      const bool keep = in && e > p;
+      if (in && !keep && t >= t_zero0 && t < t_zero1) y[r0 + t] = TA(0);
+      const uint64_t bal = ballot(keep);
       const int64_t rel = e - q0 < W + 1 ? e - q0 : W + 1;
-      lds[nc + popc(bal & below)] =
-          static_cast<int64_t>((static_cast<uint64_t>(rel) << 32) | static_cast<uint32_t>(t));
+      pkv[i] = static_cast<int64_t>((static_cast<uint64_t>(rel) << 32) | static_cast<uint32_t>(t));
+      slot[i] = keep ? nc + popc(bal & below) : -1;
+      nc += popc(bal);
     }
-    nc += popc(bal);
   }
+  *start_hi = t_hi >= 0 ? lds[t_hi] : -1;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int i = 0; i < R; i++)
+    if (slot[i] >= 0) lds[slot[i]] = pkv[i];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
