@@ -19,13 +19,12 @@ int set_error(int code, const char* fmt, ...) {
 
 void clear_error() { g_last_error[0] = '\0'; }
 
-// Grows one of the context's device arenas to at least `bytes`.  The old
+// Grows one of a stream's device arenas to at least `bytes`.  The old
 // arena may still be in use by queued work, so the device is drained first.
-static int grow_arena(fgx_ctx* ctx, void** buf, size_t* cur, size_t bytes, void** out) {
+static int grow_arena(void** buf, size_t* cur, size_t bytes, void** out) {
   if (bytes == 0) bytes = 256;
   if (bytes > *cur) {
     if (*buf) {
-      FGX_HIP(hipStreamSynchronize(ctx->stream));
       FGX_HIP(hipDeviceSynchronize());
       FGX_HIP(hipFree(*buf));
       *buf = nullptr;
@@ -44,12 +43,58 @@ static int grow_arena(fgx_ctx* ctx, void** buf, size_t* cur, size_t bytes, void*
   return FGX_OK;
 }
 
-int scratch(fgx_ctx* ctx, size_t bytes, void** out) {
-  return grow_arena(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes, out);
+// Streams with arenas of their own; beyond this the least recently used
+// stream's arenas are freed (after draining the device) and reused.
+constexpr size_t kMaxArenas = 16;
+
+static void release_arena(fgx_arena* a) {
+  if (a->scratch) (void)hipFree(a->scratch);
+  if (a->dscratch) (void)hipFree(a->dscratch);
+  *a = fgx_arena{};
 }
 
-int dense_scratch(fgx_ctx* ctx, size_t bytes, void** out) {
-  return grow_arena(ctx, &ctx->dscratch, &ctx->dscratch_bytes, bytes, out);
+static int arena_of(fgx_ctx* ctx, hipStream_t s, fgx_arena** out) {
+  for (auto& a : ctx->arenas) {
+    if (a.stream == s) {
+      a.stamp = ++ctx->clock;
+      *out = &a;
+      return FGX_OK;
+    }
+  }
+  if (ctx->arenas.size() >= kMaxArenas) {
+    size_t oldest = 0;
+    for (size_t i = 1; i < ctx->arenas.size(); i++)
+      if (ctx->arenas[i].stamp < ctx->arenas[oldest].stamp) oldest = i;
+    // Queued work on the evicted stream may still read its arenas.
+    FGX_HIP(hipDeviceSynchronize());
+    release_arena(&ctx->arenas[oldest]);
+    ctx->arenas.erase(ctx->arenas.begin() + static_cast<long>(oldest));
+  }
+  fgx_arena a;
+  a.stream = s;
+  a.stamp = ++ctx->clock;
+  ctx->arenas.push_back(a);
+  *out = &ctx->arenas.back();
+  return FGX_OK;
+}
+
+int scratch(fgx_ctx* ctx, hipStream_t s, size_t bytes, void** out) {
+  fgx_arena* a;
+  FGX_TRY(arena_of(ctx, s, &a));
+  return grow_arena(&a->scratch, &a->scratch_bytes, bytes, out);
+}
+
+int dense_scratch(fgx_ctx* ctx, hipStream_t s, size_t bytes, void** out) {
+  fgx_arena* a;
+  FGX_TRY(arena_of(ctx, s, &a));
+  return grow_arena(&a->dscratch, &a->dscratch_bytes, bytes, out);
+}
+
+void free_arenas(fgx_ctx* ctx) {
+  if (ctx->arenas.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& a : ctx->arenas) release_arena(&a);
+  ctx->arenas.clear();
 }
 
 // Enough for a serving-sized operand set (hundreds of operands, a few tile
@@ -60,7 +105,10 @@ constexpr size_t kMaxTables = 1024;
 void free_tile_tables(fgx_ctx* ctx) {
   if (ctx->tables.empty()) return;
   (void)hipDeviceSynchronize();
-  for (auto& t : ctx->tables) (void)hipFree(t.d_table);
+  for (auto& t : ctx->tables) {
+    (void)hipFree(t.d_table);
+    if (t.ready) (void)hipEventDestroy(t.ready);
+  }
   ctx->tables.clear();
 }
 
@@ -100,6 +148,16 @@ int tile_table(fgx_ctx* ctx, const fgx_compressed* A, int64_t W, hipStream_t s,
     if (t.ptr == A->ptr && t.n_outer == A->n_outer && t.nnz == A->nnz && t.W == W &&
         t.ptr_type == A->ptr_type && t.base == A->index_base) {
       t.stamp = ++ctx->clock;
+      if (!t.ready_seen && t.built_on != s) {
+        // Built on another stream: order this stream after the build.
+        const hipError_t q = hipEventQuery(t.ready);
+        if (q == hipSuccess) {
+          t.ready_seen = true;
+        } else {
+          if (q != hipErrorNotReady) (void)hipGetLastError();
+          FGX_HIP(hipStreamWaitEvent(s, t.ready, 0));
+        }
+      }
       *out = t.d_table;
       return FGX_OK;
     }
@@ -110,12 +168,24 @@ int tile_table(fgx_ctx* ctx, const fgx_compressed* A, int64_t W, hipStream_t s,
       if (ctx->tables[i].stamp < ctx->tables[oldest].stamp) oldest = i;
     FGX_HIP(hipDeviceSynchronize());
     (void)hipFree(ctx->tables[oldest].d_table);
+    if (ctx->tables[oldest].ready) (void)hipEventDestroy(ctx->tables[oldest].ready);
     ctx->tables.erase(ctx->tables.begin() + static_cast<long>(oldest));
   }
   fgx_tile_table t{A->ptr, A->n_outer, A->nnz, W, A->ptr_type, A->index_base, nullptr,
-                   ++ctx->clock};
-  FGX_TRY(build_tile_table(A, W, s, &t.d_table));
-  ctx->tables.push_back(t);
+                   ++ctx->clock, s, nullptr, false};
+  FGX_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+  const int rc = build_tile_table(A, W, s, &t.d_table);
+  if (rc == FGX_OK && hipEventRecord(t.ready, s) == hipSuccess) {
+    ctx->tables.push_back(t);
+  } else {
+    (void)hipGetLastError();
+    (void)hipEventDestroy(t.ready);
+    if (t.d_table) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(t.d_table);
+    }
+    return rc != FGX_OK ? rc : set_error(FGX_EHIP, "tile table event record failed");
+  }
   *out = t.d_table;
   return FGX_OK;
 }
@@ -224,8 +294,7 @@ int fgx_ctx_destroy(fgx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   free_tile_tables(ctx);
-  if (ctx->scratch) (void)hipFree(ctx->scratch);
-  if (ctx->dscratch) (void)hipFree(ctx->dscratch);
+  free_arenas(ctx);
   if (ctx->host_i64) (void)hipHostFree(ctx->host_i64);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;

@@ -17,12 +17,39 @@
 
 namespace fgx {
 
-template <typename TV>
-__device__ __forceinline__ TV ew_max(TV a, TV b) { return a < b ? b : a; }
+// Julia's `max` (base/math.jl): for floats, a NaN on either side gives NaN
+// and -0.0 orders below 0.0.  Julia returns `x - y` when a NaN is present,
+// which on x86-64 is the first NaN operand with its quiet bit set; the same
+// bits are returned here.
+__device__ __forceinline__ float ew_max(float x, float y) {
+  if (x != x) return __uint_as_float(__float_as_uint(x) | 0x00400000u);
+  if (y != y) return __uint_as_float(__float_as_uint(y) | 0x00400000u);
+  return (y > x || (y == x && !__builtin_signbit(y) && __builtin_signbit(x))) ? y : x;
+}
+__device__ __forceinline__ double ew_max(double x, double y) {
+  if (x != x)
+    return __longlong_as_double(__double_as_longlong(x) | 0x0008000000000000ll);
+  if (y != y)
+    return __longlong_as_double(__double_as_longlong(y) | 0x0008000000000000ll);
+  return (y > x || (y == x && !__builtin_signbit(y) && __builtin_signbit(x))) ? y : x;
+}
+__device__ __forceinline__ int32_t ew_max(int32_t x, int32_t y) { return y > x ? y : x; }
+
+// `+` and `*` on Int32 wrap modulo 2^32, as Julia's Int32 arithmetic does.
+__device__ __forceinline__ float ew_add(float a, float b) { return a + b; }
+__device__ __forceinline__ double ew_add(double a, double b) { return a + b; }
+__device__ __forceinline__ int32_t ew_add(int32_t a, int32_t b) {
+  return static_cast<int32_t>(static_cast<uint32_t>(a) + static_cast<uint32_t>(b));
+}
+__device__ __forceinline__ float ew_mul(float a, float b) { return a * b; }
+__device__ __forceinline__ double ew_mul(double a, double b) { return a * b; }
+__device__ __forceinline__ int32_t ew_mul(int32_t a, int32_t b) {
+  return static_cast<int32_t>(static_cast<uint32_t>(a) * static_cast<uint32_t>(b));
+}
 
 template <typename TV>
 __device__ __forceinline__ TV ew_both(int op, TV a, TV b) {
-  return op == FGX_OP_MUL ? a * b : (op == FGX_OP_ADD ? b + a : ew_max(a, b));
+  return op == FGX_OP_MUL ? ew_mul(a, b) : (op == FGX_OP_ADD ? ew_add(b, a) : ew_max(a, b));
 }
 template <typename TV>
 __device__ __forceinline__ TV ew_only_a(int op, TV a) { return op == FGX_OP_ADD ? a : ew_max(a, TV(0)); }
@@ -305,9 +332,11 @@ int fgx_ewise_symbolic(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_ewise_symbolic"));
+  FGX_TRY(admit_operand(ctx, B, s, "fgx_ewise_symbolic"));
   const int64_t m = A->n_outer;
   void* scr;
-  FGX_TRY(scratch(ctx, static_cast<size_t>(m + 1) * sizeof(int64_t) + scan_work_bytes(m), &scr));
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(m + 1) * sizeof(int64_t) + scan_work_bytes(m), &scr));
   int64_t* cnt = static_cast<int64_t*>(scr);
   void* work = cnt + m + 1;
   const int G = pick_group(m, A->nnz, B->nnz);
@@ -339,6 +368,8 @@ int fgx_ewise_numeric(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_c
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_ewise_numeric"));
+  FGX_TRY(admit_operand(ctx, B, s, "fgx_ewise_numeric"));
   const int64_t m = A->n_outer;
   if (m == 0) return FGX_OK;
   const int G = pick_group(m, A->nnz, B->nnz);

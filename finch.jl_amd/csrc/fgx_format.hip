@@ -558,7 +558,7 @@ int transpose_typed(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_
   if (n == 0) return write_ptr_from_keys<TP>(nullptr, 0, A->n_inner, base, static_cast<TP*>(t_ptr), s);
   const size_t bytes = sort_scratch_bytes<TW>(n) + 2 * (static_cast<size_t>(n) * 4 + 256);
   void* scr;
-  FGX_TRY(scratch(ctx, bytes, &scr));
+  FGX_TRY(scratch(ctx, s, bytes, &scr));
   Carver c{static_cast<char*>(scr)};
   uint32_t* fib = c.take<uint32_t>(n);
   uint32_t* kfin = c.take<uint32_t>(n);
@@ -588,7 +588,7 @@ int coo_typed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t n, int64_t
   const size_t bytes = sort_scratch_bytes<TV>(n) + 5 * n4 + 2 * nv +
                        (2 * static_cast<size_t>(n) + 1) * 8 + 512 + scan_work_bytes(n) + 1024;
   void* scr;
-  FGX_TRY(scratch(ctx, bytes, &scr));
+  FGX_TRY(scratch(ctx, s, bytes, &scr));
   Carver c{static_cast<char*>(scr)};
   uint32_t* k1 = c.take<uint32_t>(n);  // inner, after the sort by inner
   uint32_t* q1 = c.take<uint32_t>(n);  // outer, after the sort by inner
@@ -650,7 +650,7 @@ int dropfills_typed(fgx_ctx* ctx, const fgx_compressed* A, TV fill, void* c_ptr,
   auto run = [&](auto kept_tag) -> int {
     using TK = decltype(kept_tag);
     void* scr;
-    FGX_TRY(scratch(ctx, static_cast<size_t>(n) + 256 + (static_cast<size_t>(n) + 1) * sizeof(TK) +
+    FGX_TRY(scratch(ctx, s, static_cast<size_t>(n) + 256 + (static_cast<size_t>(n) + 1) * sizeof(TK) +
                              256 + scan_work_bytes(n) + 256,
                     &scr));
     Carver c{static_cast<char*>(scr)};
@@ -702,6 +702,7 @@ int fgx_transpose(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_id
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_transpose"));
   int rc = FGX_OK;
   FGX_DISPATCH_IDX(ic, {
     if (vb == 2) rc = transpose_typed<TP, TI, 2>(ctx, A, t_ptr, t_idx, t_val, s);
@@ -763,6 +764,7 @@ int fgx_dropfills(fgx_ctx* ctx, const fgx_compressed* A, double fill, void* c_pt
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_dropfills"));
   int rc = FGX_OK;
   switch (A->val_type) {
     case FGX_F64:

@@ -15,26 +15,46 @@
 // Cached tile table of one operand (see tile_setup): keyed by the ptr
 // buffer, its shape and the tile width.  Tables are hints verified by every
 // tile, so a ptr rewritten in place is still handled correctly.
+// A table built on one stream is used from another only after that stream
+// waits for `ready` (recorded after the build); `ready_seen` is set once the
+// event has completed, after which no wait is needed.
 struct fgx_tile_table {
   const void* ptr;
   int64_t n_outer, nnz, W;
   int32_t ptr_type, base;
   int64_t* d_table;
   uint64_t stamp;
+  hipStream_t built_on;
+  hipEvent_t ready;
+  bool ready_seen;
+};
+
+// Device scratch of one stream: calls on different streams of a context run
+// concurrently on the device, so each stream gets its own arenas.  Work on
+// one stream is ordered, which makes reuse across its calls safe.
+struct fgx_arena {
+  hipStream_t stream = nullptr;
+  // Carries, counters, bins, scan workspace.
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // Dense operands relaid out by a call (column-major dense operands
+  // transposed to k-fast rows).
+  void* dscratch = nullptr;
+  size_t dscratch_bytes = 0;
+  uint64_t stamp = 0;
 };
 
 struct fgx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
-  // Growable device scratch arena (carries, counters, bins).  One arena per
-  // context; calls on one context are serialised by `mu`.
-  void* scratch = nullptr;
-  size_t scratch_bytes = 0;
-  // Second arena for dense operands relaid out by a call (column-major
-  // dense operands transposed to k-fast rows).
-  void* dscratch = nullptr;
-  size_t dscratch_bytes = 0;
+  // Growable device scratch, one set of arenas per stream that has called
+  // into this context (least recently used evicted beyond kMaxArenas).  Host
+  // state is serialised by `mu`.
+  std::vector<fgx_arena> arenas;
+  // Safe mode (fgx_ctx_set_safe): validate every compressed operand before
+  // a kernel reads it.
+  bool safe = false;
   // Small pinned host buffer for data-dependent sizes (symbolic nnz).
   int64_t* host_i64 = nullptr;
   std::vector<fgx_tile_table> tables;
@@ -62,9 +82,11 @@ void clear_error();
     if (s_ != FGX_OK) return s_;      \
   } while (0)
 
-// Scratch of at least `bytes` (grows, never shrinks).  Caller holds ctx->mu.
-int scratch(fgx_ctx* ctx, size_t bytes, void** out);
-int dense_scratch(fgx_ctx* ctx, size_t bytes, void** out);
+// Scratch of at least `bytes` for work on stream `s` (grows, never
+// shrinks).  Caller holds ctx->mu.
+int scratch(fgx_ctx* ctx, hipStream_t s, size_t bytes, void** out);
+int dense_scratch(fgx_ctx* ctx, hipStream_t s, size_t bytes, void** out);
+void free_arenas(fgx_ctx* ctx);
 
 // The stream a call runs on: the caller's hipStream_t; NULL is the HIP null
 // (default) stream, as in rocBLAS, so calls order with the caller's other
@@ -88,6 +110,11 @@ int fill_tile_table(const fgx_compressed* A, int64_t W, hipStream_t s, int64_t* 
 
 // Checks the descriptor (non-null buffers, supported type combination).
 int check_compressed(const fgx_compressed* A, const char* name);
+
+// Device check of A's SparseList invariants (synchronises s): FGX_EPROTO or
+// FGX_EDIM on a violation.  admit_operand runs it only in safe mode.
+int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name);
+int admit_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name);
 
 // Index-type combinations the kernels are instantiated for.
 enum IdxCombo { IDX_32_32 = 0, IDX_64_64 = 1, IDX_64_32 = 2 };

@@ -71,7 +71,7 @@ static int sddmm_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const v
                             ? 0 : ((static_cast<size_t>(m * k) * sizeof(TA) + 255) & ~size_t(255));
       const size_t yb = y_layout == FGX_ROW_MAJOR ? 0 : static_cast<size_t>(n * k) * sizeof(TA);
       void* arena = nullptr;
-      FGX_TRY(dense_scratch(ctx, xb + yb, &arena));
+      FGX_TRY(dense_scratch(ctx, s, xb + yb, &arena));
       const TA* Xf = X;
       const TA* Yf = Y;
       int64_t ldxf = ldx, ldyf = ldy;
@@ -144,6 +144,7 @@ extern "C" int fgx_sddmm_csr(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, c
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_sddmm_csr"));
   int rc = FGX_OK;
   switch (A->val_type) {
     case FGX_F64: { using TV = double; FGX_DISPATCH_IDX(ic, rc = (sddmm_typed<TP, TI, TV>(ctx, A, k, X, ldx, x_layout, Y, ldy, y_layout, out_val, s))); break; }

@@ -827,10 +827,12 @@ int fgx_spgemm_symbolic(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spgemm_symbolic"));
+  FGX_TRY(admit_operand(ctx, B, s, "fgx_spgemm_symbolic"));
   const int64_t m = A->n_outer;
   void* scr;
   const size_t need = static_cast<size_t>(4 + 4 * m + m + 1) * sizeof(int64_t) + scan_work_bytes(m);
-  FGX_TRY(scratch(ctx, need, &scr));
+  FGX_TRY(scratch(ctx, s, need, &scr));
   int64_t* binmem = static_cast<int64_t*>(scr);
   int64_t* cnt = binmem + 4 + 4 * m;
   void* work = cnt + m + 1;
@@ -860,9 +862,11 @@ int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spgemm_numeric"));
+  FGX_TRY(admit_operand(ctx, B, s, "fgx_spgemm_numeric"));
   const int64_t m = A->n_outer;
   void* scr;
-  FGX_TRY(scratch(ctx, static_cast<size_t>(4 + 4 * m) * sizeof(int64_t), &scr));
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(4 + 4 * m) * sizeof(int64_t), &scr));
   int64_t* binmem = static_cast<int64_t*>(scr);
 #define FGX_GEMM_NUM(SR_, TV_)                                                            \
   FGX_DISPATCH_IDX(ic, {                                                                  \

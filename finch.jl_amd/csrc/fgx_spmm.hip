@@ -27,6 +27,7 @@ int fgx_spmm_csr(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const void* B
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spmm_csr"));
   switch (A->val_type) {
     case FGX_F64: return spmm_csr_f64(ctx, A, ic, k, B, ldb, b_layout, C, ldc, c_layout, s);
     case FGX_F32: return spmm_csr_f32(ctx, A, ic, k, B, ldb, b_layout, C, ldc, c_layout, s);

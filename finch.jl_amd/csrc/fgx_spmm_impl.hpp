@@ -69,7 +69,7 @@ static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t
   const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
   const int width = G * KV;
   void* scr;
-  FGX_TRY(scratch(ctx, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA) * width), &scr));
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA) * width), &scr));
   int64_t* crow = static_cast<int64_t*>(scr);
   TA* cval = reinterpret_cast<TA*>(crow + ntiles);
   const int64_t* table = nullptr;
@@ -102,7 +102,7 @@ static int spmm_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, cons
     const size_t bt = b_layout == FGX_ROW_MAJOR ? 0 : ((static_cast<size_t>(n * k) * sizeof(TA) + 255) & ~size_t(255));
     const size_t ct = c_layout == FGX_ROW_MAJOR ? 0 : static_cast<size_t>(m * k) * sizeof(TA);
     void* arena = nullptr;
-    FGX_TRY(dense_scratch(ctx, bt + ct, &arena));
+    FGX_TRY(dense_scratch(ctx, s, bt + ct, &arena));
     const TA* Bf = B;
     int64_t ldbf = ldb;
     if (bt) {

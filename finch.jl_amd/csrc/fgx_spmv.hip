@@ -100,7 +100,7 @@ static int spmv_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, 
   if (m == 0) return FGX_OK;
   const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
   void* scr;
-  FGX_TRY(scratch(ctx, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA)), &scr));
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA)), &scr));
   int64_t* crow = static_cast<int64_t*>(scr);
   TA* cval = reinterpret_cast<TA*>(crow + ntiles);
   const int vec = aligned16(A->val) && aligned16(A->idx);
@@ -160,6 +160,7 @@ int fgx_spmv_csr(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spmv_csr"));
   int rc = FGX_OK;
   FGX_DISPATCH_VAL(A->val_type, FGX_DISPATCH_IDX(ic, rc = (spmv_csr_typed<TP, TI, TV>(ctx, A, x, y, s))));
   return rc;
@@ -176,6 +177,7 @@ int fgx_spmv_csc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, 
   std::lock_guard<std::mutex> lk(ctx->mu);
   FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spmv_csc"));
   int rc = FGX_OK;
   FGX_DISPATCH_VAL(A->val_type, FGX_DISPATCH_IDX(ic, rc = (spmv_csc_typed<TP, TI, TV>(ctx, A, x, y, s))));
   return rc;

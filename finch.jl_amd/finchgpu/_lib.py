@@ -117,6 +117,8 @@ SIGNATURES = {
     "fgx_sync": (_I, [_P, _P]),
     "fgx_last_error": (ctypes.c_char_p, []),
     "fgx_plan_cache_clear": (_I, [_P]),
+    "fgx_validate": (_I, [_P, _PC, _P]),
+    "fgx_ctx_set_safe": (_I, [_P, _I]),
     "fgx_spmv_csr": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csc": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P]),

@@ -70,6 +70,13 @@ def context(device: int | torch.device | None = None) -> Context:
     return ctx
 
 
+def set_safe_mode(safe: bool = True, device: int | torch.device | None = None):
+    """Finch's mode=:safe for this device's context: every entry point checks
+    each compressed operand (fgx_validate) before a kernel reads it."""
+    L.check(L.load().fgx_ctx_set_safe(context(device).handle, int(bool(safe))),
+            "fgx_ctx_set_safe")
+
+
 def _stream(device) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
@@ -120,6 +127,14 @@ class Compressed:
     def to_values(self, dtype: torch.dtype) -> "Compressed":
         """Same pattern with values converted on the device (e.g. BF16 storage)."""
         return self.with_values(convert(self.val, dtype))
+
+
+def validate(A: Compressed) -> None:
+    """Checks A against the SparseList invariants on the device (fgx_validate):
+    raises FinchProtocolError for a malformed ptr or unsorted fibre and
+    DimensionMismatch for coordinates outside the shape or nnz != ptr[end]-base."""
+    L.check(L.load().fgx_validate(context(A.device).handle, ctypes.byref(A.desc()),
+                                  _stream(A.device)), "fgx_validate")
 
 
 def _check_dense(t: torch.Tensor, dt: torch.dtype, name: str):

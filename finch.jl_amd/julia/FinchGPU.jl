@@ -92,6 +92,18 @@ abi_version() = ccall((:fgx_version, libfgx), Cint, ())
 an operand's `ptr` was freed and its address reused, to release memory early."""
 clear_plans!(dev::GPU) = check(ccall((:fgx_plan_cache_clear, libfgx), Cint, (Ptr{Cvoid},), dev.ctx))
 
+"""Check a Dense(SparseList(Element)) operand against the SparseList invariants
+on the device (fgx_validate): `FinchProtocolError` for a malformed `ptr` or an
+unsorted fibre, `DimensionMismatch` for coordinates outside the shape."""
+validate(dev::GPU, A) = check(ccall((:fgx_validate, libfgx), Cint,
+                                    (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}),
+                                    dev.ctx, descriptor(A), C_NULL))
+
+"""Finch's `mode=:safe` for every later call through `dev`: each compressed
+operand is validated before a kernel reads it (`safe=false`: trusted, as `:fast`)."""
+set_safe!(dev::GPU, safe::Bool) =
+    check(ccall((:fgx_ctx_set_safe, libfgx), Cint, (Ptr{Cvoid}, Cint), dev.ctx, safe))
+
 """A device vector: the storage of a level buffer on the GPU (hipMalloc'd)."""
 mutable struct GPUVector{T} <: AbstractVector{T}
     ptr::Ptr{T}

@@ -103,7 +103,8 @@ typedef struct fgx_ctx fgx_ctx;
 int fgx_version(void);
 
 /* Create a context on HIP device `device` (one process per GPU).  The context
- * owns a stream, a scratch arena and the per-matrix plan cache.  Stands in for
+ * owns a stream, device scratch arenas (one set per calling stream, so calls
+ * on different streams may run concurrently) and the per-matrix plan cache.  Stands in for
  * a `GPU <: AbstractDevice` instance (src/architecture.jl:6-64,170-173). */
 int fgx_ctx_create(int device, fgx_ctx** out);
 int fgx_ctx_destroy(fgx_ctx* ctx);
@@ -116,6 +117,22 @@ int fgx_sync(fgx_ctx* ctx, void* stream);
 const char* fgx_last_error(void);
 /* Release cached per-matrix plans (nnz tile tables, autotune picks). */
 int fgx_plan_cache_clear(fgx_ctx* ctx);
+
+/* Structural check of a compressed operand against the SparseList level's
+ * invariants (src/tensors/levels/sparse_list_levels.jl:37-42): ptr[0] == base,
+ * ptr non-decreasing and inside [base, base+nnz], ptr[n_outer]-base == nnz,
+ * every idx in [base, base+n_inner), and idx strictly increasing inside each
+ * fibre.  Returns FGX_EPROTO (FinchProtocolError) when ptr or the order is
+ * broken and FGX_EDIM (DimensionMismatch) when a coordinate lies outside the
+ * shape or nnz disagrees with ptr.  Runs two device passes and synchronises
+ * `stream`.  Finch's counterpart is the checking done in mode=:safe
+ * (src/execute.jl:1-11; sparse_list_levels.jl:538-547). */
+int fgx_validate(fgx_ctx* ctx, const fgx_compressed* A, void* stream);
+/* safe != 0: every entry point validates each compressed operand it is given
+ * (fgx_validate) before a kernel reads it, as Finch's mode=:safe checks its
+ * programs; safe == 0 (the default) trusts the operands, as mode=:fast does.
+ * Applies to calls made through this context from then on. */
+int fgx_ctx_set_safe(fgx_ctx* ctx, int safe);
 
 /* ---- SpMV / SpMM / SDDMM ------------------------------------------------ */
 

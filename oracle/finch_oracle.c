@@ -168,12 +168,43 @@ DEF_SDDMM(float, f32)
  *               coordinates stored in both survive: value a*b;
  *   op 1 (ADD): union; both -> b + a (golden line 63 writes y + x), one-sided
  *               -> the stored value (a + 0 == a);
- *   op 2 (MAX): union; both -> max(a, b), one-sided -> max(v, 0) (fill 0).
+ *   op 2 (MAX): union; both -> max(a, b), one-sided -> max(a, 0) for an
+ *               A-only entry and max(0, b) for a B-only one (the missing side
+ *               reads as the fill 0, src/interface/eager.jl:28-30).
+ * `max` is Julia's Base.max (base/math.jl): for floats a NaN on either side
+ * gives NaN, and -0.0 orders below 0.0.  Julia returns `x - y` when a NaN is
+ * present; on x86-64 that is the first NaN operand with its quiet bit set,
+ * which JL_MAX returns bit for bit.  Int32 `+` and `*` wrap modulo 2^32 as
+ * Julia's Int32 does (computed in uint32_t, no signed overflow).
  * Returns nnz(C); cptr gets n_outer+1 entries in `base`; cidx/cval need room
  * for nnz(A)+nnz(B).
  * ------------------------------------------------------------------------- */
+static inline float JL_MAX_f32(float x, float y) {
+  uint32_t b;
+  if (isnan(x)) { memcpy(&b, &x, 4); b |= 0x00400000u; memcpy(&x, &b, 4); return x; }
+  if (isnan(y)) { memcpy(&b, &y, 4); b |= 0x00400000u; memcpy(&y, &b, 4); return y; }
+  if (y > x || (y == x && !signbit(y) && signbit(x))) return y;
+  return x;
+}
+static inline double JL_MAX_f64(double x, double y) {
+  uint64_t b;
+  if (isnan(x)) { memcpy(&b, &x, 8); b |= 0x0008000000000000ull; memcpy(&x, &b, 8); return x; }
+  if (isnan(y)) { memcpy(&b, &y, 8); b |= 0x0008000000000000ull; memcpy(&y, &b, 8); return y; }
+  if (y > x || (y == x && !signbit(y) && signbit(x))) return y;
+  return x;
+}
+static inline int32_t JL_MAX_i32(int32_t x, int32_t y) { return y > x ? y : x; }
+static inline float EW_ADD_f32(float a, float b) { return a + b; }
+static inline double EW_ADD_f64(double a, double b) { return a + b; }
+static inline int32_t EW_ADD_i32(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a + (uint32_t)b);
+}
+static inline float EW_MUL_f32(float a, float b) { return a * b; }
+static inline double EW_MUL_f64(double a, double b) { return a * b; }
+static inline int32_t EW_MUL_i32(int32_t a, int32_t b) {
+  return (int32_t)((uint32_t)a * (uint32_t)b);
+}
 #define DEF_EWISE(T, SFX)                                                                 \
-  static inline T ew_max_##SFX(T a, T b) { return a < b ? b : a; }                        \
   i64 or_ewise_##SFX(int op, i64 n_outer, const i64* aptr, const i64* aidx, const T* aval,\
                      const i64* bptr, const i64* bidx, const T* bval, int base, i64* cptr,\
                      i64* cidx, T* cval) {                                                \
@@ -188,19 +219,20 @@ DEF_SDDMM(float, f32)
         if (ia == i && ib == i) {                                                         \
           T a = aval[qa], b = bval[qb];                                                   \
           cidx[qos] = i;                                                                  \
-          cval[qos] = op == 0 ? a * b : (op == 1 ? b + a : ew_max_##SFX(a, b));           \
+          cval[qos] = op == 0 ? EW_MUL_##SFX(a, b)                                           \
+                      : (op == 1 ? EW_ADD_##SFX(b, a) : JL_MAX_##SFX(a, b));             \
           qos++; qa++; qb++;                                                              \
         } else if (ia == i) {                                                             \
           if (op != 0) {                                                                  \
             cidx[qos] = i;                                                                \
-            cval[qos] = op == 1 ? aval[qa] : ew_max_##SFX(aval[qa], (T)0);                \
+            cval[qos] = op == 1 ? aval[qa] : JL_MAX_##SFX(aval[qa], (T)0);                \
             qos++;                                                                        \
           }                                                                               \
           qa++;                                                                           \
         } else {                                                                          \
           if (op != 0) {                                                                  \
             cidx[qos] = i;                                                                \
-            cval[qos] = op == 1 ? bval[qb] : ew_max_##SFX((T)0, bval[qb]);                \
+            cval[qos] = op == 1 ? bval[qb] : JL_MAX_##SFX((T)0, bval[qb]);                \
             qos++;                                                                        \
           }                                                                               \
           qb++;                                                                           \
@@ -209,11 +241,11 @@ DEF_SDDMM(float, f32)
       if (op != 0) {                                                                      \
         for (; qa < qa_stop; qa++, qos++) {                                               \
           cidx[qos] = aidx[qa];                                                           \
-          cval[qos] = op == 1 ? aval[qa] : ew_max_##SFX(aval[qa], (T)0);                  \
+          cval[qos] = op == 1 ? aval[qa] : JL_MAX_##SFX(aval[qa], (T)0);                  \
         }                                                                                 \
         for (; qb < qb_stop; qb++, qos++) {                                               \
           cidx[qos] = bidx[qb];                                                           \
-          cval[qos] = op == 1 ? bval[qb] : ew_max_##SFX((T)0, bval[qb]);                  \
+          cval[qos] = op == 1 ? bval[qb] : JL_MAX_##SFX((T)0, bval[qb]);                  \
         }                                                                                 \
       }                                                                                   \
       cptr[p + 1] = qos + base;                                                           \

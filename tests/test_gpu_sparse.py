@@ -88,6 +88,60 @@ def test_ewise_edges(dev):
             assert np.array_equal(cval, rval)
 
 
+_SPECIALS = [1.0, -1.0, 0.0, -0.0, float("nan"), float("inf"), -float("inf"), 2.5, -2.5]
+
+
+def _bits(v):
+    return v.view(np.uint32 if v.dtype.itemsize == 4 else np.uint64)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_ewise_max_special_values_bit_exact(dev, dtype, itype):
+    # Julia's max on NaN, +-0 and +-Inf at both-sided and one-sided positions,
+    # including signalling and negative NaN payloads: bits must match the oracle.
+    vals = np.array(_SPECIALS, dtype)
+    if dtype == np.float32:
+        sn = np.array([0x7FA00001, 0xFFC00002], np.uint32).view(np.float32)
+        vals = np.concatenate([vals, sn])
+    k = vals.size
+    # Row 0: every pair (x, y) both stored.  Row 1: A-only coordinates (odd)
+    # and B-only coordinates (even), each value once on each side.
+    xs, ys = np.meshgrid(vals, vals, indexing="ij")
+    xs, ys = xs.ravel(), ys.ravel()
+    n = max(xs.size, 2 * k)
+    ai = np.concatenate([np.arange(1, xs.size + 1), np.arange(1, 2 * k, 2)])
+    bi = np.concatenate([np.arange(1, ys.size + 1), np.arange(2, 2 * k + 1, 2)])
+    ap = np.array([1, xs.size + 1, xs.size + k + 1])
+    bp = np.array([1, ys.size + 1, ys.size + k + 1])
+    av = np.concatenate([xs, vals]).astype(dtype)
+    bv = np.concatenate([ys, vals]).astype(dtype)
+    A = operand(ap, ai, av, 2, n, dev, itype)
+    B = operand(bp, bi, bv, 2, n, dev, itype)
+    cptr, cidx, cval = host(ops.ewise("max", A, B))
+    rptr, ridx, rval = oracle.ewise("max", ap, ai, av, bp, bi, bv)
+    assert np.array_equal(cptr, rptr) and np.array_equal(cidx, ridx)
+    assert np.array_equal(_bits(cval), _bits(rval))
+    # Spot checks of Julia's definition on the device result itself.
+    both = cval[:xs.size]
+    j = np.flatnonzero((xs == 0) & (ys == 0) & (np.signbit(xs) != np.signbit(ys)))
+    assert j.size == 2 and not np.signbit(both[j]).any()          # max(-0.0, 0.0) == 0.0
+    assert np.isnan(both[np.isnan(xs) | np.isnan(ys)]).all()       # NaN from either side
+
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+def test_ewise_int32_wraps_bit_exact(dev, itype):
+    big = np.array([2**31 - 1, 65536, 46341, -(2**31), 7], np.int32)
+    two = np.array([1, 65536, 46341, -1, -9], np.int32)
+    idx = np.arange(1, 6)
+    A = operand([1, 6], idx, big, 1, 5, dev, itype)
+    B = operand([1, 6], idx, two, 1, 5, dev, itype)
+    for op in ("add", "mul", "max"):
+        _, _, got = host(ops.ewise(op, A, B))
+        _, _, want = oracle.ewise(op, [1, 6], idx, big, [1, 6], idx, two)
+        assert got.tolist() == want.tolist(), op
+
+
 def _rows_csr(lengths, n, rng, dtype):
     """1-based CSR with the given row lengths, distinct sorted columns."""
     ptr = np.concatenate([[1], 1 + np.cumsum(lengths)]).astype(np.int64)

@@ -147,6 +147,84 @@ def test_ewise_max_stores_zero_from_negative_one_sided():
     assert cptr.tolist() == [1, 2] and cidx.tolist() == [3] and cval.tolist() == [0.0]
 
 
+# Julia's Base.max on floats (base/math.jl): a NaN on either side gives NaN,
+# -0.0 orders below 0.0, otherwise the larger value.  Expected values are
+# written from that definition, not computed by the oracle's formula.
+_NAN = float("nan")
+_INF = float("inf")
+JULIA_MAX_TABLE = [
+    # (x, y, max(x, y))
+    (1.0, 2.0, 2.0), (2.0, 1.0, 2.0), (-3.0, -5.0, -3.0),
+    (1.0, _NAN, _NAN), (_NAN, 1.0, _NAN), (_NAN, _NAN, _NAN),
+    (_INF, _NAN, _NAN), (_NAN, -_INF, _NAN),
+    (-0.0, 0.0, 0.0), (0.0, -0.0, 0.0), (-0.0, -0.0, -0.0), (0.0, 0.0, 0.0),
+    (-_INF, -0.0, -0.0), (-_INF, _INF, _INF), (_INF, -_INF, _INF), (-_INF, -_INF, -_INF),
+    (-1.0, -0.0, -0.0), (-0.0, -1.0, -0.0),
+]
+
+
+def _same(a, b):
+    """Equal including the sign of zero; NaN equals NaN."""
+    if np.isnan(a) or np.isnan(b):
+        return bool(np.isnan(a) and np.isnan(b))
+    return a == b and np.signbit(a) == np.signbit(b)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ewise_max_is_julia_max_both_sided(dtype):
+    # Both sides stored at every coordinate: C[i] = max(A[i], B[i]).
+    k = len(JULIA_MAX_TABLE)
+    x = np.array([t[0] for t in JULIA_MAX_TABLE], dtype)
+    y = np.array([t[1] for t in JULIA_MAX_TABLE], dtype)
+    idx = np.arange(1, k + 1)
+    cptr, cidx, cval = oracle.ewise("max", [1, k + 1], idx, x, [1, k + 1], idx, y)
+    assert cptr.tolist() == [1, k + 1] and cidx.tolist() == idx.tolist()
+    for (a, b, want), got in zip(JULIA_MAX_TABLE, cval):
+        assert _same(got, dtype(want)), (a, b, got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ewise_max_is_julia_max_one_sided(dtype):
+    # A missing side reads as the fill 0: an A-only entry a gives max(a, 0)
+    # and a B-only entry b gives max(0, b).
+    vals = [-0.0, 0.0, _NAN, -_INF, _INF, -2.0, 3.0]
+    want = [0.0, 0.0, _NAN, 0.0, _INF, 0.0, 3.0]
+    k = len(vals)
+    a_idx = np.arange(1, 2 * k, 2)          # odd coordinates: A only
+    b_idx = np.arange(2, 2 * k + 1, 2)      # even coordinates: B only
+    v = np.array(vals, dtype)
+    cptr, cidx, cval = oracle.ewise("max", [1, k + 1], a_idx, v, [1, k + 1], b_idx, v)
+    assert cidx.tolist() == list(range(1, 2 * k + 1))
+    for j, w in enumerate(want):
+        assert _same(cval[2 * j], dtype(w)), ("A-only", vals[j], cval[2 * j])
+        assert _same(cval[2 * j + 1], dtype(w)), ("B-only", vals[j], cval[2 * j + 1])
+
+
+def test_ewise_max_nan_bits_follow_x_minus_y():
+    # Julia returns `x - y` when a NaN is present; on x86-64 that is the first
+    # NaN operand with its quiet bit set.  Signalling payloads come back quiet.
+    sn = np.array([0x7FA00001, 0xFFC00002], np.uint32).view(np.float32)   # sNaN, -qNaN
+    x = np.array([sn[0], 1.0, sn[1], sn[0]], np.float32)
+    y = np.array([1.0, sn[1], sn[0], np.float32(2.0)], np.float32)
+    idx = np.arange(1, 5)
+    _, _, c = oracle.ewise("max", [1, 5], idx, x, [1, 5], idx, y)
+    got = c.view(np.uint32).tolist()
+    assert got == [0x7FE00001, 0xFFC00002, 0xFFC00002, 0x7FE00001]
+
+
+def test_ewise_int32_wraps():
+    # Julia's Int32 `+` and `*` wrap modulo 2^32.
+    big = np.array([2**31 - 1, 65536, 46341, -(2**31)], np.int32)
+    two = np.array([1, 65536, 46341, -1], np.int32)
+    idx = np.arange(1, 5)
+    _, _, s = oracle.ewise("add", [1, 5], idx, big, [1, 5], idx, two)
+    _, _, p = oracle.ewise("mul", [1, 5], idx, big, [1, 5], idx, two)
+    _, _, m = oracle.ewise("max", [1, 5], idx, big, [1, 5], idx, two)
+    assert s.tolist() == [-(2**31), 131072, 92682, 2**31 - 1]
+    assert p.tolist() == [2**31 - 1, 0, 46341 * 46341 - 2**32, -(2**31)]
+    assert m.tolist() == [2**31 - 1, 65536, 46341, -1]
+
+
 # ---- SpGEMM --------------------------------------------------------------------
 
 @pytest.mark.parametrize("name", ["west0132.mtx", "lpi_itest6.mtx", "lp_blend.mtx", "Trec4.mtx"])
