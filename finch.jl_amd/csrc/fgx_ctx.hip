@@ -285,6 +285,15 @@ int fgx_ctx_create(int device, fgx_ctx** out) {
     delete c;
     return set_error(FGX_EHIP, "fgx_ctx_create: pinned host buffer allocation failed");
   }
+  // With `device` current: the no-allocation entry point's scratch address
+  // on this device.
+  const int rc = noalloc_setup(c);
+  if (rc != FGX_OK) {
+    (void)hipHostFree(c->host_i64);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
   *out = c;
   return FGX_OK;
 }

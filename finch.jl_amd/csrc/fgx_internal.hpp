@@ -55,6 +55,9 @@ struct fgx_ctx {
   // Safe mode (fgx_ctx_set_safe): validate every compressed operand before
   // a kernel reads it.
   bool safe = false;
+  // Device address of the no-allocation scratch array on this context's
+  // device (fgx_spmv_csr_noalloc), from hipGetSymbolAddress at creation.
+  void* noalloc_scratch = nullptr;
   // Small pinned host buffer for data-dependent sizes (symbolic nnz).
   int64_t* host_i64 = nullptr;
   std::vector<fgx_tile_table> tables;
@@ -113,8 +116,14 @@ int check_compressed(const fgx_compressed* A, const char* name);
 
 // Device check of A's SparseList invariants (synchronises s): FGX_EPROTO or
 // FGX_EDIM on a violation.  admit_operand runs it only in safe mode.
-int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name);
+// `flags` is a device word to use instead of the stream's scratch arena
+// (callers that must not allocate).
+int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name,
+                     uint32_t* flags = nullptr);
 int admit_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name);
+
+// Looks up the no-allocation scratch address for ctx's (current) device.
+int noalloc_setup(fgx_ctx* ctx);
 
 // Index-type combinations the kernels are instantiated for.
 enum IdxCombo { IDX_32_32 = 0, IDX_64_64 = 1, IDX_64_32 = 2 };

@@ -90,31 +90,101 @@ __global__ __launch_bounds__(256) void spmv_csc_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Tiles of one CSR SpMV call: W positions per wave tile, ntiles tiles.
+template <typename TV>
+static void spmv_csr_shape(const fgx_compressed* A, int64_t* W, int64_t* ntiles) {
+  *W = kWave * spmv_v<TV>::value;
+  *ntiles = A->nnz > 0 ? (A->nnz + *W - 1) / *W : 1;
+}
+
+// The kernel and its fix-up over caller-provided buffers: the tile table
+// (ntiles + 1 entries, already filled) and one carry slot per tile.
 template <typename TP, typename TI, typename TV>
-static int spmv_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
-                          hipStream_t s) {
+static int spmv_csr_launch(const fgx_compressed* A, const void* x, void* y, hipStream_t s,
+                           const int64_t* table, int64_t* crow,
+                           typename acc_of<TV>::type* cval) {
   using TA = typename acc_of<TV>::type;
   constexpr int V = spmv_v<TV>::value;
-  const int64_t W = kWave * V;
-  const int64_t m = A->n_outer;
-  if (m == 0) return FGX_OK;
-  const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
-  void* scr;
-  FGX_TRY(scratch(ctx, s, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA)), &scr));
-  int64_t* crow = static_cast<int64_t*>(scr);
-  TA* cval = reinterpret_cast<TA*>(crow + ntiles);
+  int64_t W, ntiles;
+  spmv_csr_shape<TV>(A, &W, &ntiles);
   const int vec = aligned16(A->val) && aligned16(A->idx);
-  const int64_t* table = nullptr;
-  FGX_TRY(tile_table(ctx, A, W, s, &table));
   const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
   hipLaunchKernelGGL((spmv_csr_kernel<TP, TI, TV, TA, V>), dim3(static_cast<unsigned>(blocks)),
                      dim3(256), 0, s, static_cast<const TP*>(A->ptr),
                      static_cast<const TI*>(A->idx), static_cast<const TV*>(A->val),
-                     static_cast<int64_t>(A->index_base), m, A->nnz, static_cast<const TA*>(x),
-                     static_cast<TA*>(y), ntiles, vec, table, crow, cval);
+                     static_cast<int64_t>(A->index_base), A->n_outer, A->nnz,
+                     static_cast<const TA*>(x), static_cast<TA*>(y), ntiles, vec, table, crow,
+                     cval);
   FGX_TRY(launch_check("spmv_csr_kernel"));
   if (ntiles > 1) FGX_TRY(launch_fixup<TA>(s, crow, cval, ntiles, 1, static_cast<TA*>(y), 1, 0, 1));
   return FGX_OK;
+}
+
+template <typename TP, typename TI, typename TV>
+static int spmv_csr_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                          hipStream_t s) {
+  using TA = typename acc_of<TV>::type;
+  if (A->n_outer == 0) return FGX_OK;
+  int64_t W, ntiles;
+  spmv_csr_shape<TV>(A, &W, &ntiles);
+  void* scr;
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(ntiles) * (sizeof(int64_t) + sizeof(TA)), &scr));
+  int64_t* crow = static_cast<int64_t*>(scr);
+  TA* cval = reinterpret_cast<TA*>(crow + ntiles);
+  const int64_t* table = nullptr;
+  FGX_TRY(tile_table(ctx, A, W, s, &table));
+  return spmv_csr_launch<TP, TI, TV>(A, x, y, s, table, crow, cval);
+}
+
+// ---- CSR SpMV that allocates nothing ----------------------------------------------
+// Device memory for the tile table and the carries of fgx_spmv_csr_noalloc:
+// one array per GPU, held until the process ends.  Host code never uses this
+// name as an address; it uses the device address that noalloc_setup obtains
+// with hipGetSymbolAddress (kept per context, i.e. per device).
+static __device__ __align__(256) unsigned char noalloc_scratch[FGX_NOALLOC_SCRATCH_BYTES];
+
+static size_t round256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
+// Bytes of noalloc_scratch one call needs: table, carry rows, carry values.
+template <typename TV>
+static size_t noalloc_bytes(const fgx_compressed* A) {
+  using TA = typename acc_of<TV>::type;
+  int64_t W, ntiles;
+  spmv_csr_shape<TV>(A, &W, &ntiles);
+  const size_t nt = static_cast<size_t>(ntiles);
+  return round256((nt + 1) * sizeof(int64_t)) + round256(nt * sizeof(int64_t)) +
+         round256(nt * sizeof(TA));
+}
+
+int noalloc_setup(fgx_ctx* ctx) {
+  void* p = nullptr;
+  FGX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+  ctx->noalloc_scratch = p;
+  return FGX_OK;
+}
+
+template <typename TP, typename TI, typename TV>
+static int spmv_csr_noalloc_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                                  hipStream_t s) {
+  using TA = typename acc_of<TV>::type;
+  if (A->n_outer == 0) return FGX_OK;
+  const size_t need = noalloc_bytes<TV>(A);
+  if (need > FGX_NOALLOC_SCRATCH_BYTES - 256)
+    return set_error(FGX_EINVAL,
+                     "fgx_spmv_csr_noalloc: %lld stored entries need %zu bytes of scratch; the "
+                     "limit is %zu (use fgx_spmv_csr)",
+                     static_cast<long long>(A->nnz), need,
+                     static_cast<size_t>(FGX_NOALLOC_SCRATCH_BYTES - 256));
+  int64_t W, ntiles;
+  spmv_csr_shape<TV>(A, &W, &ntiles);
+  char* base = static_cast<char*>(ctx->noalloc_scratch);
+  int64_t* table = reinterpret_cast<int64_t*>(base);
+  base += round256((static_cast<size_t>(ntiles) + 1) * sizeof(int64_t));
+  int64_t* crow = reinterpret_cast<int64_t*>(base);
+  base += round256(static_cast<size_t>(ntiles) * sizeof(int64_t));
+  TA* cval = reinterpret_cast<TA*>(base);
+  FGX_TRY(fill_tile_table(A, W, s, table));
+  return spmv_csr_launch<TP, TI, TV>(A, x, y, s, table, crow, cval);
 }
 
 template <typename TP, typename TI, typename TV>
@@ -163,6 +233,32 @@ int fgx_spmv_csr(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, 
   FGX_TRY(admit_operand(ctx, A, s, "fgx_spmv_csr"));
   int rc = FGX_OK;
   FGX_DISPATCH_VAL(A->val_type, FGX_DISPATCH_IDX(ic, rc = (spmv_csr_typed<TP, TI, TV>(ctx, A, x, y, s))));
+  return rc;
+}
+
+int fgx_spmv_csr_noalloc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                         void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_spmv_csr_noalloc: NULL ctx");
+  FGX_TRY(check_compressed(A, "fgx_spmv_csr_noalloc"));
+  if ((A->n_inner > 0 && !x && A->nnz > 0) || (A->n_outer > 0 && !y))
+    return set_error(FGX_EINVAL, "fgx_spmv_csr_noalloc: NULL x/y");
+  if (!ctx->noalloc_scratch)
+    return set_error(FGX_EHIP, "fgx_spmv_csr_noalloc: scratch address not set up");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  if (ctx->safe) {
+    // The flag word sits past the largest region a call may use.
+    uint32_t* flags = reinterpret_cast<uint32_t*>(static_cast<char*>(ctx->noalloc_scratch) +
+                                                  FGX_NOALLOC_SCRATCH_BYTES - 256);
+    FGX_TRY(validate_operand(ctx, A, s, "fgx_spmv_csr_noalloc", flags));
+  }
+  int rc = FGX_OK;
+  FGX_DISPATCH_VAL(A->val_type,
+                   FGX_DISPATCH_IDX(ic, rc = (spmv_csr_noalloc_typed<TP, TI, TV>(ctx, A, x, y, s))));
   return rc;
 }
 

@@ -85,14 +85,20 @@ static unsigned grid_of(int64_t n) {
   return static_cast<unsigned>(b);
 }
 
-int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name) {
+__global__ void clear_flags_kernel(uint32_t* flags) { *flags = 0u; }
+
+int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name,
+                     uint32_t* flags) {
   FGX_TRY(check_compressed(A, name));
   IdxCombo ic;
   FGX_TRY(idx_combo(A, &ic));
-  void* scr;
-  FGX_TRY(scratch(ctx, s, 256, &scr));
-  uint32_t* flags = static_cast<uint32_t*>(scr);
-  FGX_HIP(hipMemsetAsync(flags, 0, sizeof(uint32_t), s));
+  if (!flags) {
+    void* scr;
+    FGX_TRY(scratch(ctx, s, 256, &scr));
+    flags = static_cast<uint32_t*>(scr);
+  }
+  hipLaunchKernelGGL(clear_flags_kernel, dim3(1), dim3(1), 0, s, flags);
+  FGX_TRY(launch_check("clear_flags_kernel"));
   const int64_t base = A->index_base;
   FGX_DISPATCH_IDX(ic, {
     hipLaunchKernelGGL((validate_ptr_kernel<TP>), dim3(grid_of(A->n_outer + 1)), dim3(256), 0, s,

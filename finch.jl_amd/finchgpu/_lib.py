@@ -22,6 +22,7 @@ OP_MUL, OP_ADD, OP_MAX = 0, 1, 2
 PLUS_TIMES, MIN_PLUS = 0, 1
 REQ_SPMV, REQ_SPMM, REQ_SDDMM = 0, 1, 2
 BATCH_FORCE_ROWSPLIT, BATCH_FORCE_NNZSPLIT, BATCH_NO_GRAPH = 1, 2, 4
+NOALLOC_SCRATCH_BYTES = 16 << 20     # FGX_NOALLOC_SCRATCH_BYTES
 
 
 class FinchGPUError(RuntimeError):
@@ -120,6 +121,7 @@ SIGNATURES = {
     "fgx_validate": (_I, [_P, _PC, _P]),
     "fgx_ctx_set_safe": (_I, [_P, _I]),
     "fgx_spmv_csr": (_I, [_P, _PC, _P, _P, _P]),
+    "fgx_spmv_csr_noalloc": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csc": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P]),
     "fgx_sddmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P, _P]),

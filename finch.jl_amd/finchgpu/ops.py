@@ -165,6 +165,22 @@ def spmv_csr(A: Compressed, x: torch.Tensor, y: torch.Tensor | None = None) -> t
     return y
 
 
+def spmv_csr_noalloc(A: Compressed, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+    """y = A*x like spmv_csr, through the entry point that allocates nothing
+    (fgx_spmv_csr_noalloc): the caller passes y, the tile table and carries
+    live in the library's fixed per-GPU scratch array."""
+    dt = acc_dtype(A.val.dtype)
+    _check_dense(x, dt, "x")
+    _check_dense(y, dt, "y")
+    if x.numel() != A.n_inner or y.numel() != A.n_outer:
+        raise L.DimensionMismatch(L.EDIM, f"x has {x.numel()} / y has {y.numel()} entries for "
+                                          f"A of {A.n_outer} x {A.n_inner}")
+    L.check(L.load().fgx_spmv_csr_noalloc(context(A.device).handle, ctypes.byref(A.desc()),
+                                          _ptr(x), _ptr(y), _stream(A.device)),
+            "fgx_spmv_csr_noalloc")
+    return y
+
+
 def spmv_csc(A: Compressed, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
     """y = A*x with A read column-wise (fgx_spmv_csc)."""
     dt = acc_dtype(A.val.dtype)

@@ -165,6 +165,18 @@ function spmv!(dev::GPU, y::GPUVector, S::SwizzleArray{(2, 1)}, x::GPUVector)
     y
 end
 
+# The same CSR SpMV through the entry point that allocates nothing, for code
+# under HIP graph capture or loops that may not allocate; its scratch is a
+# fixed per-GPU array of FGX_NOALLOC_SCRATCH_BYTES, shared by every call.
+function spmv_noalloc!(dev::GPU, y::GPUVector, S::SwizzleArray{(2, 1)}, x::GPUVector;
+                       stream::Ptr{Cvoid} = C_NULL)
+    d = descriptor(S.body)
+    check(ccall((:fgx_spmv_csr_noalloc, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, d, x.ptr, y.ptr, stream))
+    y
+end
+
 # @einsum C[i, k] += A[i, j] * B[j, k]; B, C column-major (Finch's Dense(Dense)).
 function spmm!(dev::GPU, C::GPUVector, S::SwizzleArray{(2, 1)}, B::GPUVector, k::Integer)
     d = descriptor(S.body)

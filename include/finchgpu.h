@@ -143,6 +143,20 @@ int fgx_ctx_set_safe(fgx_ctx* ctx, int safe);
  * (F64 for F64 values, F32 otherwise). */
 int fgx_spmv_csr(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, void* stream);
 
+/* fgx_spmv_csr for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, kernels and results
+ * (bit for bit) as fgx_spmv_csr; the tile table and carries live in a fixed
+ * device array of FGX_NOALLOC_SCRATCH_BYTES per GPU instead of the context's
+ * arenas and plan cache.  The call allocates nothing; it returns FGX_EINVAL
+ * when A needs more scratch than that (about 200M stored FP32 entries, 85M
+ * FP64).  Every call on one GPU shares the array: calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race and must be
+ * ordered by the caller.  Safe mode synchronises `stream` and cannot be used
+ * under graph capture. */
+#define FGX_NOALLOC_SCRATCH_BYTES (16u << 20)
+int fgx_spmv_csr_noalloc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                         void* stream);
 /* y = A*x with A read column-wise (CSC): y .= 0; for j, i: y[i] += A[i,j]*x[j].
  * Replaces test/reference64/typical/typical_spmv_csc.txt:26-83
  * (program test/suites/typical_tests.jl:47-73).  x has n_outer entries, y has
