@@ -253,6 +253,13 @@ __global__ void convert_kernel(const void* __restrict__ src, void* __restrict__ 
   }
 }
 
+template <typename T>
+__global__ void sequence_kernel(T* __restrict__ out, int64_t n, int64_t start) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) out[i] = static_cast<T>(start + i);
+}
+
 }  // namespace fgx
 
 using namespace fgx;
@@ -325,6 +332,29 @@ int fgx_plan_cache_clear(fgx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   free_tile_tables(ctx);
   return FGX_OK;
+}
+
+int fgx_sequence(fgx_ctx* ctx, int dtype, void* out, int64_t n, int64_t start, void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_sequence: NULL ctx");
+  if (n < 0) return set_error(FGX_EINVAL, "fgx_sequence: negative n");
+  if (n == 0) return FGX_OK;
+  if (!out) return set_error(FGX_EINVAL, "fgx_sequence: NULL out");
+  if (dtype != FGX_I32 && dtype != FGX_I64)
+    return set_error(FGX_EUNSUP, "fgx_sequence: dtype %d is not an index type", dtype);
+  if (dtype == FGX_I32 && (start < INT32_MIN || start + n - 1 > INT32_MAX))
+    return set_error(FGX_EINVAL, "fgx_sequence: values exceed Int32");
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (dtype == FGX_I32)
+    hipLaunchKernelGGL((sequence_kernel<int32_t>), dim3(static_cast<unsigned>(blocks)), dim3(256),
+                       0, s, static_cast<int32_t*>(out), n, start);
+  else
+    hipLaunchKernelGGL((sequence_kernel<int64_t>), dim3(static_cast<unsigned>(blocks)), dim3(256),
+                       0, s, static_cast<int64_t*>(out), n, start);
+  return launch_check("fgx_sequence");
 }
 
 int fgx_convert(fgx_ctx* ctx, const void* src, int src_type, void* dst, int dst_type, int64_t n,

@@ -120,6 +120,7 @@ SIGNATURES = {
     "fgx_plan_cache_clear": (_I, [_P]),
     "fgx_validate": (_I, [_P, _PC, _P]),
     "fgx_ctx_set_safe": (_I, [_P, _I]),
+    "fgx_sequence": (_I, [_P, _I, _P, _I64, _I64, _P]),
     "fgx_spmv_csr": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csr_noalloc": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csc": (_I, [_P, _PC, _P, _P, _P]),

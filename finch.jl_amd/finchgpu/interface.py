@@ -1,37 +1,36 @@
 """Finch programs routed to the backend's kernels.
 
-``einsum(program, **tensors)`` takes the index notation of Finch's ``@einsum``
-(src/interface/einsum.jl:64-82,190-201) and ``@finch`` increments, recognises
-the programs the backend implements, and runs them on the device through the C
-ABI.  Which kernel a program needs follows from the operands' storage, the way
-Finch's lowering follows the level formats:
+Three entry points mirror how Finch reaches its generated loops:
 
-==========================================  ==========================================
-program                                     kernel
-==========================================  ==========================================
-``y[i] += A[i, j] * x[j]``, A CSC           CSC SpMV (typical_spmv_csc.txt:26-83)
-``y[i] += A[i, j] * x[j]``, A swizzled CSR   CSR SpMV (parallel_spmv.txt)
-``C[i, k] += A[i, j] * B[j, k]``, A sparse   SpMM (A swizzled CSR), B/C dense
-``C[i, k] += A[i, j] * B[j, k]``, both sparse  Gustavson SpGEMM, plus-times
-``C[i, k] <<min>>= A[i, j] + B[j, k]``       Gustavson SpGEMM, min-plus (fill Inf)
-``C[i, j] = A[i, j] * B[i, j]`` (``+``, ``max``)  element-wise coiteration
-==========================================  ==========================================
+* ``finch(program, **tensors)`` — ``@finch`` (src/execute.jl:101-126): the
+  output tensor is declared by the caller and keeps its format; a dense output
+  is written in place, a sparse output gets new ``ptr``/``idx``/``val``
+  buffers (its ``resize!`` + freeze).
+* ``einsum(program, **tensors)`` — ``@einsum`` (src/interface/einsum.jl:190),
+  which Finch runs through the ``compute`` scheduler.
+* ``compute(lazy(...))`` — ``compute`` (src/interface/lazy.jl:707-730),
+  including the fused SDDMM plan of ``compute(lazy(M) .* (lazy(X) * lazy(Y)))``
+  (benchmark/benchmarks.jl:141-168).  Sparse results of ``einsum`` and
+  ``compute`` are ``Dense(SparseDict)`` tensors, the format Finch's scheduler
+  chooses (src/interface/traits.jl:509-517), in SparseDict's frozen layout.
 
-When the storage order of an operand disagrees with the program (a CSC
-operand of SpMM, or sparse operands of different orders), the operand is
-transposed on the device first (``fgx_transpose``), as Finch's concordize
-pass inserts a transpose (src/scheduler/optimize.jl:396-452); the transposed
-copy is cached on the tensor.  ``permutedims``, ``fsparse`` and ``dropfills``
-are the device forms of src/interface/copy.jl:64-117 and
-src/interface/fsparse.jl:64-97.
+Which kernel runs is looked up in ``finch.jl_amd/routes.toml``, the routing
+table the Julia glue (``FinchGPU.jl``) reads too: a program is reduced to a
+canonical form (operands renamed, indices renamed, sparse operands written in
+storage order), and that form and the operand kinds select a handler and its
+C entry points.  A program with no route raises ``UnsupportedError``
+(FGX_EUNSUP); nothing falls back to the host.
 
-``compute(lazy(A) * (lazy(X) @ lazy(Y)))`` is the fused SDDMM plan of
-``compute(lazy(M) .* (lazy(A)*lazy(B)))`` (benchmark/benchmarks.jl:141-168).
-Programs outside this set raise ``UnsupportedError``; nothing falls back to
-the host.
+When the storage order of an operand disagrees with the program (a CSC operand
+of SpMM, or sparse operands of different orders), the route transposes it on
+the device first (``fgx_transpose``), as Finch's concordize pass inserts a
+transpose (src/scheduler/optimize.jl:396-452); the transposed copy is cached on
+the tensor.  ``permutedims``, ``fsparse`` and ``dropfills`` are the device forms
+of src/interface/copy.jl:64-117 and src/interface/fsparse.jl:64-97.
 """
 from __future__ import annotations
 
+import os
 import re
 
 import numpy as np
@@ -39,7 +38,10 @@ import torch
 
 from . import _lib as L
 from . import ops
-from .tensors import Dense, Element, SparseList, SwizzleArray, Tensor
+from .tensors import Dense, Element, SparseDict, SparseList, SwizzleArray, Tensor
+
+ROUTES_FILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                           "routes.toml")
 
 _TERM = r"(\w+)\[\s*([\w\s,]*)\]"
 _PROG = re.compile(r"^\s*" + _TERM + r"\s*(\+=|=|<<\s*(\w+)\s*>>=)\s*(.+?)\s*$")
@@ -53,7 +55,7 @@ def _term(s):
 
 
 def parse(program: str):
-    """(out, out_indices, op, reduce_op, [(name, indices)...], combine) of a program."""
+    """(out, out_indices, reduce_op, [(name, indices)...], combine) of a program."""
     m = _PROG.match(program)
     if not m:
         raise L.UnsupportedError(L.EUNSUP, f"cannot parse program {program!r}")
@@ -67,7 +69,7 @@ def parse(program: str):
     else:
         for c in ("*", "+"):
             parts = rhs.split(c)
-            if len(parts) == 2:
+            if len(parts) >= 2:
                 combine = c
                 break
         else:
@@ -78,10 +80,63 @@ def parse(program: str):
     return out, out_idx, reduce_op, terms, combine
 
 
+# ---- the routing table ------------------------------------------------------------------
+
+_ROUTES = None
+
+
+def routes() -> list:
+    """The [[route]] entries of routes.toml."""
+    global _ROUTES
+    if _ROUTES is None:
+        try:
+            import tomllib as toml  # Python 3.11+
+        except ImportError:
+            import tomli as toml
+        with open(ROUTES_FILE, "rb") as f:
+            _ROUTES = toml.load(f)["route"]
+    return _ROUTES
+
+
+def canonical(out_idx, reduce_op, terms, combine, operands):
+    """The canonical program text of routes.toml and the operand kinds.
+
+    `operands` holds, per right-hand term, (kind, reversed): kind "sparse" or
+    "dense", and reversed True when a sparse operand's storage order is the
+    reverse of how the program reads it (swizzle(A, 2, 1))."""
+    letters = "ijklmn"
+    names = {}
+
+    def nm(x):
+        if x not in names:
+            names[x] = letters[len(names)]
+        return names[x]
+
+    out = [nm(x) for x in out_idx]
+    accs = []
+    for (_, idx), (kind, rev) in zip(terms, operands):
+        accs.append([nm(x) for x in (tuple(reversed(idx)) if kind == "sparse" and rev else idx)])
+    tns = ["A", "B", "D", "E"]
+    acc = [f"{tns[n]}[{','.join(a)}]" for n, a in enumerate(accs)]
+    assign = "=" if reduce_op is None else ("+=" if reduce_op == "+" else f"<<{reduce_op}>>=")
+    rhs = f"max({acc[0]}, {acc[1]})" if combine == "max" else f" {combine} ".join(acc)
+    return f"C[{','.join(out)}] {assign} {rhs}", [k for k, _ in operands]
+
+
+def lookup(program: str, kinds) -> dict | None:
+    for r in routes():
+        if r["program"] == program and list(r["operands"]) == list(kinds):
+            return r
+    return None
+
+
 # ---- operand views -----------------------------------------------------------------
 
 def _compressed(t: Tensor) -> ops.Compressed:
     sl = t.lvl.lvl
+    if isinstance(sl, SparseDict) and not sl.in_order:
+        raise L.UnsupportedError(L.EUNSUP, "a SparseDict operand whose values are not in "
+                                           "coordinate order (val != 1:nnz)")
     return ops.Compressed(sl.ptr, sl.idx, t.element.val, t.lvl.shape, sl.shape, 1)
 
 
@@ -96,13 +151,16 @@ def _other_order(t: Tensor) -> ops.Compressed:
     return cached[1]
 
 
+_SPARSE = ("csc", "cscdict")
+
+
 def _kind(x):
     """('csc', Tensor) | ('csr', Tensor) | ('dense', torch) for a device operand."""
-    if isinstance(x, SwizzleArray) and x.dims == (2, 1) and x.body.format == "csc":
+    if isinstance(x, SwizzleArray) and x.dims == (2, 1) and x.body.format in _SPARSE:
         return "csr", x.body
     if isinstance(x, Tensor):
         f = x.format
-        if f == "csc":
+        if f in _SPARSE:
             return "csc", x
         if f in ("dense", "dense2"):
             return "dense", _dense_view(x)
@@ -133,66 +191,159 @@ def _dense_tensor(v: torch.Tensor) -> Tensor:
     return t
 
 
-def _sparse_tensor(C: ops.Compressed, fill=0.0) -> Tensor:
-    t = Tensor(Dense(SparseList(Element(fill), C.n_inner), C.n_outer))
-    t.lvl.lvl.ptr, t.lvl.lvl.idx = C.ptr, C.idx
+def _sparse_tensor(C: ops.Compressed, fill=0.0, level="list") -> Tensor:
+    """Dense(SparseList(Element)) or, with level="dict", Dense(SparseDict(Element))
+    over device result buffers.  The kernels write each fibre's entries in
+    coordinate order, so SparseDict's positions are 1:nnz."""
+    if level == "dict":
+        pos = ops.sequence(C.nnz, C.idx.dtype, C.idx.device)
+        sl = SparseDict(Element(fill), C.n_inner, C.ptr, C.idx, pos, in_order=True)
+    else:
+        sl = SparseList(Element(fill), C.n_inner, C.ptr, C.idx)
+    t = Tensor(Dense(sl, C.n_outer))
     t.element.val = C.val
     return t
 
 
-# ---- einsum --------------------------------------------------------------------------
-
-def einsum(program: str, **tensors):
-    out, oidx, red, terms, comb = parse(program)
-    (a_name, a_idx), (b_name, b_idx) = terms
-    A, B = tensors[a_name], tensors[b_name]
-    ka, va = _kind(A)
-    kb, vb = _kind(B)
-
-    # SpMV: y[i] += A[i, j] * x[j]
-    if red == "+" and comb == "*" and len(oidx) == 1 and len(a_idx) == 2 and len(b_idx) == 1:
-        i, j = a_idx
-        if oidx != (i,) or b_idx != (j,):
-            raise L.UnsupportedError(L.EUNSUP, f"index pattern of {program!r}")
-        if ka == "csc":
-            return _dense_tensor(ops.spmv_csc(_compressed(va), vb))
-        if ka == "csr":
-            return _dense_tensor(ops.spmv_csr(_compressed(va), vb))
-    # Two-operand matrix programs.
-    if len(a_idx) == 2 and len(b_idx) == 2 and len(oidx) == 2:
-        i, j = a_idx
-        j2, k = b_idx
-        if red is not None and j == j2 and oidx == (i, k):
-            if comb == "*" and red == "+" and ka in ("csr", "csc") and kb == "dense":
-                Ar = _compressed(va) if ka == "csr" else _other_order(va)
-                return _dense_tensor(ops.spmm_csr(Ar, vb))
-            if kb in ("csc", "csr") and ka in ("csc", "csr") and ka != kb:
-                # Bring B to A's storage order.
-                vb = _sparse_tensor(_other_order(vb))
-                kb = ka
-            if kb in ("csc", "csr") and ka == kb:
-                semiring = {("*", "+"): "plus_times", ("+", "min"): "min_plus"}.get((comb, red))
-                if semiring is None:
-                    raise L.UnsupportedError(L.EUNSUP, f"semiring ({comb}, {red}) unsupported")
-                if ka == "csr":  # rows of A times rows of B
-                    C = ops.spgemm(semiring, _compressed(va), _compressed(vb))
-                    return SwizzleArray(_sparse_tensor(C, _fill(semiring)), (2, 1))
-                # Column-compressed: C = A*B column by column is (B^T A^T)^T.
-                C = ops.spgemm(semiring, _compressed(vb), _compressed(va))
-                return _sparse_tensor(C, _fill(semiring))
-        if red is None and a_idx == b_idx == oidx and ka in ("csc", "csr") and kb in ("csc", "csr"):
-            if ka != kb:
-                vb = _sparse_tensor(_other_order(vb))
-            op = {"*": "mul", "+": "add", "max": "max"}[comb]
-            C = ops.ewise(op, _compressed(va), _compressed(vb))
-            t = _sparse_tensor(C)
-            return SwizzleArray(t, (2, 1)) if ka == "csr" else t
-    raise L.UnsupportedError(L.EUNSUP, f"program {program!r} over ({ka}, {kb}) operands is not "
-                                       "routed to the backend")
-
-
 def _fill(semiring):
     return float("inf") if semiring == "min_plus" else 0.0
+
+
+# ---- handlers: one per routes.toml `handler` ------------------------------------------------
+# Each takes the route and the operand views (kind, view) in program order and
+# returns (result, storage) where storage is "csc"/"csr" for a sparse result
+# (an ops.Compressed) and "dense" for a dense one (a torch tensor).
+
+def _h_spmv_csc(r, v, out=None):
+    return ops.spmv_csc(_compressed(v[0][1]), v[1][1], out), "dense"
+
+
+def _h_spmv_csr(r, v, out=None):
+    return ops.spmv_csr(_compressed(v[0][1]), v[1][1], out), "dense"
+
+
+def _h_spmm_csr(r, v, out=None):
+    return ops.spmm_csr(_compressed(v[0][1]), v[1][1], out), "dense"
+
+
+def _h_spmm_csc(r, v, out=None):
+    return ops.spmm_csr(_other_order(v[0][1]), v[1][1], out), "dense"
+
+
+def _h_sddmm_csr(r, v, out=None):
+    A = _compressed(v[0][1])
+    # X is m x k (rows of A's fibres); the kernel takes Y's factor as n x k.
+    return A.with_values(ops.sddmm_csr(A, v[1][1], v[2][1].t())), "csr"
+
+
+def _h_sddmm_csc(r, v, out=None):
+    # Column fibres j over rows i: the same row dots with the factors' roles
+    # exchanged, on A's own pattern.
+    A = _compressed(v[0][1])
+    return A.with_values(ops.sddmm_csr(A, v[2][1].t(), v[1][1])), "csc"
+
+
+def _h_ewise(r, v, out=None):
+    return ops.ewise(r["op"], _compressed(v[0][1]), _compressed(v[1][1])), "csc"
+
+
+def _h_ewise_rows(r, v, out=None):
+    return ops.ewise(r["op"], _compressed(v[0][1]), _compressed(v[1][1])), "csr"
+
+
+def _h_ewise_transpose_b(r, v, out=None):
+    return ops.ewise(r["op"], _compressed(v[0][1]), _other_order(v[1][1])), "csc"
+
+
+def _h_ewise_rows_transpose_b(r, v, out=None):
+    return ops.ewise(r["op"], _compressed(v[0][1]), _other_order(v[1][1])), "csr"
+
+
+def _h_spgemm_csc(r, v, out=None):
+    # Column-compressed C = A*B column by column is (B^T A^T)^T.
+    return ops.spgemm(r["semiring"], _compressed(v[1][1]), _compressed(v[0][1])), "csc"
+
+
+def _h_spgemm_csr(r, v, out=None):
+    return ops.spgemm(r["semiring"], _compressed(v[0][1]), _compressed(v[1][1])), "csr"
+
+
+def _h_spgemm_csr_transpose_b(r, v, out=None):
+    return ops.spgemm(r["semiring"], _compressed(v[0][1]), _other_order(v[1][1])), "csr"
+
+
+def _h_spgemm_csc_transpose_b(r, v, out=None):
+    return ops.spgemm(r["semiring"], _other_order(v[1][1]), _compressed(v[0][1])), "csc"
+
+
+HANDLERS = {
+    "spmv_csc": _h_spmv_csc, "spmv_csr": _h_spmv_csr,
+    "spmm_csr": _h_spmm_csr, "spmm_csc": _h_spmm_csc,
+    "sddmm_csr": _h_sddmm_csr, "sddmm_csc": _h_sddmm_csc,
+    "ewise": _h_ewise, "ewise_rows": _h_ewise_rows,
+    "ewise_transpose_b": _h_ewise_transpose_b,
+    "ewise_rows_transpose_b": _h_ewise_rows_transpose_b,
+    "spgemm_csc": _h_spgemm_csc, "spgemm_csr": _h_spgemm_csr,
+    "spgemm_csr_transpose_b": _h_spgemm_csr_transpose_b,
+    "spgemm_csc_transpose_b": _h_spgemm_csc_transpose_b,
+}
+
+
+def route(program: str, **tensors):
+    """(route, operand views) for a program over device operands."""
+    out, oidx, red, terms, comb = parse(program)
+    views = [_kind(tensors[name]) for name, _ in terms]
+    operands = [("dense", False) if k == "dense" else ("sparse", k == "csr") for k, _ in views]
+    key, kinds = canonical(oidx, red, terms, comb, operands)
+    r = lookup(key, kinds)
+    if r is None:
+        raise L.UnsupportedError(L.EUNSUP, f"program {program!r} (canonical {key!r} over "
+                                           f"{kinds}) is not routed to the backend")
+    return r, views
+
+
+def _result(res, storage, r, level):
+    if storage == "dense":
+        return _dense_tensor(res)
+    t = _sparse_tensor(res, _fill(r.get("semiring")), level)
+    return SwizzleArray(t, (2, 1)) if storage == "csr" else t
+
+
+def einsum(program: str, **tensors):
+    """@einsum over device operands; sparse results are Dense(SparseDict)."""
+    r, views = route(program, **tensors)
+    res, storage = HANDLERS[r["handler"]](r, views)
+    return _result(res, storage, r, "dict")
+
+
+def finch(program: str, **tensors):
+    """@finch over device operands with the output tensor declared by the
+    caller (the tensor named on the left-hand side).  A dense output is
+    written in place; a Dense(SparseList) output (or its swizzle) gets the
+    result's buffers.  Returns {output name: output tensor}, as @finch binds
+    its outputs (src/execute.jl:180-220)."""
+    out_name = parse(program)[0]
+    target = tensors[out_name]
+    r, views = route(program, **tensors)
+    if r["output"] == "dense":
+        kind, buf = _kind(target)
+        if kind != "dense":
+            raise L.UnsupportedError(L.EUNSUP, f"output {out_name} of {program!r} must be dense")
+        res, _ = HANDLERS[r["handler"]](r, views, buf)
+        if res.data_ptr() != buf.data_ptr():
+            raise L.FinchGPUError(L.EINVAL, "dense output was not written in place")
+        return {out_name: target}
+    res, storage = HANDLERS[r["handler"]](r, views)
+    kind, body = _kind(target)
+    if kind not in ("csc", "csr") or not isinstance(body.lvl.lvl, SparseList):
+        raise L.UnsupportedError(L.EUNSUP, f"output {out_name} of {program!r} must be "
+                                           "Dense(SparseList) or its swizzle")
+    if kind != storage:                      # the declared output's storage order
+        res = ops.transpose(res)
+    sl = body.lvl.lvl
+    sl.ptr, sl.idx, body.element.val = res.ptr, res.idx, res.val
+    sl.shape, body.lvl.shape = res.n_inner, res.n_outer
+    return {out_name: target}
 
 
 # ---- lazy / compute ------------------------------------------------------------------------
@@ -227,34 +378,30 @@ def maximum(a, b) -> LazyTensor:
     return LazyTensor("max", (_lazy(a), _lazy(b)))
 
 
+def _ndim(x):
+    if isinstance(x, torch.Tensor):
+        return x.dim()
+    return len(x.shape)
+
+
 def compute(expr: LazyTensor):
-    """Runs a recognised plan (src/interface/lazy.jl:707-730)."""
+    """Runs a recognised plan (src/interface/lazy.jl:707-730) through the
+    routing table; sparse results are Dense(SparseDict), as Finch's
+    scheduler returns them."""
     if expr.op == "leaf":
         return expr.args[0]
     a, b = expr.args
-    # Fused SDDMM: A .* (X @ Y) with A sparse and X, Y dense.
+    # Fused SDDMM: M .* (X @ Y) with M sparse and X, Y dense.
     if expr.op == "mul" and a.op == "leaf" and b.op == "matmul":
         x, y = b.args
         if x.op == "leaf" and y.op == "leaf":
-            ka, va = _kind(a.args[0])
-            kx, vx = _kind(x.args[0])
-            ky, vy = _kind(y.args[0])
-            if ka == "csr" and kx == ky == "dense":
-                # X is m x k; Y holds the k x n factor; the kernel takes Y^T (n x k).
-                out = ops.sddmm_csr(_compressed(va), vx, vy.t())
-                C = _compressed(va).with_values(out)
-                return SwizzleArray(_sparse_tensor(C), (2, 1))
-            if ka == "csc" and kx == ky == "dense":
-                # Column fibres j over rows i: the same row dots with the
-                # factors' roles exchanged, on A's own pattern.
-                out = ops.sddmm_csr(_compressed(va), vy.t(), vx)
-                return _sparse_tensor(_compressed(va).with_values(out))
+            return einsum("C[i, j] += M[i, j] * X[i, k] * Y[k, j]", M=a.args[0], X=x.args[0],
+                          Y=y.args[0])
     names = {"matmul": None, "mul": "*", "add": "+", "max": "max"}
     if expr.op in names and a.op == "leaf" and b.op == "leaf":
         A, B = a.args[0], b.args[0]
         if expr.op == "matmul":
-            kb, _ = _kind(B)
-            if kb == "dense" and (B.dim() if isinstance(B, torch.Tensor) else len(B.shape)) == 1:
+            if _kind(B)[0] == "dense" and _ndim(B) == 1:
                 return einsum("y[i] += A[i, j] * x[j]", A=A, x=B)
             return einsum("C[i, k] += A[i, j] * B[j, k]", A=A, B=B)
         c = names[expr.op]
@@ -304,18 +451,21 @@ def dropfills(t, fill=None):
     if k not in ("csc", "csr"):
         raise L.UnsupportedError(L.EUNSUP, "dropfills of a dense operand is not routed")
     f = v.fill_value if fill is None else fill
-    out = _sparse_tensor(ops.dropfills(_compressed(v), float(f)), f)
+    level = "dict" if v.format == "cscdict" else "list"
+    out = _sparse_tensor(ops.dropfills(_compressed(v), float(f)), f, level)
     return SwizzleArray(out, (2, 1)) if k == "csr" else out
 
 
 # ---- SpGEMM formulations (docs/examples/spgemm.jl) ------------------------------------------
 
 def _gemm_csc(A, B):
-    ka, va = _kind(A)
-    kb, vb = _kind(B)
+    ka, _ = _kind(A)
+    kb, _ = _kind(B)
     if ka != "csc" or kb != "csc":
         raise L.UnsupportedError(L.EUNSUP, "the SpGEMM examples take Dense(SparseList) operands")
-    return einsum("C[i, k] += A[i, j] * B[j, k]", A=A, B=B)
+    # The examples declare a Dense(SparseList) output and run @finch.
+    C = Tensor(Dense(SparseList(Element(0.0))))
+    return finch("C[i, k] += A[i, j] * B[j, k]", C=C, A=A, B=B)["C"]
 
 
 def spgemm_inner(A, B):

@@ -129,6 +129,15 @@ class Compressed:
         return self.with_values(convert(self.val, dtype))
 
 
+def sequence(n: int, dtype: torch.dtype, device, start: int = 1) -> torch.Tensor:
+    """start, start+1, ..., start+n-1 on the device (fgx_sequence): the
+    positions of a frozen SparseDict filled in coordinate order."""
+    out = torch.empty(n, dtype=dtype, device=device)
+    L.check(L.load().fgx_sequence(context(out.device).handle, fgx_dtype(dtype), _ptr(out), n,
+                                  start, _stream(out.device)), "fgx_sequence")
+    return out
+
+
 def validate(A: Compressed) -> None:
     """Checks A against the SparseList invariants on the device (fgx_validate):
     raises FinchProtocolError for a malformed ptr or unsorted fibre and

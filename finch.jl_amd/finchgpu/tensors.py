@@ -6,6 +6,11 @@ and write:
 * ``Element(fill)`` — leaf values (src/tensors/levels/element_levels.jl:20-22);
 * ``SparseList(lvl, shape)`` — 1-based ``ptr``/``idx`` compressed level
   (src/tensors/levels/sparse_list_levels.jl:37-42);
+* ``SparseDict(lvl, shape)`` — the hash-table level the ``compute`` scheduler
+  gives sparse results (src/interface/traits.jl:509-517), held in its frozen
+  form (src/tensors/levels/sparse_dict_levels.jl:356-400): ``ptr`` prefix
+  sums, ``idx`` sorted within each fibre, ``val`` the 1-based positions of
+  the child level's entries;
 * ``Dense(lvl, shape)`` — dense level (src/tensors/levels/dense_levels.jl:26-29);
 * ``Tensor(lvl, arr)`` — a tensor over a level tree, built from a dense array
   the way ``dropfills!`` does (src/interface/copy.jl:113-116): column-major
@@ -54,6 +59,40 @@ class SparseList:
 
     def __repr__(self):
         return f"SparseList{{{self.Ti}}}({self.lvl!r}, {self.shape}, {_show(self.ptr)}, {_show(self.idx)})"
+
+
+class SparseDict:
+    """Finch's SparseDict level after freeze.  Freezing counts each fibre's
+    entries into ``ptr``, then places every (fibre, coordinate) -> position
+    entry of the table in coordinate order, so ``idx`` is sorted within each
+    fibre and ``val[q]`` is the position in the child level of the entry
+    stored at q.  Entries inserted fibre by fibre in coordinate order (a
+    copy from another format, and every result the device kernels write)
+    give ``val == 1:nnz``; ``in_order`` records that."""
+
+    def __init__(self, lvl, shape=None, ptr=None, idx=None, val=None, Ti=np.int64,
+                 in_order=None):
+        self.lvl = lvl
+        self.shape = shape
+        self.Ti = np.dtype(Ti)
+        self.ptr = ptr if ptr is not None else np.ones(1, self.Ti)
+        self.idx = idx if idx is not None else np.zeros(0, self.Ti)
+        n = len(self.idx) if not _is_device(self.idx) else int(self.idx.numel())
+        if val is None:
+            val = np.arange(1, n + 1, dtype=self.Ti)
+            in_order = True
+        self.val = val
+        if in_order is None:
+            if _is_device(val):
+                ar = torch.arange(1, n + 1, device=val.device, dtype=val.dtype)
+                in_order = bool(torch.equal(val, ar))
+            else:
+                in_order = bool(np.array_equal(np.asarray(val), np.arange(1, n + 1)))
+        self.in_order = in_order
+
+    def __repr__(self):
+        return (f"SparseDict{{{self.Ti}}}({self.lvl!r}, {self.shape}, {_show(self.ptr)}, "
+                f"{_show(self.idx)}, {_show(self.val)})")
 
 
 class Dense:
@@ -106,6 +145,8 @@ class Tensor:
             return "sparsevec"
         if isinstance(l, Dense) and isinstance(l.lvl, SparseList) and isinstance(l.lvl.lvl, Element):
             return "csc"
+        if isinstance(l, Dense) and isinstance(l.lvl, SparseDict) and isinstance(l.lvl.lvl, Element):
+            return "cscdict"
         raise NotImplementedError(f"level tree {l!r} is not a backend format")
 
     @property
@@ -134,7 +175,7 @@ class Tensor:
 
     def countstored(self) -> int:
         f = self.format
-        if f in ("csc",):
+        if f in ("csc", "cscdict"):
             return int(self.lvl.lvl.idx.numel() if self.on_device else len(self.lvl.lvl.idx))
         if f == "sparsevec":
             return int(len(self.lvl.idx))
@@ -146,7 +187,7 @@ class Tensor:
         el = self.element
         fill = el.fill
         if hasattr(arr, "tocsc"):  # scipy.sparse (zero-copy in Finch; a copy here)
-            if f != "csc":
+            if f not in ("csc", "cscdict"):
                 raise NotImplementedError("scipy input needs Dense(SparseList(Element))")
             S = arr.tocsc()
             S.sort_indices()
@@ -155,6 +196,8 @@ class Tensor:
             sl.ptr = S.indptr.astype(sl.Ti) + 1
             sl.idx = S.indices.astype(sl.Ti) + 1
             el.val = S.data.astype(el.dtype if el.val.size == 0 else el.val.dtype)
+            if f == "cscdict":
+                sl.val, sl.in_order = np.arange(1, S.nnz + 1, dtype=sl.Ti), True
             return
         a = np.asarray(arr)
         dt = np.result_type(a.dtype, np.asarray(fill).dtype) if el.val.size == 0 else el.val.dtype
@@ -180,6 +223,8 @@ class Tensor:
             rows, cols = np.nonzero(keep.T)           # column-major order
             sl.idx = (cols + 1).astype(sl.Ti)
             el.val = a.T[rows, cols].astype(dt)
+            if f == "cscdict":   # inserted column by column, rows ascending
+                sl.val, sl.in_order = np.arange(1, len(sl.idx) + 1, dtype=sl.Ti), True
         el.dtype = np.dtype(dt) if not _is_device(el.val) else el.dtype
 
     @classmethod
@@ -216,6 +261,8 @@ class Tensor:
         ptr, idx = host(sl.ptr).astype(np.int64), host(sl.idx).astype(np.int64)
         out = np.full((m, n), el.fill, dtype=val.dtype)
         cols = np.repeat(np.arange(n), np.diff(ptr))
+        if f == "cscdict":
+            val = val[host(sl.val).astype(np.int64) - 1]
         out[idx - 1, cols] = val
         return out
 
@@ -262,6 +309,8 @@ def transfer(device: GPU, t):
     while True:
         if isinstance(l, SparseList):
             l.ptr, l.idx = mv(l.ptr), mv(l.idx)
+        if isinstance(l, SparseDict):
+            l.ptr, l.idx, l.val = mv(l.ptr), mv(l.idx), mv(l.val)
         if isinstance(l, Element):
             l.val = mv(l.val)
             break
@@ -274,6 +323,8 @@ def _copy_levels(l):
         return Element(l.fill, l.dtype, l.val)
     if isinstance(l, SparseList):
         return SparseList(_copy_levels(l.lvl), l.shape, l.ptr, l.idx, l.Ti)
+    if isinstance(l, SparseDict):
+        return SparseDict(_copy_levels(l.lvl), l.shape, l.ptr, l.idx, l.val, l.Ti, l.in_order)
     return Dense(_copy_levels(l.lvl), l.shape)
 
 

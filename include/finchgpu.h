@@ -134,6 +134,13 @@ int fgx_validate(fgx_ctx* ctx, const fgx_compressed* A, void* stream);
  * Applies to calls made through this context from then on. */
 int fgx_ctx_set_safe(fgx_ctx* ctx, int safe);
 
+/* out[q] = start + q for q in [0, n), dtype FGX_I32 or FGX_I64.  The `val`
+ * (positions) array of a frozen SparseDict level whose entries were inserted
+ * fibre by fibre in coordinate order is 1:nnz (freeze_level! of
+ * src/tensors/levels/sparse_dict_levels.jl:356-400); the routers build the
+ * SparseDict results of `compute` with it. */
+int fgx_sequence(fgx_ctx* ctx, int dtype, void* out, int64_t n, int64_t start, void* stream);
+
 /* ---- SpMV / SpMM / SDDMM ------------------------------------------------ */
 
 /* y = A*x with A read row-wise (CSR): y .= 0; for i, j: y[i] += A[j,i]*x[j]

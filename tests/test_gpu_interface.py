@@ -9,7 +9,8 @@ import torch
 import oracle
 from conftest import read_mtx
 from finchgpu import interface as fi
-from finchgpu.tensors import GPU, Dense, Element, SparseList, Tensor, swizzle, transfer
+from finchgpu.tensors import (GPU, Dense, Element, SparseDict, SparseList, SwizzleArray, Tensor,
+                              swizzle, transfer)
 
 pytestmark = pytest.mark.gpu
 
@@ -168,7 +169,7 @@ def test_fused_sddmm_on_column_compressed_pattern(dev):
     Y = transfer(GPU(0), Tensor(Dense(Dense(Element(np.float32(0)))), Yn))
     C = fi.compute(fi.lazy(M) * (fi.lazy(X) @ fi.lazy(Y)))
     ref = S.toarray().astype(np.float64) * (Xn.astype(np.float64) @ Yn.astype(np.float64))
-    assert C.format == "csc"
+    assert C.format == "cscdict"          # compute() results are Dense(SparseDict)
     assert rel_err(C.todense(), ref) <= 1e-5
 
 
@@ -218,3 +219,75 @@ def test_spgemm_formulations_match_the_examples_test(dev):
     sl0 = outs[0].lvl.lvl
     for C in outs[1:]:
         assert torch.equal(C.lvl.lvl.ptr, sl0.ptr) and torch.equal(C.lvl.lvl.idx, sl0.idx)
+
+
+# ---- @finch with declared outputs, and SparseDict results --------------------------------
+
+def test_finch_declared_dense_output_written_in_place(dev, goldens):
+    # @finch (y .= 0; for j=_, i=_; y[i] += A[i,j] * x[j]) with y declared by
+    # the caller (typical_tests.jl:47-73): y's own buffer receives the result.
+    g = goldens["spmv_csc"]
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), np.array(g["A_dense"])))
+    x = transfer(GPU(0), Tensor(Dense(Element(0.0)), np.array(g["x"])))
+    y = transfer(GPU(0), Tensor(Dense(Element(0.0)), np.full(4, 123.0)))
+    buf = y.element.val
+    out = fi.finch("y[i] += A[i, j] * x[j]", y=y, A=A, x=x)
+    assert out["y"] is y and y.element.val is buf
+    assert y.todense().tolist() == g["y"]
+    # CSR read of the same buffers (benchmarks.jl:365-376: y[i] += A[j, i] * x[j]).
+    z = transfer(GPU(0), Tensor(Dense(Element(0.0)), np.zeros(3)))
+    fi.finch("z[i] += A[j, i] * w[j]", z=z, A=A,
+             w=transfer(GPU(0), Tensor(Dense(Element(0.0)), np.ones(4))))
+    assert np.allclose(z.todense(), np.array(g["A_dense"]).sum(axis=0))
+
+
+def test_finch_declared_sparse_outputs_keep_their_format(dev):
+    S, T = _rand(60, 50, 0.1, 31), _rand(60, 50, 0.2, 32)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T))
+    C = Tensor(Dense(SparseList(Element(0.0))))
+    fi.finch("C[i, j] = A[i, j] + B[i, j]", C=C, A=A, B=B)
+    assert C.format == "csc" and C.shape == (60, 50)
+    assert np.array_equal(C.todense(), (S + T).toarray())
+    # A swizzled declared output stores the result's rows as its fibres.
+    R = SwizzleArray(Tensor(Dense(SparseList(Element(0.0)))), (2, 1))
+    fi.finch("R[i, j] = A[i, j] * B[i, j]", R=R, A=A, B=B)
+    assert np.array_equal(R.todense(), S.multiply(T).toarray())
+    assert R.body.shape == (50, 60)
+
+
+def test_compute_results_are_sparse_dict_in_frozen_layout(dev):
+    # Finch's scheduler returns Dense(SparseDict) (traits.jl:509-517); after
+    # freeze (sparse_dict_levels.jl:356-400) ptr/idx are the sorted
+    # coordinates and val the positions of the values, 1:nnz when the
+    # entries were inserted in coordinate order.
+    S, T = _rand(70, 40, 0.1, 33), _rand(70, 40, 0.15, 34)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), T))
+    D = fi.compute(fi.maximum(fi.lazy(A), fi.lazy(B)))
+    assert D.format == "cscdict" and isinstance(D.lvl.lvl, SparseDict)
+    C = Tensor(Dense(SparseList(Element(0.0))))
+    fi.finch("C[i, j] = max(A[i, j], B[i, j])", C=C, A=A, B=B)
+    sd = D.lvl.lvl
+    assert torch.equal(sd.ptr, C.lvl.lvl.ptr) and torch.equal(sd.idx, C.lvl.lvl.idx)
+    assert sd.val.tolist() == list(range(1, C.countstored() + 1)) and sd.in_order
+    assert np.array_equal(D.todense(), C.todense())
+    # A host-built SparseDict of the same matrix freezes to the same buffers
+    # (a sum of two random patterns stores no zeros).
+    P = fi.compute(fi.lazy(A) + fi.lazy(B))
+    H = Tensor(Dense(SparseDict(Element(0.0))), P.todense())
+    assert H.lvl.lvl.ptr.tolist() == P.lvl.lvl.ptr.tolist()
+    assert H.lvl.lvl.idx.tolist() == P.lvl.lvl.idx.tolist()
+    assert H.lvl.lvl.val.tolist() == P.lvl.lvl.val.tolist()
+    # SparseDict results feed later programs (values in coordinate order).
+    E = fi.einsum("C[i, j] = A[i, j] * B[i, j]", A=D, B=A)
+    assert np.array_equal(E.todense(), np.maximum(S.toarray(), T.toarray()) * S.toarray())
+
+
+def test_unrouted_device_programs_raise(dev):
+    S = _rand(20, 20, 0.2, 35)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    B = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    from finchgpu._lib import UnsupportedError
+    with pytest.raises(UnsupportedError):
+        fi.einsum("C[i, k] <<max>>= A[i, j] + B[j, k]", A=A, B=B)
