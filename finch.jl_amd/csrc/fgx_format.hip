@@ -712,6 +712,32 @@ int fgx_transpose(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_id
   return rc;
 }
 
+}  // extern "C"
+
+namespace fgx {
+
+// fgx_transpose's work for callers that already hold ctx->mu and checked A.
+int transpose_into(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_idx, void* t_val,
+                   hipStream_t s) {
+  if (A->n_outer > UINT32_MAX || A->n_inner > UINT32_MAX || A->nnz > INT32_MAX)
+    return set_error(FGX_EUNSUP, "transpose: dimensions beyond 2^32 or nnz beyond 2^31");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  const int vb = elem_size(A->val_type);
+  if (vb == 0) return set_error(FGX_EUNSUP, "transpose: value type");
+  int rc = FGX_OK;
+  FGX_DISPATCH_IDX(ic, {
+    if (vb == 2) rc = transpose_typed<TP, TI, 2>(ctx, A, t_ptr, t_idx, t_val, s);
+    else if (vb == 4) rc = transpose_typed<TP, TI, 4>(ctx, A, t_ptr, t_idx, t_val, s);
+    else rc = transpose_typed<TP, TI, 8>(ctx, A, t_ptr, t_idx, t_val, s);
+  });
+  return rc;
+}
+
+}  // namespace fgx
+
+extern "C" {
+
 int fgx_coo_to_compressed(fgx_ctx* ctx, int64_t n_outer, int64_t n_inner, int64_t nnz,
                           int idx_type, int val_type, int index_base, const void* outer,
                           const void* inner, const void* val, void* c_ptr, void* c_idx,

@@ -122,6 +122,12 @@ int validate_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const
                      uint32_t* flags = nullptr);
 int admit_operand(fgx_ctx* ctx, const fgx_compressed* A, hipStream_t s, const char* name);
 
+// Transpose (fgx_transpose) and CSR SpMV (fgx_spmv_csr) for entry points
+// that already hold ctx->mu and checked their operands.
+int transpose_into(fgx_ctx* ctx, const fgx_compressed* A, void* t_ptr, void* t_idx, void* t_val,
+                   hipStream_t s);
+int spmv_csr_into(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, hipStream_t s);
+
 // Looks up the no-allocation scratch address for ctx's (current) device.
 int noalloc_setup(fgx_ctx* ctx);
 

@@ -66,10 +66,22 @@ __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int6
     for (int q = 0; q < kSdChunk / 4; q++) {
       const float4 a4 = xv[q];
       const float4 b4 = *reinterpret_cast<const float4*>(&ys[lane][4 * q]);
+#ifdef FGX_SDDMM_VALU_DOT
+      // Measurement build only (MFMA-vs-VALU comparison): the same staging,
+      // with the dot accumulated by VALU FMAs into this lane's diagonal slot.
+      const int d = lane & 3;
+      float acc = d == 0 ? c[0] : (d == 1 ? c[1] : (d == 2 ? c[2] : c[3]));
+      acc = __builtin_fmaf(a4.x, b4.x, acc);
+      acc = __builtin_fmaf(a4.y, b4.y, acc);
+      acc = __builtin_fmaf(a4.z, b4.z, acc);
+      acc = __builtin_fmaf(a4.w, b4.w, acc);
+      c[0] = acc; c[1] = acc; c[2] = acc; c[3] = acc;
+#else
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.x, b4.x, c, 0, 0, 0);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.y, b4.y, c, 0, 0, 0);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.z, b4.z, c, 0, 0, 0);
       c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.w, b4.w, c, 0, 0, 0);
+#endif
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();

@@ -213,6 +213,16 @@ static int spmv_csc_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, 
   return launch_check("spmv_csc_kernel");
 }
 
+int spmv_csr_into(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, hipStream_t s) {
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  int rc = FGX_OK;
+  FGX_DISPATCH_VAL(A->val_type, FGX_DISPATCH_IDX(ic, rc = (spmv_csr_typed<TP, TI, TV>(ctx, A, x, y, s))));
+  return rc;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~static_cast<size_t>(255); }
+
 }  // namespace fgx
 
 using namespace fgx;
@@ -260,6 +270,38 @@ int fgx_spmv_csr_noalloc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, v
   FGX_DISPATCH_VAL(A->val_type,
                    FGX_DISPATCH_IDX(ic, rc = (spmv_csr_noalloc_typed<TP, TI, TV>(ctx, A, x, y, s))));
   return rc;
+}
+
+int fgx_spmv_csc_ordered(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                         void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_spmv_csc_ordered: NULL ctx");
+  FGX_TRY(check_compressed(A, "fgx_spmv_csc_ordered"));
+  if ((A->n_outer > 0 && !x && A->nnz > 0) || (A->n_inner > 0 && !y))
+    return set_error(FGX_EINVAL, "fgx_spmv_csc_ordered: NULL x/y");
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_spmv_csc_ordered"));
+  if (A->n_inner == 0) return FGX_OK;
+  // A's rows as fibres, in this stream's dense arena: ptr, idx, val.
+  const size_t pb = align256(static_cast<size_t>(A->n_inner + 1) * elem_size(A->ptr_type));
+  const size_t ib = align256(static_cast<size_t>(A->nnz) * elem_size(A->idx_type));
+  const size_t vb = align256(static_cast<size_t>(A->nnz) * elem_size(A->val_type));
+  void* arena;
+  FGX_TRY(dense_scratch(ctx, s, pb + ib + vb, &arena));
+  char* base = static_cast<char*>(arena);
+  fgx_compressed T = *A;
+  T.n_outer = A->n_inner;
+  T.n_inner = A->n_outer;
+  T.ptr = base;
+  T.idx = base + pb;
+  T.val = base + pb + ib;
+  FGX_TRY(transpose_into(ctx, A, const_cast<void*>(T.ptr), const_cast<void*>(T.idx),
+                         const_cast<void*>(T.val), s));
+  return spmv_csr_into(ctx, &T, x, y, s);
 }
 
 int fgx_spmv_csc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, void* stream) {

@@ -124,6 +124,7 @@ SIGNATURES = {
     "fgx_spmv_csr": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csr_noalloc": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmv_csc": (_I, [_P, _PC, _P, _P, _P]),
+    "fgx_spmv_csc_ordered": (_I, [_P, _PC, _P, _P, _P]),
     "fgx_spmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P]),
     "fgx_sddmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P, _P]),
     "fgx_ewise_symbolic": (_I, [_P, _I, _PC, _PC, _P, ctypes.POINTER(_I64), _P]),

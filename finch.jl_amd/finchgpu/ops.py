@@ -203,6 +203,21 @@ def spmv_csc(A: Compressed, x: torch.Tensor, y: torch.Tensor | None = None) -> t
     return y
 
 
+def spmv_csc_ordered(A: Compressed, x: torch.Tensor, y: torch.Tensor | None = None) -> torch.Tensor:
+    """y = A*x with A read column-wise, run-to-run deterministic
+    (fgx_spmv_csc_ordered: device transpose, then row sums)."""
+    dt = acc_dtype(A.val.dtype)
+    _check_dense(x, dt, "x")
+    if x.numel() != A.n_outer:
+        raise L.DimensionMismatch(L.EDIM, f"x has {x.numel()} entries, A has {A.n_outer} columns")
+    if y is None:
+        y = torch.empty(A.n_inner, dtype=dt, device=A.device)
+    L.check(L.load().fgx_spmv_csc_ordered(context(A.device).handle, ctypes.byref(A.desc()),
+                                          _ptr(x), _ptr(y), _stream(A.device)),
+            "fgx_spmv_csc_ordered")
+    return y
+
+
 def _layout(t: torch.Tensor):
     """(layout, ld) of a 2-D tensor: row-major or column-major strides."""
     r, c = t.shape

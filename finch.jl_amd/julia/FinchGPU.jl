@@ -307,6 +307,14 @@ function spmv_csc!(dev::GPU, y::GPUVector, A::Tensor, x::GPUVector)
     y
 end
 
+# The same y, run-to-run deterministic: a device transpose, then row sums.
+function spmv_csc_ordered!(dev::GPU, y::GPUVector, A::Tensor, x::GPUVector)
+    check(ccall((:fgx_spmv_csc_ordered, libfgx), Cint,
+                (Ptr{Cvoid}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                dev.ctx, descriptor(A), x.ptr, y.ptr, C_NULL))
+    y
+end
+
 # y .= 0; for i = _, j = _; y[i] += A[j, i] * x[j]: A's fibres are rows
 # (benchmark/benchmarks.jl:365-400).
 function spmv_csr!(dev::GPU, y::GPUVector, A::Tensor, x::GPUVector)

@@ -150,6 +150,17 @@ int fgx_sequence(fgx_ctx* ctx, int dtype, void* out, int64_t n, int64_t start, v
  * (F64 for F64 values, F32 otherwise). */
 int fgx_spmv_csr(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, void* stream);
 
+/* y = A*x with A read column-wise, deterministically: A is transposed on the
+ * device into the stream's scratch (its rows as fibres), then summed row by
+ * row as fgx_spmv_csr does, so every y[i] is the same bits from run to run
+ * and in any mix of other work.  fgx_spmv_csc instead scatters with atomic
+ * adds, as Finch's AtomicElement form does (benchmark/benchmarks.jl:402-413),
+ * whose order of adds varies.  Replaces the serial
+ * test/reference64/typical/typical_spmv_csc.txt:26-83 where run-to-run
+ * identical results are needed.  Limits of fgx_transpose apply (nnz < 2^31). */
+int fgx_spmv_csc_ordered(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                         void* stream);
+
 /* fgx_spmv_csr for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * No counterpart in the reference.  Same arguments, kernels and results

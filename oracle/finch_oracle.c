@@ -131,6 +131,82 @@ DEF_SPMM_CSR(double, f64)
 DEF_SPMM_CSR(float, f32)
 
 /* ---------------------------------------------------------------------------
+ * Threaded CSC SpMV with atomic adds:
+ *   y = Tensor(Dense(AtomicElement(0.0))); y .= 0;
+ *   for j = parallel(_), i = _; y[i] += A[i, j] * x[j]
+ * benchmark/benchmarks.jl:402-413; AtomicElement's update is an atomic
+ * read-modify-write of y[i] (src/tensors/levels/atomic_element_levels.jl:
+ * 203-224, atomic_modify! src/architecture.jl:487-547).  Columns are split in
+ * Finch's static fld blocks; each product x[j]*A[i,j] is rounded, then added
+ * atomically.  The order of the adds into one y[i] depends on the threads'
+ * interleaving, so the result is deterministic only at nthreads <= 1.
+ * ------------------------------------------------------------------------- */
+#define DEF_SPMV_CSC_ATOMIC(T, SFX)                                                      \
+  void or_spmv_csc_atomic_##SFX(i64 n_inner, i64 n_outer, const i64* ptr, const i64* idx,\
+                                const T* val, int base, const T* x, T* y, int nthreads) { \
+    for (i64 i = 0; i < n_inner; i++) y[i] = (T)0;                                       \
+    if (nthreads < 1) nthreads = 1;                                                      \
+    _Pragma("omp parallel for schedule(static, 1) num_threads(nthreads)")               \
+    for (int tid = 1; tid <= nthreads; tid++) {                                          \
+      i64 j0 = fld(n_outer * (tid - 1), nthreads);                                       \
+      i64 j1 = fld(n_outer * tid, nthreads);                                             \
+      for (i64 j = j0; j < j1; j++) {                                                    \
+        const T xj = x[j];                                                               \
+        for (i64 q = ptr[j] - base; q < ptr[j + 1] - base; q++) {                        \
+          const T p = xj * val[q];                                                       \
+          T* dst = &y[idx[q] - base];                                                    \
+          _Pragma("omp atomic")                                                          \
+          *dst += p;                                                                     \
+        }                                                                                \
+      }                                                                                  \
+    }                                                                                    \
+  }
+DEF_SPMV_CSC_ATOMIC(double, f64)
+DEF_SPMV_CSC_ATOMIC(float, f32)
+
+/* ---------------------------------------------------------------------------
+ * Greedy schedule of the threaded CSR loops (src/architecture.jl:721-767):
+ * threads repeatedly take the next chunk of `chk` rows from a shared atomic
+ * counter until the rows run out; each row is summed as in the static
+ * programs above, so the result equals the serial one bit for bit.
+ * ------------------------------------------------------------------------- */
+#define DEF_GREEDY(T, SFX)                                                               \
+  void or_spmv_csr_greedy_##SFX(i64 n_outer, const i64* ptr, const i64* idx,            \
+                                const T* val, int base, const T* x, T* y, i64 chk,       \
+                                int nthreads) {                                          \
+    if (chk < 1) chk = 1;                                                                \
+    i64 next = 0;                                                                        \
+    _Pragma("omp parallel num_threads(nthreads)")                                        \
+    for (;;) {                                                                           \
+      i64 c;                                                                             \
+      _Pragma("omp atomic capture")                                                      \
+      c = next++;                                                                        \
+      const i64 r0 = c * chk;                                                            \
+      if (r0 >= n_outer) break;                                                          \
+      const i64 r1 = r0 + chk < n_outer ? r0 + chk : n_outer;                            \
+      spmv_csr_rows_##SFX(r0, r1, ptr, idx, val, base, x, y);                            \
+    }                                                                                    \
+  }                                                                                      \
+  void or_spmm_csr_greedy_##SFX(i64 n_outer, i64 k, const i64* ptr, const i64* idx,     \
+                                const T* val, int base, const T* B, T* C, i64 chk,       \
+                                int nthreads) {                                          \
+    if (chk < 1) chk = 1;                                                                \
+    i64 next = 0;                                                                        \
+    _Pragma("omp parallel num_threads(nthreads)")                                        \
+    for (;;) {                                                                           \
+      i64 c;                                                                             \
+      _Pragma("omp atomic capture")                                                      \
+      c = next++;                                                                        \
+      const i64 r0 = c * chk;                                                            \
+      if (r0 >= n_outer) break;                                                          \
+      const i64 r1 = r0 + chk < n_outer ? r0 + chk : n_outer;                            \
+      spmm_csr_rows_##SFX(r0, r1, k, ptr, idx, val, base, B, C);                         \
+    }                                                                                    \
+  }
+DEF_GREEDY(double, f64)
+DEF_GREEDY(float, f32)
+
+/* ---------------------------------------------------------------------------
  * SDDMM:  C[i,j] = A[i,j] * sum_k X[i,k]*Y[j,k] on pattern(A)
  * The fused plan of compute(lazy(A) .* (lazy(X)*lazy(Yt)))
  * (benchmark/benchmarks.jl:141-168) is the @finch nest

@@ -74,6 +74,48 @@ def spmv_csc(n_inner, ptr, idx, val, x, base=1):
     return y
 
 
+def spmv_csc_atomic(n_inner, ptr, idx, val, x, base=1, nthreads=1):
+    """y = A*x, A column-compressed, columns split over threads with atomic
+    adds into y (benchmark/benchmarks.jl:402-413, AtomicElement)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    x = np.ascontiguousarray(x, dtype=val.dtype)
+    ptr, idx = _i64(ptr), _i64(idx)
+    y = np.empty(n_inner, dtype=val.dtype)
+    f = getattr(lib(), f"or_spmv_csc_atomic_{s}")
+    f(_I64(n_inner), _I64(len(ptr) - 1), _p(ptr), _p(idx), _p(val), _INT(base), _p(x), _p(y),
+      _INT(nthreads))
+    return y
+
+
+def spmv_csr_greedy(ptr, idx, val, x, chunk, base=1, nthreads=1):
+    """CSR SpMV with Finch's greedy schedule (src/architecture.jl:721-767)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    x = np.ascontiguousarray(x, dtype=val.dtype)
+    ptr, idx = _i64(ptr), _i64(idx)
+    m = len(ptr) - 1
+    y = np.empty(m, dtype=val.dtype)
+    f = getattr(lib(), f"or_spmv_csr_greedy_{s}")
+    f(_I64(m), _p(ptr), _p(idx), _p(val), _INT(base), _p(x), _p(y), _I64(chunk), _INT(nthreads))
+    return y
+
+
+def spmm_csr_greedy(ptr, idx, val, B, chunk, base=1, nthreads=1):
+    """CSR SpMM with Finch's greedy schedule (src/architecture.jl:721-767)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    B = np.ascontiguousarray(B, dtype=val.dtype)
+    ptr, idx = _i64(ptr), _i64(idx)
+    m = len(ptr) - 1
+    k = B.shape[1]
+    C = np.empty((m, k), dtype=val.dtype)
+    f = getattr(lib(), f"or_spmm_csr_greedy_{s}")
+    f(_I64(m), _I64(k), _p(ptr), _p(idx), _p(val), _INT(base), _p(B), _p(C), _I64(chunk),
+      _INT(nthreads))
+    return C
+
+
 def spmv_csr(ptr, idx, val, x, base=1, nthreads=1):
     """y = A*x, A row-compressed (parallel_spmv.txt:36-76)."""
     val = np.ascontiguousarray(val)

@@ -281,3 +281,84 @@ def test_bf16_values_fp32_accumulate(dev, powerlaw_1m):
     vb = Ab.val.float().cpu().numpy().astype(np.float64)
     refb = oracle.spmv_csr(ptr, idx, vb, B[:, 0].cpu().numpy().astype(np.float64), nthreads=8)
     assert rel_err(Cb[:, 0], refb) <= FP32_NORMWISE
+
+
+# ---- FP16-stored values, FP32 accumulation ---------------------------------------
+
+@pytest.mark.parametrize("itype", [torch.int32, torch.int64])
+@pytest.mark.parametrize("k", [1, 8, 64])
+def test_fp16_values_fp32_accumulate(dev, itype, k):
+    # Values stored as IEEE half, widened on load, summed in FP32.  Against the
+    # FP64 oracle on the half-rounded values the result is an FP32 sum again
+    # (1e-5); against the unrounded values only storage rounding remains (1e-2).
+    m, n = 120_000, 90_000
+    ptr, idx, val = synth.powerlaw_csr(m, n, 1_500_000, cap=20_000, seed=41)
+    A = operand(ptr, idx, val, m, n, dev, itype)
+    Ah = A.to_values(torch.float16)
+    assert Ah.val.dtype == torch.float16
+    vh = Ah.val.float().cpu().numpy().astype(np.float64)
+    # The conversion kernel rounds to nearest even like numpy's float16 cast.
+    assert np.array_equal(vh, val.astype(np.float16).astype(np.float64))
+    Bn = synth.dense(n, k, seed=42)
+    if k == 1:
+        got = ops.spmv_csr(Ah, up(Bn[:, 0].copy(), dev)).cpu().numpy()
+        ref_h = oracle.spmv_csr(ptr, idx, vh, Bn[:, 0].astype(np.float64), nthreads=8)
+        ref_v = oracle.spmv_csr(ptr, idx, val.astype(np.float64), Bn[:, 0].astype(np.float64),
+                                nthreads=8)
+    else:
+        got = ops.spmm_csr(Ah, up(Bn, dev)).cpu().numpy()
+        ref_h = oracle.spmm_csr(ptr, idx, vh, Bn.astype(np.float64), nthreads=8)
+        ref_v = oracle.spmm_csr(ptr, idx, val.astype(np.float64), Bn.astype(np.float64),
+                                nthreads=8)
+    assert rel_err(got, ref_h) <= FP32_NORMWISE
+    assert rel_err(got, ref_v) <= 1e-2
+
+
+def test_fp16_values_sddmm_and_csc(dev):
+    m, n, k = 40_000, 30_000, 32
+    ptr, idx, val = synth.powerlaw_csr(m, n, 400_000, cap=5_000, seed=43)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32).to_values(torch.float16)
+    vh = A.val.float().cpu().numpy().astype(np.float64)
+    X, Y = synth.dense(m, k, seed=44), synth.dense(n, k, seed=45)
+    out = ops.sddmm_csr(A, up(X, dev), up(Y, dev)).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, vh, X.astype(np.float64), Y.astype(np.float64))
+    assert rel_err(out, ref) <= FP32_NORMWISE
+    # The same buffers read column-wise (CSC SpMV: y = A^T x here).
+    x = synth.dense(m, 1, seed=46)[:, 0].copy()
+    y = ops.spmv_csc(A, up(x, dev)).cpu().numpy()
+    ref_c = oracle.spmv_csc(n, ptr, idx, vh, x.astype(np.float64))
+    assert rel_err(y, ref_c) <= FP32_NORMWISE
+
+
+# ---- deterministic CSC SpMV --------------------------------------------------------
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, FP64_REL), (np.float32, FP32_NORMWISE)])
+@pytest.mark.parametrize("itype", [(torch.int32, torch.int32), (torch.int64, torch.int64),
+                                   (torch.int64, torch.int32)])
+def test_spmv_csc_ordered_matches_oracle_and_repeats(dev, dtype, tol, itype):
+    # 10k x 10k at density 1e-3 (the FP64 parity workload) and a power-law
+    # operand with long columns: same bits on every call, unlike the atomic
+    # scatter, and within tolerance of the serial oracle.
+    cases = [synth.fsprand(10_000, 10_000, 1e-3, seed=0) + (10_000, 10_000),
+             synth.powerlaw_csr(30_000, 20_000, 500_000, cap=15_000, seed=61) + (30_000, 20_000)]
+    for ptr, idx, val, n_outer, n_inner in cases:
+        A = ops.Compressed(up(ptr, dev, itype[0]), up(idx, dev, itype[1]),
+                           up(val.astype(dtype), dev), n_outer, n_inner, 1)
+        x = synth.dense(n_outer, 1, seed=62)[:, 0].astype(dtype)
+        ref = oracle.spmv_csc(n_inner, ptr, idx, val.astype(np.float64), x.astype(np.float64))
+        y1 = ops.spmv_csc_ordered(A, up(x, dev)).cpu().numpy()
+        y2 = ops.spmv_csc_ordered(A, up(x, dev)).cpu().numpy()
+        assert np.array_equal(y1, y2)
+        assert rel_err(y1, ref) <= tol
+        ya = ops.spmv_csc(A, up(x, dev)).cpu().numpy()
+        assert rel_err(ya, ref) <= tol
+
+
+def test_spmv_csc_ordered_empty_and_golden(dev, goldens):
+    g = goldens["spmv_csc"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    A = operand(ptr, idx, val, 3, 4, dev)
+    y = ops.spmv_csc_ordered(A, up(np.array(g["x"]), dev)).cpu().numpy()
+    assert y.tolist() == g["y"]           # at most two terms per row: exact
+    E = operand(np.ones(6, np.int64), np.zeros(0, np.int64), np.zeros(0), 5, 7, dev)
+    assert ops.spmv_csc_ordered(E, up(np.ones(5), dev)).cpu().tolist() == [0.0] * 7

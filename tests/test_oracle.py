@@ -410,3 +410,41 @@ def test_min_plus_uses_julia_min():
         p, i, v = oracle.spgemm("min_plus", 1, 1, np.array([1, 3]), np.array([1, 2]),
                                 np.array(order), B_ptr, B_idx, np.array([0.0, 0.0]))
         assert np.isnan(v[0])
+
+
+# ---- threaded baselines: AtomicElement CSC and the greedy schedule --------------------
+
+def test_spmv_csc_atomic_golden_and_scipy(goldens):
+    # benchmark/benchmarks.jl:402-413 (AtomicElement y, parallel columns).  On
+    # the golden of typical_spmv_csc.txt:92 every y[i] has at most two terms,
+    # and two FP adds onto 0 give the same bits in either order.
+    g = goldens["spmv_csc"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    for nth in (1, 2, 3):
+        y = oracle.spmv_csc_atomic(4, ptr, idx, val, np.array(g["x"], np.float64), nthreads=nth)
+        assert y.tolist() == g["y"]
+    rng = np.random.default_rng(51)
+    S = sp.random(3000, 2000, density=0.01, format="csc", random_state=rng)
+    x = rng.standard_normal(2000)
+    ref = S @ x
+    serial = oracle.spmv_csc(3000, S.indptr + 1, S.indices + 1, S.data, x)
+    assert np.array_equal(oracle.spmv_csc_atomic(3000, S.indptr + 1, S.indices + 1, S.data, x,
+                                                 nthreads=1), serial)
+    y8 = oracle.spmv_csc_atomic(3000, S.indptr + 1, S.indices + 1, S.data, x, nthreads=8)
+    assert np.linalg.norm(y8 - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("chunk", [1, 7, 1000])
+@pytest.mark.parametrize("nth", [1, 3, 8])
+def test_greedy_schedule_equals_serial_bit_for_bit(chunk, nth):
+    rng = np.random.default_rng(52)
+    S = sp.random(5000, 4000, density=0.003, format="csr", random_state=rng)
+    S.sort_indices()
+    x = rng.standard_normal(4000)
+    B = rng.standard_normal((4000, 5))
+    p, i = S.indptr + 1, S.indices + 1
+    y0 = oracle.spmv_csr(p, i, S.data, x)
+    C0 = oracle.spmm_csr(p, i, S.data, B)
+    assert np.array_equal(oracle.spmv_csr_greedy(p, i, S.data, x, chunk, nthreads=nth), y0)
+    assert np.array_equal(oracle.spmm_csr_greedy(p, i, S.data, B, chunk, nthreads=nth), C0)
+    assert np.linalg.norm(y0 - S @ x) <= 1e-13 * np.linalg.norm(S @ x)
