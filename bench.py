@@ -667,35 +667,75 @@ def bench_serving(dev, slo_ms, large):
     operands32 = small + big
     operands16 = [A.to_values(torch.bfloat16) for A in operands32]
     inputs = serve.Inputs(operands32, seed=1)
+    inputs16 = serve.Inputs(operands32, seed=1, dtype=torch.bfloat16)
+    in_of = {"fp32": (inputs, torch.float32), "bf16": (inputs16, torch.bfloat16)}
     qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 100_000, 140_000,
                 200_000, 300_000]
 
-    def run(ops_, mode, q, seed):
+    def run(ops_, mode, q, seed, inp="fp32"):
         n = int(min(20_000, max(2_000, q * 0.25)))
-        reqs = serve.make_trace(operands32, inputs, qps=q, n=n, seed=seed)
-        s = serve.serve(ops_, reqs, mode).summary()
+        src, dt = in_of[inp]
+        reqs = serve.make_trace(operands32, src, qps=q, n=n, seed=seed)
+        s = serve.serve(ops_, reqs, mode, input_dtype=dt).summary()
         return {"offered_qps": q, "req_per_s": round(s["req_per_s"], 1),
                 "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
                 "launches": s["launches"], "ok": s["p99_ms"] <= slo_ms and s["rejected"] == 0}
 
-    def passes(ops_, mode, q, points):
+    def passes(ops_, mode, q, points, inp):
         # A rate passes when two runs (different arrival draws) both keep p99
         # within the target; the slower of the two is recorded.
-        a = run(ops_, mode, q, 7)
+        a = run(ops_, mode, q, 7, inp)
         points.append(a)
         if not a["ok"]:
             return None
-        b = run(ops_, mode, q, 8)
+        b = run(ops_, mode, q, 8, inp)
         points.append(b)
         if not b["ok"]:
             return None
         return min((a, b), key=lambda r: r["req_per_s"]) | {"p99_ms": max(a["p99_ms"], b["p99_ms"])}
 
-    def sweep(ops_, mode):
-        run(ops_, mode, 20_000, 1)          # warm-up (plans, tables, scratch)
+    def serving_error(ops_, mode, inp):
+        """Worst and mean normwise error vs the FP64 oracle (on the FP32
+        operands and inputs) over 60 requests of the mix, at least one per
+        large operand; the checker runs on the host after serving."""
+        sys.path.insert(0, os.path.join(HERE, "oracle"))
+        import oracle
+        src, dt = in_of[inp]
+        rng = np.random.default_rng(9)
+        reqs = serve.make_trace(operands32, src, qps=20_000, n=56, seed=9, outputs=True)
+        for j in range(4):
+            kind = ["spmv", "spmm", "sddmm", "spmm"][j]
+            k = {"spmv": 1, "spmm": 64, "sddmm": 32}[kind]
+            ins = src.get(256 + j, kind, k)
+            A = operands32[256 + j]
+            shape = (A.n_outer,) if kind == "spmv" else ((A.nnz,) if kind == "sddmm"
+                                                          else (A.n_outer, k))
+            reqs.append(serve.Request(kind, 256 + j, k, float(0.0028 + 1e-4 * j), ins[0],
+                                      ins[1] if len(ins) > 1 else None,
+                                      torch.empty(shape, dtype=torch.float32, device=dev)))
+        serve.serve(ops_, reqs, mode, input_dtype=dt)
+        errs = []
+        for r in reqs:
+            A = operands32[r.operand]
+            p, i = A.ptr.cpu().numpy(), A.idx.cpu().numpy()
+            v = A.val.cpu().numpy().astype(np.float64)
+            x = inputs.get(r.operand, r.kind, r.k)
+            x0 = x[0].cpu().numpy().astype(np.float64)
+            if r.kind == "spmv":
+                ref = oracle.spmv_csr(p, i, v, x0, nthreads=16)
+            elif r.kind == "spmm":
+                ref = oracle.spmm_csr(p, i, v, x0, nthreads=16)
+            else:
+                ref = oracle.sddmm_csr(p, i, v, x0, x[1].cpu().numpy().astype(np.float64))
+            errs.append(_normwise(r.out, ref))
+        del rng
+        return {"max": max(errs), "mean": float(f"{float(np.mean(errs)):.3e}"), "requests": len(errs)}
+
+    def sweep(ops_, mode, inp):
+        run(ops_, mode, 20_000, 1, inp)     # warm-up (plans, tables, scratch)
         best, points, lo, hi = None, [], None, None
         for q in qps_grid:
-            r = passes(ops_, mode, q, points)
+            r = passes(ops_, mode, q, points, inp)
             if r is None:
                 hi = q
                 break
@@ -704,26 +744,31 @@ def bench_serving(dev, slo_ms, large):
         if lo is not None and hi is not None:
             for _ in range(4):
                 mid = (lo + hi) // 2
-                r = passes(ops_, mode, mid, points)
+                r = passes(ops_, mode, mid, points, inp)
                 if r is None:
                     hi = mid
                 else:
                     best, lo = r, mid
         return best, points
 
+    variants = (("plain_fp32", operands32, "plain", "fp32"),
+                ("batched_fp32", operands32, "batched", "fp32"),
+                ("batched_bf16", operands16, "batched", "fp32"),
+                ("batched_bf16_values_and_inputs", operands16, "batched", "bf16"))
     res = {}
-    for name, ops_, mode in (("plain_fp32", operands32, "plain"),
-                             ("batched_fp32", operands32, "batched"),
-                             ("batched_bf16", operands16, "batched")):
-        best, points = sweep(ops_, mode)
-        res[name] = {"best": best, "sweep": points}
+    for name, ops_, mode, inp in variants:
+        best, points = sweep(ops_, mode, inp)
+        res[name] = {"best": best, "sweep": points,
+                     "normwise_err_vs_fp64": serving_error(ops_, mode, inp)}
         print(f"# serving {name}: {best}", file=sys.stderr, flush=True)
     base = (res["plain_fp32"]["best"] or {}).get("req_per_s")
-    for name in ("batched_fp32", "batched_bf16"):
+    for name, *_ in variants[1:]:
         b = (res[name]["best"] or {}).get("req_per_s")
         res[name]["x_plain"] = round(b / base, 2) if (b and base) else None
     res["p99_target_ms"] = slo_ms
-    res["operands"] = "256 batched-distribution CSR + 4 x 1M-row Zipf(1.5), FP32 and BF16 values"
+    res["operands"] = ("256 batched-distribution CSR + 4 x 1M-row Zipf(1.5); FP32 values, BF16 "
+                       "values, and BF16 values with BF16 dense inputs (FP32 accumulation and "
+                       "outputs throughout)")
     return res
 
 

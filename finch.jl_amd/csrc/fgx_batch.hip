@@ -108,7 +108,12 @@ static int launch_raw(fgx_batch_plan* p, hipStream_t s) {
     if (p->segmented && total == 0) continue;
     const SegInline* one = (p->segmented && p->cls_inl[c].n > 0) ? &p->cls_inl[c] : nullptr;
     int rc;
-    if (p->val_type == FGX_F32)
+    if (p->dense_type == FGX_BF16) {
+      if (p->val_type != FGX_BF16)
+        return set_error(FGX_EUNSUP, "batch: BF16 dense inputs need BF16 values");
+      rc = cls == kClsSddmm ? batch_sddmm_bf16_bf16in(p, cls, w0, w1, seg, total, one, s)
+                            : batch_spmm_bf16_bf16in(p, cls, w0, w1, seg, total, one, s);
+    } else if (p->val_type == FGX_F32)
       rc = cls == kClsSddmm ? batch_sddmm_f32(p, cls, w0, w1, seg, total, one, s)
                             : batch_spmm_f32(p, cls, w0, w1, seg, total, one, s);
     else if (p->val_type == FGX_BF16)

@@ -26,8 +26,8 @@ struct BatchDesc {
   const void* ptr;
   const void* idx;
   const void* val;
-  const float* B;   // SpMM/SpMV: dense input (n x k); SDDMM: X (m x k)
-  const float* B2;  // SDDMM: Y (n x k)
+  const void* B;    // SpMM/SpMV: dense input (n x k); SDDMM: X (m x k); FP32 or BF16
+  const void* B2;   // SDDMM: Y (n x k), same type as B
   float* C;         // SpMM/SpMV: output (m x k); SDDMM: output values (nnz)
   int64_t m, n, nnz, k, ldb, ldc;
   const int64_t* table; // tile table hint of TILES items (may be null)
@@ -102,6 +102,7 @@ __host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
 struct fgx_batch_plan {
   int device = 0;
   int val_type = FGX_F32;
+  int dense_type = FGX_F32;   // dense inputs: FGX_F32, or FGX_BF16 (with BF16 values)
   int combo = 0;
   int64_t count = 0;
   int64_t grid = 0;

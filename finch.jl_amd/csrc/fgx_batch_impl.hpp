@@ -20,7 +20,7 @@ namespace fgx {
 // a chunk of kRowsSpmvRun * G positions, then a shuffle tree per row.
 constexpr int kRowsSpmvRun = 4;
 
-template <typename TP, typename TI, typename TV>
+template <typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_t r1) {
   constexpr int G = kSpmvRowGroup;
   constexpr int S = kRowsSpmvRun;
@@ -29,6 +29,8 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
   const TP* ptr = static_cast<const TP*>(d.ptr);
   const TI* idx = static_cast<const TI*>(d.idx);
   const TV* val = static_cast<const TV*>(d.val);
+  const TD* x = static_cast<const TD*>(d.B);
+  float* y = static_cast<float*>(d.C);
   const int64_t base = d.base;
   for (int64_t r = r0 + g; r < r1; r += kWave / G) {
     const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
@@ -46,14 +48,14 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
       // accumulated), so they issue back to back.
       float xg[S];
 #pragma unroll
-      for (int u = 0; u < S; u++) xg[u] = d.B[static_cast<int64_t>(jj[u]) - base];
+      for (int u = 0; u < S; u++) xg[u] = widen<float>(x[static_cast<int64_t>(jj[u]) - base]);
 #pragma unroll
       for (int u = 0; u < S; u++)
         if (c + gl + u * G < q1) acc += widen<float>(vv[u]) * xg[u];
     }
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
-    if (gl == 0) d.C[r] = acc;
+    if (gl == 0) y[r] = acc;
   }
 }
 
@@ -61,7 +63,7 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
 // row is walked in chunks of G positions (lane gl loads position c + gl),
 // broadcast within the group, with U rows of B gathered at a time; the sum
 // runs over the row's positions in order.
-template <int G, typename TP, typename TI, typename TV>
+template <int G, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_t r1) {
   constexpr int U = G < 8 ? G : 8;
   const int lane = lane_id();
@@ -71,6 +73,8 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
   const TI* idx = static_cast<const TI*>(d.idx);
   const TV* val = static_cast<const TV*>(d.val);
   const int64_t base = d.base;
+  const TD* Bd = static_cast<const TD*>(d.B);
+  float* Cd = static_cast<float*>(d.C);
   const int64_t col0 = static_cast<int64_t>(gl) * 4;
   const int vc = static_cast<int>(col0 < d.k ? (d.k - col0 < 4 ? d.k - col0 : 4) : 0);
   for (int64_t r = r0 + g; r < r1; r += kWave / G) {
@@ -82,44 +86,44 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
       const float vcv = inc ? widen<float>(ld_stream(val + c + gl)) : 0.f;
       const int cnt = static_cast<int>(q1 - c < G ? q1 - c : G);
       for (int e0 = 0; e0 < cnt; e0 += U) {
-        float bv[U][4];
+        DenseRaw<4, TD> bv[U];
         float ve[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int src = gbase + ((e0 + u) % G);
           const int64_t je = shfl(jc, src);
           ve[u] = shfl(vcv, src);
-          if (e0 + u < cnt && vc > 0) {
-            ld_dense<4, float>(d.B + je * d.ldb + col0, vc, bv[u]);
-          } else {
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) bv[u][cc] = 0.f;
-          }
+          if (e0 + u < cnt && vc > 0) ld_raw<4, TD>(Bd + je * d.ldb + col0, vc, bv[u]);
+          else zero_raw<4, TD>(bv[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
           if (e0 + u < cnt) {
+            float bw[4];
+            widen_raw<4, TD, float>(bv[u], bw);
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) acc[cc] += ve[u] * bv[u][cc];
+            for (int cc = 0; cc < 4; cc++) acc[cc] += ve[u] * bw[cc];
           }
       }
     }
-    if (vc > 0) st_dense<4, float>(d.C + r * d.ldc + col0, vc, acc);
+    if (vc > 0) st_dense<4, float>(Cd + r * d.ldc + col0, vc, acc);
   }
 }
 
 // ---- SDDMM units: wave tiles of kWave * kSddmmSub positions ---------------------------------
 // `lds` holds the wave's fibre window (kSdWin entries) followed by the Y
 // staging tile of the matrix-core path.
-template <typename TP, typename TI, typename TV>
+template <typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t t0, int64_t t1,
                                            unsigned char* lds) {
   int64_t* win_lds = reinterpret_cast<int64_t*>(lds);
   float(*ys)[kSdPitch] = reinterpret_cast<float(*)[kSdPitch]>(lds + kSdWin * sizeof(int64_t));
   for (int64_t t = t0; t < t1; t++)
-    sddmm_tile<TP, TI, TV>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
-                           static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, d.B, d.ldb,
-                           d.B2, d.ldc, d.C, t, d.table, win_lds, ys, d.mfma != 0);
+    sddmm_tile<TP, TI, TV, TD>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+                               static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k,
+                               static_cast<const TD*>(d.B), d.ldb, static_cast<const TD*>(d.B2),
+                               d.ldc, static_cast<float*>(d.C), t, d.table, win_lds, ys,
+                               d.mfma != 0);
 }
 
 // ---- TILES units ------------------------------------------------------------------------------
@@ -129,7 +133,7 @@ constexpr int tile_w_for_g() { return (kWave / G) * tile_e_for_g(G); }
 
 // The tile's coordinates and values are staged in LDS after the fibre window
 // (`lds` + kWinCap entries), as in the single-operand SpMM kernel.
-template <int G, typename TP, typename TI, typename TV>
+template <int G, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
   constexpr int U = G < 8 ? G : 8;
@@ -138,19 +142,21 @@ __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64
   TI* lj = reinterpret_cast<TI*>(lds + kWinCap);
   float* lv = reinterpret_cast<float*>(lj + W);
   for (int64_t t = t0; t < t1; t++)
-    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U>(
+    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U, TD>(
         static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
-        static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, d.B, d.ldb, d.C, d.ldc, t,
+        static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, static_cast<const TD*>(d.B),
+        d.ldb, static_cast<float*>(d.C), d.ldc, t,
         d.table, lds, carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4),
         lj, lv);
 }
 
-template <typename TP, typename TI, typename TV>
+template <typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void tiles_spmv(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
   for (int64_t t = t0; t < t1; t++)
-    spmv_tile<TP, TI, TV, float, kBatchSpmvRun>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
-                                    static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.B, d.C,
+    spmv_tile<TP, TI, TV, float, kBatchSpmvRun, TD>(static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
+                                    static_cast<const TV*>(d.val), d.base, d.m, d.nnz,
+                                    static_cast<const TD*>(d.B), static_cast<float*>(d.C),
                                     t, d.vec != 0, d.table, lds, carry_row + d.carry_tile0 + t,
                                     carry_val + d.carry_val0 + t);
 }
@@ -195,19 +201,19 @@ __device__ __forceinline__ int64_t find_segment(const int64_t* __restrict__ seg_
   return lo;
 }
 
-template <typename TP, typename TI, typename TV, int CLS>
+template <typename TP, typename TI, typename TV, typename TD, int CLS>
 __device__ __forceinline__ void run_unit(const BatchDesc& d, int32_t kind, int64_t b, int64_t e,
                                          unsigned char* smem, int64_t* carry_row,
                                          float* carry_val) {
   int64_t* lds = reinterpret_cast<int64_t*>(smem);
   if constexpr (CLS == kClsSddmm) {
-    sddmm_unit<TP, TI, TV>(d, b, e, smem);
+    sddmm_unit<TP, TI, TV, TD>(d, b, e, smem);
   } else if constexpr (CLS == kClsSpmv) {
-    if (kind == BK_ROWS) rows_spmv<TP, TI, TV>(d, b, e);
-    else tiles_spmv<TP, TI, TV>(d, b, e, lds, carry_row, carry_val);
+    if (kind == BK_ROWS) rows_spmv<TP, TI, TV, TD>(d, b, e);
+    else tiles_spmv<TP, TI, TV, TD>(d, b, e, lds, carry_row, carry_val);
   } else {
-    if (kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV>(d, b, e);
-    else tiles_spmm<CLS, TP, TI, TV>(d, b, e, lds, carry_row, carry_val);
+    if (kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV, TD>(d, b, e);
+    else tiles_spmm<CLS, TP, TI, TV, TD>(d, b, e, lds, carry_row, carry_val);
   }
 }
 
@@ -222,7 +228,7 @@ __device__ __forceinline__ void run_unit(const BatchDesc& d, int32_t kind, int64
 //     hardware can interleave the blocks of launches on different streams.
 // The two forms are separate instantiations (SEG), so the single-tile
 // segmented body is not held to the registers of the persistent unit loop.
-template <typename TP, typename TI, typename TV, int CLS, bool SEG>
+template <typename TP, typename TI, typename TV, typename TD, int CLS, bool SEG>
 __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t w0,
     int64_t w1, const int64_t* __restrict__ seg_off, int64_t total,
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
       int64_t u = 0;
       for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= t ? 1 : 0;
       const int64_t b = t - inl.seg[u];
-      run_unit<TP, TI, TV, CLS>(inl.desc[u], inl.desc[u].kind, b, b + 1, smem[w], carry_row,
+      run_unit<TP, TI, TV, TD, CLS>(inl.desc[u], inl.desc[u].kind, b, b + 1, smem[w], carry_row,
                                 carry_val);
       return;
     }
@@ -246,13 +252,13 @@ __global__ __launch_bounds__(kBatchThreads) void batch_kernel(
     const int64_t u = w0 + find_segment(seg_off, w1 - w0, t);
     const BatchDesc d = desc[u];
     const int64_t b = t - seg_off[u - w0];
-    run_unit<TP, TI, TV, CLS>(d, d.kind, b, b + 1, smem[w], carry_row, carry_val);
+    run_unit<TP, TI, TV, TD, CLS>(d, d.kind, b, b + 1, smem[w], carry_row, carry_val);
     return;
   }
   const int64_t nw = static_cast<int64_t>(gridDim.x) * kWavesPerBlock;
   for (int64_t u = w0 + xcd_block() * kWavesPerBlock + w; u < w1; u += nw) {
     const BatchWork wk = work[u];
-    run_unit<TP, TI, TV, CLS>(desc[wk.item], wk.kind, wk.begin, wk.end, smem[w], carry_row,
+    run_unit<TP, TI, TV, TD, CLS>(desc[wk.item], wk.kind, wk.begin, wk.end, smem[w], carry_row,
                               carry_val);
   }
 }
@@ -268,7 +274,7 @@ inline int64_t persistent_grid(int64_t n_work, int num_cus, int blocks_per_cu) {
   return std::max<int64_t>(grid, 1);
 }
 
-template <typename TP, typename TI, typename TV, int CLS, bool SEG>
+template <typename TP, typename TI, typename TV, typename TD, int CLS, bool SEG>
 int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off, int64_t total,
                 const SegInline* inl, hipStream_t s) {
   int64_t grid;
@@ -279,7 +285,7 @@ int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_of
     int o = occ.load();
     if (o == 0) {
       int n = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<TP, TI, TV, CLS, false>,
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, batch_kernel<TP, TI, TV, TD, CLS, false>,
                                                        kBatchThreads, 0) != hipSuccess ||
           n < 1) {
         (void)hipGetLastError();
@@ -291,36 +297,37 @@ int launch_form(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_of
     grid = persistent_grid(w1 - w0, p->num_cus, o);
   }
   p->grid = std::max<int64_t>(p->grid, grid);
-  hipLaunchKernelGGL((batch_kernel<TP, TI, TV, CLS, SEG>), dim3(static_cast<unsigned>(grid)),
+  hipLaunchKernelGGL((batch_kernel<TP, TI, TV, TD, CLS, SEG>), dim3(static_cast<unsigned>(grid)),
                      dim3(kBatchThreads), 0, s, p->d_desc, p->d_work, w0, w1, seg_off, total,
                      p->d_carry_row, p->d_carry_val, inl ? *inl : SegInline{});
   return launch_check("batch_kernel");
 }
 
-template <typename TP, typename TI, typename TV, int CLS>
+template <typename TP, typename TI, typename TV, typename TD, int CLS>
 int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_off,
                  int64_t total, const SegInline* inl, hipStream_t s) {
-  return seg_off ? launch_form<TP, TI, TV, CLS, true>(p, w0, w1, seg_off, total, inl, s)
-                 : launch_form<TP, TI, TV, CLS, false>(p, w0, w1, nullptr, 0, nullptr, s);
+  return seg_off ? launch_form<TP, TI, TV, TD, CLS, true>(p, w0, w1, seg_off, total, inl, s)
+                 : launch_form<TP, TI, TV, TD, CLS, false>(p, w0, w1, nullptr, 0, nullptr, s);
 }
 
 // Launchers of one value type: NAME(p, cls, w0, w1, seg_off, total, inl, s)
 // runs class `cls` over work entries [w0, w1) for the plan's index types
 // (`inl`: the entries of a segmented class passed by value, or null).
-#define FGX_BATCH_DEFINE_SPMM(NAME, TYPE)                                                   \
+#define FGX_BATCH_DEFINE_SPMM(NAME, TYPE, DTYPE)                                            \
   int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
            int64_t total, const SegInline* inl, hipStream_t s) {                            \
     using TV = TYPE;                                                                        \
+    using TD = DTYPE;                                                                       \
     int rc = set_error(FGX_EUNSUP, "batch: unit class %d", cls);                           \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo), {                                     \
       switch (cls) {                                                                        \
-        case kClsSpmv: rc = launch_class<TP, TI, TV, kClsSpmv>(p, w0, w1, seg_off, total, inl, s); break;        \
-        case 2: rc = launch_class<TP, TI, TV, 2>(p, w0, w1, seg_off, total, inl, s); break;                      \
-        case 4: rc = launch_class<TP, TI, TV, 4>(p, w0, w1, seg_off, total, inl, s); break;                      \
-        case 8: rc = launch_class<TP, TI, TV, 8>(p, w0, w1, seg_off, total, inl, s); break;                      \
-        case 16: rc = launch_class<TP, TI, TV, 16>(p, w0, w1, seg_off, total, inl, s); break;                    \
-        case 32: rc = launch_class<TP, TI, TV, 32>(p, w0, w1, seg_off, total, inl, s); break;                    \
-        case 64: rc = launch_class<TP, TI, TV, 64>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case kClsSpmv: rc = launch_class<TP, TI, TV, TD, kClsSpmv>(p, w0, w1, seg_off, total, inl, s); break;        \
+        case 2: rc = launch_class<TP, TI, TV, TD, 2>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 4: rc = launch_class<TP, TI, TV, TD, 4>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 8: rc = launch_class<TP, TI, TV, TD, 8>(p, w0, w1, seg_off, total, inl, s); break;                      \
+        case 16: rc = launch_class<TP, TI, TV, TD, 16>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case 32: rc = launch_class<TP, TI, TV, TD, 32>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case 64: rc = launch_class<TP, TI, TV, TD, 64>(p, w0, w1, seg_off, total, inl, s); break;                    \
         default: break;                                                                     \
       }                                                                                     \
     });                                                                                     \
@@ -328,14 +335,15 @@ int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_o
     return rc;                                                                              \
   }
 
-#define FGX_BATCH_DEFINE_SDDMM(NAME, TYPE)                                                  \
+#define FGX_BATCH_DEFINE_SDDMM(NAME, TYPE, DTYPE)                                           \
   int NAME(fgx_batch_plan* p, int cls, int64_t w0, int64_t w1, const int64_t* seg_off,     \
            int64_t total, const SegInline* inl, hipStream_t s) {                            \
     using TV = TYPE;                                                                        \
+    using TD = DTYPE;                                                                       \
     if (cls != kClsSddmm) return set_error(FGX_EUNSUP, "batch: unit class %d", cls);       \
     int rc = FGX_OK;                                                                        \
     FGX_DISPATCH_IDX(static_cast<IdxCombo>(p->combo),                                       \
-                     rc = (launch_class<TP, TI, TV, kClsSddmm>(p, w0, w1, seg_off, total, inl, s)));             \
+                     rc = (launch_class<TP, TI, TV, TD, kClsSddmm>(p, w0, w1, seg_off, total, inl, s)));             \
     return rc;                                                                              \
   }
 
@@ -346,5 +354,8 @@ int batch_spmm_f32(FGX_BATCH_LAUNCH_ARGS);
 int batch_sddmm_f32(FGX_BATCH_LAUNCH_ARGS);
 int batch_spmm_bf16(FGX_BATCH_LAUNCH_ARGS);
 int batch_sddmm_bf16(FGX_BATCH_LAUNCH_ARGS);
+// BF16 values with BF16 dense inputs (the serving loop's all-BF16 form).
+int batch_spmm_bf16_bf16in(FGX_BATCH_LAUNCH_ARGS);
+int batch_sddmm_bf16_bf16in(FGX_BATCH_LAUNCH_ARGS);
 
 }  // namespace fgx

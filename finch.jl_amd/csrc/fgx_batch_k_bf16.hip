@@ -2,5 +2,5 @@
 #include "fgx_batch_impl.hpp"
 
 namespace fgx {
-FGX_BATCH_DEFINE_SPMM(batch_spmm_bf16, bf16_t)
+FGX_BATCH_DEFINE_SPMM(batch_spmm_bf16, bf16_t, float)
 }  // namespace fgx

@@ -2,5 +2,5 @@
 #include "fgx_batch_impl.hpp"
 
 namespace fgx {
-FGX_BATCH_DEFINE_SDDMM(batch_sddmm_bf16, bf16_t)
+FGX_BATCH_DEFINE_SDDMM(batch_sddmm_bf16, bf16_t, float)
 }  // namespace fgx

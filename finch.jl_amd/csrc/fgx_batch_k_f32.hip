@@ -2,5 +2,5 @@
 #include "fgx_batch_impl.hpp"
 
 namespace fgx {
-FGX_BATCH_DEFINE_SPMM(batch_spmm_f32, float)
+FGX_BATCH_DEFINE_SPMM(batch_spmm_f32, float, float)
 }  // namespace fgx

@@ -2,5 +2,5 @@
 #include "fgx_batch_impl.hpp"
 
 namespace fgx {
-FGX_BATCH_DEFINE_SDDMM(batch_sddmm_f32, float)
+FGX_BATCH_DEFINE_SDDMM(batch_sddmm_f32, float, float)
 }  // namespace fgx

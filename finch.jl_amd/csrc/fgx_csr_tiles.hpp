@@ -18,6 +18,8 @@
 // Sums are therefore deterministic (no atomics) and independent of what
 // else runs on the device, which the serving path relies on.
 #pragma once
+#include <type_traits>
+
 #include "fgx_device.hpp"
 
 namespace fgx {
@@ -307,10 +309,10 @@ __device__ __forceinline__ void spmv_tile_walk_packed(
 // SpMV tile: lane-level runs of V positions.  Writes y for every row the tile
 // finalises and (carry_row, carry_val) of the row continuing into the next
 // tile (carry_row = -1 when no positions of this tile belong to it).
-template <typename TP, typename TI, typename TV, typename TA, int V>
+template <typename TP, typename TI, typename TV, typename TA, int V, typename TX = TA>
 __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                           const TV* __restrict__ val, int64_t base, int64_t m,
-                                          int64_t nnz, const TA* __restrict__ x,
+                                          int64_t nnz, const TX* __restrict__ x,
                                           TA* __restrict__ y, int64_t tile, bool vec,
                                           const int64_t* __restrict__ table, int64_t* lds_win,
                                           int64_t* carry_row_out, TA* carry_val_out) {
@@ -350,7 +352,7 @@ __device__ __forceinline__ void spmv_tile(const TP* __restrict__ ptr, const TI* 
     for (int s = 0; s < V; s++) {
       const bool in = a + s < b;
       const int64_t j = in ? static_cast<int64_t>(jj[s]) - base : 0;
-      xg[s] = x[j];
+      xg[s] = widen<TA>(x[j]);
     }
   } else {
 #pragma unroll
@@ -405,6 +407,66 @@ __device__ __forceinline__ void ld_dense(const TA* __restrict__ p, int valid_col
   for (int c = 0; c < KV; c++) o[c] = c < valid_cols ? p[c] : TA(0);
 }
 
+// A gather of KV dense input values kept in their storage form until use:
+// converting right after each load would put the conversion (and a wait for
+// that load) inside the load's branch, so consecutive gathers would no longer
+// be in flight together.  ld_raw loads, widen_raw converts.
+template <int KV, typename TD>
+struct DenseRaw {
+  TD v[KV];
+};
+template <int KV>
+struct DenseRaw<KV, bf16_t> {
+  uint32_t w[(KV + 1) / 2];  // two BF16 values per word, low half first
+};
+
+template <int KV, typename TD>
+__device__ __forceinline__ void ld_raw(const TD* __restrict__ p, int valid_cols,
+                                       DenseRaw<KV, TD>& o) {
+  if constexpr (std::is_same<TD, bf16_t>::value) {
+    if constexpr (KV == 4) {
+      if (valid_cols >= 4) {
+        const uint2 u = *reinterpret_cast<const uint2*>(p);
+        o.w[0] = u.x;
+        o.w[1] = u.y;
+        return;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < (KV + 1) / 2; c++) {
+      const uint32_t lo = 2 * c < valid_cols ? p[2 * c].bits : 0u;
+      const uint32_t hi = 2 * c + 1 < valid_cols ? p[2 * c + 1].bits : 0u;
+      o.w[c] = lo | (hi << 16);
+    }
+  } else {
+    ld_dense<KV, TD>(p, valid_cols, o.v);
+  }
+}
+
+template <int KV, typename TD>
+__device__ __forceinline__ void zero_raw(DenseRaw<KV, TD>& o) {
+  if constexpr (std::is_same<TD, bf16_t>::value) {
+#pragma unroll
+    for (int c = 0; c < (KV + 1) / 2; c++) o.w[c] = 0u;
+  } else {
+#pragma unroll
+    for (int c = 0; c < KV; c++) o.v[c] = TD(0);
+  }
+}
+
+template <int KV, typename TD, typename TA>
+__device__ __forceinline__ void widen_raw(const DenseRaw<KV, TD>& r, TA (&o)[KV]) {
+  if constexpr (std::is_same<TD, bf16_t>::value) {
+#pragma unroll
+    for (int c = 0; c < KV; c++)
+      o[c] = static_cast<TA>(__uint_as_float((c & 1) ? (r.w[c / 2] & 0xFFFF0000u)
+                                                     : (r.w[c / 2] << 16)));
+  } else {
+#pragma unroll
+    for (int c = 0; c < KV; c++) o[c] = static_cast<TA>(r.v[c]);
+  }
+}
+
 template <int KV, typename TA>
 __device__ __forceinline__ void st_dense(TA* __restrict__ p, int valid_cols, const TA (&o)[KV]) {
   if constexpr (KV == 4 && sizeof(TA) == 4) {
@@ -447,11 +509,12 @@ __device__ __forceinline__ void st_dense_nt(TA* __restrict__ p, int valid_cols, 
   st_dense<KV, TA>(p, valid_cols, o);
 }
 
-template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U,
+          typename TB = TA>
 __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                           const TV* __restrict__ val, int64_t base, int64_t m,
                                           int64_t nnz, int64_t k, int64_t kofs,
-                                          const TA* __restrict__ B, int64_t ldb,
+                                          const TB* __restrict__ B, int64_t ldb,
                                           TA* __restrict__ C, int64_t ldc, int64_t tile,
                                           const int64_t* __restrict__ table, int64_t* lds_win,
                                           int64_t* carry_row_out,
@@ -527,26 +590,25 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
   const TA* gv = lds_v + g * E;
 #pragma unroll 1
   for (int e0 = 0; e0 < E; e0 += U) {
-    TA bv[U][KV];
+    DenseRaw<KV, TB> bv[U];
     TA ve[U];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
       ve[u] = gv[e0 + u];
       const bool in = (a + e0 + u < b) && valid_cols > 0;
-      if (in) ld_dense<KV, TA>(B + je * ldb + col0, valid_cols, bv[u]);
-      else {
-#pragma unroll
-        for (int c = 0; c < KV; c++) bv[u][c] = 0;
-      }
+      if (in) ld_raw<KV, TB>(B + je * ldb + col0, valid_cols, bv[u]);
+      else zero_raw<KV, TB>(bv[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int64_t q = a + e0 + u;
       if (q < b) {
         while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
+        TA bw[KV];
+        widen_raw<KV, TB, TA>(bv[u], bw);
 #pragma unroll
-        for (int c = 0; c < KV; c++) acc[c] += ve[u] * bv[u][c];
+        for (int c = 0; c < KV; c++) acc[c] += ve[u] * bw[c];
       }
     }
   }

@@ -28,8 +28,11 @@ constexpr int kSdWin = 256;      // fibre ends cached per wave
 // by each lane directly: positions of one row share its address, so a wave
 // touches only the few rows its positions fall in.  Each lane then feeds its
 // X and Y values to the matrix core.
-__device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int64_t ldx,
-                                               const float* __restrict__ Y, int64_t ldy,
+// TD is the dense factors' storage type: float, or BF16 (read 8 bytes per
+// 4-column slice and widened before staging).
+template <typename TD>
+__device__ __forceinline__ void sddmm_sub_mfma(const TD* __restrict__ X, int64_t ldx,
+                                               const TD* __restrict__ Y, int64_t ldy,
                                                int64_t k, bool in, int64_t i, int64_t j,
                                                float (*ys)[kSdPitch], f32x4& c) {
   constexpr int kLanesPerRow = kSdChunk / 4;          // 16-byte slices per staged row
@@ -37,27 +40,37 @@ __device__ __forceinline__ void sddmm_sub_mfma(const float* __restrict__ X, int6
   const int lane = lane_id();
   const int part = lane % kLanesPerRow;
   for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
-    float4 yv[kLanesPerRow];
+    // Loads first (storage form), conversions after every load is issued.
+    DenseRaw<4, TD> yr[kLanesPerRow];
 #pragma unroll
     for (int s = 0; s < kLanesPerRow; s++) {
       const int t = lane / kLanesPerRow + kRowsPerLoad * s;
       const int64_t jt = shfl(j, t);
       const int tin = shfl(static_cast<int>(in), t);
       const int64_t col = c0 + 4 * part;
-      yv[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (tin && col < k) yv[s] = *reinterpret_cast<const float4*>(Y + jt * ldy + col);
+      if (tin && col < k) ld_raw<4, TD>(Y + jt * ldy + col, 4, yr[s]);
+      else zero_raw<4, TD>(yr[s]);
+    }
+    DenseRaw<4, TD> xr[kSdChunk / 4];
+#pragma unroll
+    for (int q = 0; q < kSdChunk / 4; q++) {
+      const int64_t col = c0 + 4 * q;
+      if (in && col < k) ld_raw<4, TD>(X + i * ldx + col, 4, xr[q]);
+      else zero_raw<4, TD>(xr[q]);
     }
     float4 xv[kSdChunk / 4];
 #pragma unroll
     for (int q = 0; q < kSdChunk / 4; q++) {
-      const int64_t col = c0 + 4 * q;
-      xv[q] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (in && col < k) xv[q] = *reinterpret_cast<const float4*>(X + i * ldx + col);
+      float o[4];
+      widen_raw<4, TD, float>(xr[q], o);
+      xv[q] = make_float4(o[0], o[1], o[2], o[3]);
     }
 #pragma unroll
     for (int s = 0; s < kLanesPerRow; s++) {
       const int t = lane / kLanesPerRow + kRowsPerLoad * s;
-      *reinterpret_cast<float4*>(&ys[t][4 * part]) = yv[s];
+      float o[4];
+      widen_raw<4, TD, float>(yr[s], o);
+      *reinterpret_cast<float4*>(&ys[t][4 * part]) = make_float4(o[0], o[1], o[2], o[3]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -101,11 +114,11 @@ __device__ __forceinline__ float mfma_diag(const f32x4& c) {
 // the matrix cores when `mfma`, else on the VALU in column order) and
 // stores out[q] = A[q] * dot.  Shared by the SDDMM kernel and the batch /
 // serving kernels, so a position's result is the same in either.
-template <typename TP, typename TI, typename TV>
+template <typename TP, typename TI, typename TV, typename TD = float>
 __device__ __forceinline__ void sddmm_tile(const TP* __restrict__ ptr, const TI* __restrict__ idx,
                                            const TV* __restrict__ val, int64_t base, int64_t m,
-                                           int64_t nnz, int64_t k, const float* __restrict__ X,
-                                           int64_t ldx, const float* __restrict__ Y, int64_t ldy,
+                                           int64_t nnz, int64_t k, const TD* __restrict__ X,
+                                           int64_t ldx, const TD* __restrict__ Y, int64_t ldy,
                                            float* __restrict__ out, int64_t tile,
                                            const int64_t* __restrict__ table, int64_t* lds_win,
                                            float (*ys)[kSdPitch], bool mfma) {
@@ -130,12 +143,12 @@ __device__ __forceinline__ void sddmm_tile(const TP* __restrict__ ptr, const TI*
     float dot = 0.f;
     if (mfma) {
       f32x4 c = {0.f, 0.f, 0.f, 0.f};
-      sddmm_sub_mfma(X, ldx, Y, ldy, k, in, i, j, ys, c);
+      sddmm_sub_mfma<TD>(X, ldx, Y, ldy, k, in, i, j, ys, c);
       dot = mfma_diag(c);
     } else if (in) {
-      const float* xr = X + i * ldx;
-      const float* yr = Y + j * ldy;
-      for (int64_t kk = 0; kk < k; kk++) dot += xr[kk] * yr[kk];
+      const TD* xr = X + i * ldx;
+      const TD* yr = Y + j * ldy;
+      for (int64_t kk = 0; kk < k; kk++) dot += widen<float>(xr[kk]) * widen<float>(yr[kk]);
     }
     if (in) __builtin_nontemporal_store(a * dot, out + q);
     if (q0 + (sub + 1) * kWave >= qend) break;

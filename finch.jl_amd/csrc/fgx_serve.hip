@@ -133,23 +133,25 @@ struct Sizes {
   int64_t footprint() const { return in + in2 + out; }
 };
 
-Sizes request_sizes(const fgx_request& r, const fgx_compressed& A) {
+// Bytes of a request's dense inputs (`dbytes` per element: 4 for FP32, 2
+// for BF16 inputs) and of its FP32 output.
+Sizes request_sizes(const fgx_request& r, const fgx_compressed& A, int dbytes) {
   Sizes z;
   const int64_t k = r.kind == FGX_REQ_SPMV ? 1 : r.k;
   if (r.kind == FGX_REQ_SDDMM) {
-    z.in = A.n_outer * k * 4;
-    z.in2 = A.n_inner * k * 4;
+    z.in = A.n_outer * k * dbytes;
+    z.in2 = A.n_inner * k * dbytes;
     z.out = A.nnz * 4;
   } else {
-    z.in = A.n_inner * k * 4;
+    z.in = A.n_inner * k * dbytes;
     z.out = A.n_outer * k * 4;
   }
   return z;
 }
 
 // Algorithmic bytes of one request (its operand read once, dense in/out once).
-double request_cost(const fgx_request& r, const fgx_compressed& A, int vbytes) {
-  const Sizes z = request_sizes(r, A);
+double request_cost(const fgx_request& r, const fgx_compressed& A, int vbytes, int dbytes) {
+  const Sizes z = request_sizes(r, A, dbytes);
   return static_cast<double>(A.nnz) * (vbytes + 4) + 4.0 * (A.n_outer + 1) +
          static_cast<double>(z.footprint());
 }
@@ -168,6 +170,13 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
     return set_error(FGX_EINVAL, "fgx_serve_run: NULL argument");
   if (n_operands <= 0 && n_reqs > 0) return set_error(FGX_EINVAL, "fgx_serve_run: no operands");
   if (cfg->mode != 0 && cfg->mode != 1) return set_error(FGX_EINVAL, "fgx_serve_run: bad mode");
+  // Dense inputs: FP32 (0 or FGX_F32), or BF16 with BF16 values in batched mode.
+  const int dense_type = cfg->input_type == 0 ? FGX_F32 : cfg->input_type;
+  if (dense_type != FGX_F32 && dense_type != FGX_BF16)
+    return set_error(FGX_EUNSUP, "fgx_serve_run: input type %d (F32 or BF16)", cfg->input_type);
+  if (dense_type == FGX_BF16 && cfg->mode != 0)
+    return set_error(FGX_EUNSUP, "fgx_serve_run: BF16 inputs in batched mode only");
+  const int dbytes = dense_type == FGX_BF16 ? 2 : 4;
   const int max_slots = std::max(1, cfg->max_slots);
   const int max_batch = std::max(1, cfg->max_batch);
   const int64_t budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : INT64_MAX;
@@ -182,6 +191,8 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       return set_error(FGX_EUNSUP, "fgx_serve_run: operands must share index and value types");
     if (operands[i].val_type != FGX_F32 && operands[i].val_type != FGX_BF16)
       return set_error(FGX_EUNSUP, "fgx_serve_run: values must be F32 or BF16");
+    if (dense_type == FGX_BF16 && operands[i].val_type != FGX_BF16)
+      return set_error(FGX_EUNSUP, "fgx_serve_run: BF16 inputs need BF16 values");
   }
   for (int64_t r = 0; r < n_reqs; r++) {
     const fgx_request& q = reqs[r];
@@ -193,6 +204,11 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
                                    "of 4 up to 256", (long long)r);
     if (!q.in || (q.kind == FGX_REQ_SDDMM && !q.in2))
       return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: NULL input", (long long)r);
+    // BF16 rows are read 8 bytes per 4-column slice.
+    if (dense_type == FGX_BF16 && ((reinterpret_cast<uintptr_t>(q.in) & 7u) ||
+                                   (reinterpret_cast<uintptr_t>(q.in2) & 7u)))
+      return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: BF16 inputs must be 8-byte "
+                                   "aligned", (long long)r);
   }
   FGX_HIP(hipSetDevice(ctx->device));
   const int vbytes = n_operands > 0 ? elem_size(operands[0].val_type) : 4;
@@ -250,7 +266,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       const int64_t nv = nt * (k == 1 ? 1 : group_for_k(k) * 4);
       top_t = std::max(top_t, nt);
       top_v = std::max(top_v, nv);
-      if (request_cost(q, A, vbytes) <= kLargeRequestBytes) need.push_back({nt, nv});
+      if (request_cost(q, A, vbytes, dbytes) <= kLargeRequestBytes) need.push_back({nt, nv});
     }
     std::sort(need.begin(), need.end(),
               [](const std::pair<int64_t, int64_t>& a, const std::pair<int64_t, int64_t>& b) {
@@ -273,7 +289,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
   for (int64_t r = 0; r < n_reqs; r++) {
     if (!reqs[r].out) {
       needs_arena = true;
-      max_out = std::max(max_out, request_sizes(reqs[r], operands[reqs[r].operand]).out);
+      max_out = std::max(max_out, request_sizes(reqs[r], operands[reqs[r].operand], dbytes).out);
     }
   }
   if (rc == FGX_OK && needs_arena) {
@@ -307,7 +323,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       const fgx_request& q = reqs[r];
       const fgx_compressed& A = operands[q.operand];
       while (now_s() < q.arrival_s) std::this_thread::yield();
-      const Sizes z = request_sizes(q, A);
+      const Sizes z = request_sizes(q, A, dbytes);
       if (z.footprint() > budget) {
         results[r].status = FGX_EADMIT;
         results[r].admit_s = results[r].done_s = now_s();
@@ -354,7 +370,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
         const double td = now_s();
         for (int64_t r : l.reqs) {
           results[r].done_s = td;
-          const Sizes z = request_sizes(reqs[r], operands[reqs[r].operand]);
+          const Sizes z = request_sizes(reqs[r], operands[reqs[r].operand], dbytes);
           inflight_bytes -= z.footprint();
           inflight_reqs--;
           if (!reqs[r].out) arena.release(outp[static_cast<size_t>(r)], static_cast<size_t>(z.out));
@@ -381,7 +397,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           const int64_t r = *it;
           const fgx_request& q = reqs[r];
           const fgx_compressed& A = operands[q.operand];
-          const Sizes z = request_sizes(q, A);
+          const Sizes z = request_sizes(q, A, dbytes);
           if (z.footprint() > budget || (!q.out && static_cast<size_t>(z.out) + 256 > arena.size())) {
             it = pending.erase(it);
             results[r].status = FGX_EADMIT;
@@ -390,7 +406,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
             continue;
           }
           if (inflight_bytes + z.footprint() > budget) break;
-          const double cost = request_cost(q, A, vbytes);
+          const double cost = request_cost(q, A, vbytes, dbytes);
           if (!l.reqs.empty() && batch_cost > kLaunchBytes) break;
           // A large request opens a launch; met later, it waits for the
           // next free stream and the requests behind it go ahead.
@@ -418,8 +434,8 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           d.ptr = A.ptr;
           d.idx = A.idx;
           d.val = A.val;
-          d.B = static_cast<const float*>(q.in);
-          d.B2 = static_cast<const float*>(q.in2);
+          d.B = q.in;
+          d.B2 = q.in2;
           d.C = static_cast<float*>(out);
           d.m = A.n_outer;
           d.n = A.n_inner;
@@ -461,6 +477,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
         }
         if (l.reqs.empty()) continue;
         l.plan.val_type = operands[0].val_type;
+        l.plan.dense_type = dense_type;
         l.plan.combo = combo;
         rc = batch_launch_staged(&l.plan, hdesc.data(), static_cast<int64_t>(hdesc.size()),
                                  hwork.data(), static_cast<int64_t>(hwork.size()), hfix.data(),

@@ -94,7 +94,7 @@ class ServeConfig(ctypes.Structure):
 
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("input_type", ctypes.c_int32)]
 
 
 class RequestResult(ctypes.Structure):

@@ -65,8 +65,17 @@ class Results:
 
 def serve(operands: list[Compressed], requests: list[Request], mode: str = "batched",
           hbm_budget_bytes: int = 200 * 10**9, max_slots: int = 4096,
-          max_batch: int = 512) -> Results:
+          max_batch: int = 512, input_dtype: torch.dtype = torch.float32) -> Results:
+    """input_dtype torch.bfloat16: the requests' dense inputs are BF16 (with
+    BF16-stored operand values, batched mode), read at half the bytes and
+    accumulated in FP32; outputs stay FP32."""
     lib = L.load()
+    in_type = {torch.float32: 0, torch.bfloat16: L.BF16}[input_dtype]
+    for r in requests:
+        for t in (r.inp, r.inp2):
+            if t is not None and t.dtype != input_dtype:
+                raise L.UnsupportedError(L.EUNSUP, f"request input is {t.dtype}, "
+                                                   f"expected {input_dtype}")
     dev = operands[0].device
     ops_arr = (L.Compressed * len(operands))(*[A.desc() for A in operands])
     req_arr = (L.Request * len(requests))()
@@ -74,7 +83,7 @@ def serve(operands: list[Compressed], requests: list[Request], mode: str = "batc
         req_arr[i] = L.Request(KINDS[r.kind], r.operand, r.k if r.kind != "spmv" else 1,
                                float(r.arrival_s), _ptr(r.inp), _ptr(r.inp2), _ptr(r.out))
     cfg = L.ServeConfig(int(hbm_budget_bytes), int(max_slots), int(max_batch),
-                        0 if mode == "batched" else 1, 0)
+                        0 if mode == "batched" else 1, in_type)
     res = (L.RequestResult * len(requests))()
     torch.cuda.synchronize(dev)
     L.check(lib.fgx_serve_run(context(dev).handle, ops_arr, len(operands), req_arr, len(requests),
@@ -85,11 +94,14 @@ def serve(operands: list[Compressed], requests: list[Request], mode: str = "batc
 
 
 class Inputs:
-    """Shared, device-resident dense inputs per (operand, kind, width)."""
+    """Shared, device-resident dense inputs per (operand, kind, width); with
+    dtype torch.bfloat16 they are the FP32 draws rounded to BF16 on the
+    device (fgx_convert)."""
 
-    def __init__(self, operands, seed=0):
+    def __init__(self, operands, seed=0, dtype=torch.float32):
         self.operands = operands
         self.seed = seed
+        self.dtype = dtype
         self._cache = {}
 
     def get(self, op, kind, k):
@@ -106,6 +118,9 @@ class Inputs:
             else:
                 t = (torch.randn((A.n_outer, k), device=A.device, generator=g),
                      torch.randn((A.n_inner, k), device=A.device, generator=g))
+            if self.dtype != torch.float32:
+                from .ops import convert
+                t = tuple(convert(u.contiguous(), self.dtype) for u in t)
             self._cache[key] = t
         return t
 

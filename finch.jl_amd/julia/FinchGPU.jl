@@ -960,7 +960,7 @@ struct FgxServeConfig         # struct fgx_serve_config
     max_slots::Int32
     max_batch::Int32
     mode::Int32               # 0: continuous batching, 1: plain per-call path
-    reserved::Int32
+    input_type::Int32         # dense inputs: 0 / FGX_F32, or FGX_BF16 (batched, BF16 values)
 end
 
 struct FgxRequestResult       # struct fgx_request_result
@@ -974,8 +974,9 @@ end
 (continuous batching with HBM admission); returns one result per request."""
 function serve(dev::GPU, operands::Vector{FgxCompressed}, reqs::Vector{FgxRequest};
                hbm_budget_bytes::Integer = 200 * 10^9, max_slots::Integer = 4096,
-               max_batch::Integer = 512, plain::Bool = false)
-    cfg = FgxServeConfig(hbm_budget_bytes, max_slots, max_batch, plain ? 1 : 0, 0)
+               max_batch::Integer = 512, plain::Bool = false, bf16_inputs::Bool = false)
+    cfg = FgxServeConfig(hbm_budget_bytes, max_slots, max_batch, plain ? 1 : 0,
+                         bf16_inputs ? FGX_BF16 : Int32(0))
     res = Vector{FgxRequestResult}(undef, length(reqs))
     check(ccall((:fgx_serve_run, libfgx), Cint,
                 (Ptr{Cvoid}, Ptr{FgxCompressed}, Int64, Ptr{FgxRequest}, Int64,

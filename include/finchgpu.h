@@ -327,7 +327,8 @@ typedef struct fgx_serve_config {
   int32_t max_slots;         /* max requests in flight */
   int32_t max_batch;         /* max requests fused into one launch */
   int32_t mode;              /* 0: continuous batching; 1: plain per-call path */
-  int32_t reserved;
+  int32_t input_type;        /* dense inputs: 0 or FGX_F32; FGX_BF16 (batched mode, BF16
+                                values; 8-byte aligned inputs) */
 } fgx_serve_config;
 
 typedef struct fgx_request_result {

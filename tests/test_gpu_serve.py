@@ -89,6 +89,46 @@ def test_bf16_operands_within_1e2(dev, operands):
         assert rel_err(r.out.cpu().numpy(), g.cpu().numpy()) <= 1e-2
 
 
+def test_bf16_values_and_inputs_within_1e2_of_fp64(dev, operands):
+    # The all-BF16 form: BF16 operand values and BF16 dense inputs, FP32
+    # accumulation and FP32 outputs.  Each result within 1e-2 (normwise) of
+    # the FP64 oracle on the unrounded FP32 inputs, within FP32 tolerance of
+    # the oracle on the BF16-rounded ones, and the same bits alone and in the mix.
+    in32 = serve.Inputs(operands, seed=1)
+    in16 = serve.Inputs(operands, seed=1, dtype=torch.bfloat16)
+    r32 = serve.make_trace(operands, in32, qps=50_000, n=200, seed=6, outputs=True)
+    r16 = serve.make_trace(operands, in16, qps=50_000, n=200, seed=6, outputs=True)
+    ops16 = [A.to_values(torch.bfloat16) for A in operands]
+    res = serve.serve(ops16, r16, "batched", max_batch=64, input_dtype=torch.bfloat16)
+    assert (res.status == 0).all()
+    assert res.summary()["launches"] < len(r16)
+    for a, b in zip(r32, r16):
+        assert (a.kind, a.operand, a.k) == (b.kind, b.operand, b.k)
+        assert rel_err(b.out.cpu().numpy(), reference(operands[a.operand], a)) <= 1e-2
+        # The same sums in FP64 on the BF16-rounded values and inputs.
+        r64 = serve.Request(b.kind, b.operand, b.k, 0.0, b.inp.float(),
+                            b.inp2.float() if b.inp2 is not None else None)
+        assert rel_err(b.out.cpu().numpy(), reference(ops16[b.operand], r64)) <= FP32_NORMWISE
+    mixed = [r.out.clone() for r in r16]
+    for i in range(0, len(r16), 15):
+        r = r16[i]
+        alone = serve.Request(r.kind, r.operand, r.k, 0.0, r.inp, r.inp2,
+                              torch.full_like(r.out, float("nan")))
+        one = serve.serve(ops16, [alone], "batched", input_dtype=torch.bfloat16)
+        assert (one.status == 0).all()
+        assert torch.equal(alone.out, mixed[i])
+
+
+def test_bf16_inputs_refused_where_unsupported(dev, operands):
+    in16 = serve.Inputs(operands, seed=1, dtype=torch.bfloat16)
+    r16 = serve.make_trace(operands, in16, qps=50_000, n=5, seed=7, outputs=True)
+    with pytest.raises(L.UnsupportedError):          # FP32 values with BF16 inputs
+        serve.serve(operands, r16, "batched", input_dtype=torch.bfloat16)
+    ops16 = [A.to_values(torch.bfloat16) for A in operands]
+    with pytest.raises(L.UnsupportedError):          # the plain path is FP32 only
+        serve.serve(ops16, r16, "plain", input_dtype=torch.bfloat16)
+
+
 def test_admission_budget_and_slots(dev, operands):
     inputs = serve.Inputs(operands, seed=1)
     reqs = serve.make_trace(operands, inputs, qps=50_000, n=80, seed=5, outputs=True)
