@@ -27,6 +27,8 @@
 // (exact for Int32 and min-plus; FP plus-times sums there use LDS atomics).
 #include "fgx_scan.hpp"
 
+#include <type_traits>
+
 namespace fgx {
 
 constexpr int kWaveCap = 256;    // products (and lists) per wave-team row
@@ -561,6 +563,15 @@ __device__ __forceinline__ void lds_accumulate(TV* p, TV v) {
   }
 }
 
+// acc = a*b + acc with the product rounded before the add, as the
+// reference's `w[i] = A[i,k]*B[k,j] + w[i]` without a fused multiply-add.
+template <typename TV>
+__device__ __forceinline__ void add_rounded_product(TV* acc, TV a, TV b) {
+#pragma clang fp contract(off)
+  const TV prod = a * b;
+  *acc = prod + *acc;
+}
+
 template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
 __global__ __launch_bounds__(256) void spgemm_window_kernel(
     const int64_t* __restrict__ rows, int64_t nrows, int64_t base, int64_t n_inner,
@@ -584,23 +595,49 @@ __global__ __launch_bounds__(256) void spgemm_window_kernel(
       acc[c] = sr_zero<TV, SR>();
     }
     __syncthreads();
-    for (int64_t q = a0 + tt; q < a1; q += 256) {
-      const int64_t k = static_cast<int64_t>(aidx[q]) - base;
-      const TV a = NUMERIC ? aval[q] : TV(0);
-      const int64_t b0 = pos_at(bptr, base, k), b1 = pos_at(bptr, base, k + 1);
-      int64_t lo = b0, hi = b1;
-      while (lo < hi) {
-        const int64_t mid = lo + ((hi - lo) >> 1);
-        if (static_cast<int64_t>(bidx[mid]) - base < w0) lo = mid + 1; else hi = mid;
+    if constexpr (NUMERIC && SR == FGX_PLUS_TIMES && !std::is_integral<TV>::value) {
+      // Floating-point sums in the reference's order (docs/examples/spgemm.jl
+      // :59-76): A positions ascending, then each B row's positions.  One A
+      // position per step, its B row's columns (distinct) spread over the
+      // block, so every column's terms are added in that order.
+      for (int64_t q = a0; q < a1; q++) {
+        const int64_t k = static_cast<int64_t>(aidx[q]) - base;
+        const TV a = aval[q];
+        const int64_t b0 = pos_at(bptr, base, k), b1 = pos_at(bptr, base, k + 1);
+        int64_t lo = b0, hi = b1;
+        while (lo < hi) {
+          const int64_t mid = lo + ((hi - lo) >> 1);
+          if (static_cast<int64_t>(bidx[mid]) - base < w0) lo = mid + 1; else hi = mid;
+        }
+        for (int64_t p = lo + tt; p < b1; p += 256) {
+          const int64_t j = static_cast<int64_t>(bidx[p]) - base;
+          if (j >= w1) break;
+          flag[j - w0] = 1;
+          add_rounded_product(&acc[j - w0], a, bval[p]);
+        }
+        __syncthreads();
       }
-      for (int64_t p = lo; p < b1; p++) {
-        const int64_t j = static_cast<int64_t>(bidx[p]) - base;
-        if (j >= w1) break;
-        flag[j - w0] = 1;
-        if constexpr (NUMERIC) lds_accumulate<TV, SR>(&acc[j - w0], sr_mul<TV, SR>(a, bval[p]));
+    } else {
+      // Counting, Int32 sums (wrapping adds commute) and min-plus (Julia's
+      // `min` is order-free): every thread takes A positions in parallel.
+      for (int64_t q = a0 + tt; q < a1; q += 256) {
+        const int64_t k = static_cast<int64_t>(aidx[q]) - base;
+        const TV a = NUMERIC ? aval[q] : TV(0);
+        const int64_t b0 = pos_at(bptr, base, k), b1 = pos_at(bptr, base, k + 1);
+        int64_t lo = b0, hi = b1;
+        while (lo < hi) {
+          const int64_t mid = lo + ((hi - lo) >> 1);
+          if (static_cast<int64_t>(bidx[mid]) - base < w0) lo = mid + 1; else hi = mid;
+        }
+        for (int64_t p = lo; p < b1; p++) {
+          const int64_t j = static_cast<int64_t>(bidx[p]) - base;
+          if (j >= w1) break;
+          flag[j - w0] = 1;
+          if constexpr (NUMERIC) lds_accumulate<TV, SR>(&acc[j - w0], sr_mul<TV, SR>(a, bval[p]));
+        }
       }
+      __syncthreads();
     }
-    __syncthreads();
     // Ordered emit of the window's touched columns.
     for (int c = 0; c < kWindow; c += 256) {
       const int j = c + tt;

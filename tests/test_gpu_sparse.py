@@ -262,12 +262,13 @@ def test_spgemm_all_bins_exact(dev):
 
 
 def test_spgemm_fp_plus_times_all_bins(dev):
-    # Wave/block bins fold in k order (bit-exact); the dense-window bin sums
-    # with LDS atomics, so FP sums there are held to the tolerance.
+    # Every bin folds each column's products in the reference's order (A
+    # positions ascending, then B positions), the dense window included, with
+    # each product rounded before its add: FP sums are bit-exact.
     ap, ai, av, m, l, bp, bi, bv, n = _skewed(4, np.float32, "randn")
-    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n, exact=False, tol=1e-5)
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n)
     ap, ai, av, m, l, bp, bi, bv, n = _skewed(5, np.float64, "randn")
-    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n, exact=False, tol=1e-12)
+    _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n)
 
 
 def test_spgemm_empty(dev):
