@@ -80,6 +80,11 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="time only SpMM, SpMV, SDDMM and the vendor comparison (for profiler "
                          "passes): no oracle errors, variants, CPU leg or other sections")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched-form section (FP32, BF16 values, BF16 values and "
+                         "inputs) and print its JSON (for per-class profiler passes)")
+    ap.add_argument("--no-loop", action="store_true",
+                    help="skip the batched section's per-matrix loop (for profiler passes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy B (256 MB) once: a kernel of known traffic that calibrates "
                          "rocprofv3 byte counters")
@@ -97,6 +102,10 @@ def main():
     from finchgpu import ops, synth
 
     dev = torch.device("cuda:0")
+    if args.batch_only:
+        print(json.dumps({"batched": bench_batched(dev, args.batch_count, not args.no_loop)}),
+              flush=True)
+        return
     t0 = time.time()
     ptr, idx, val = synth.powerlaw_csr(M, N, NNZ_TARGET, seed=0)
     nnz = int(idx.size)
@@ -594,7 +603,7 @@ def bench_fp64_10k(dev):
             "csc_ordered_spmv_us": round(csco * 1e3, 2), "csr_spmv_us": round(csr * 1e3, 2)}
 
 
-def bench_batched(dev, count):
+def bench_batched(dev, count, with_loop=True):
     """Batched form: `count` independent square CSR matrices (rows log-uniform in
     [256, 65536], Poisson nnz/row with mean log-uniform in [4, 64], n_rhs in
     {1, 8, 32, 128}): one autotuned persistent hipGraph launch vs a per-matrix
@@ -613,25 +622,55 @@ def bench_batched(dev, count):
     plan = batch.BatchPlan(items)
     plan_ms = _events_ms(plan.run, 5)
     ref = [it.C.clone() for it in items]
-    loop_ms = _events_ms(lambda: batch.run_loop(items), 3)
+    loop_ms = _events_ms(lambda: batch.run_loop(items), 3) if with_loop else None
     stats = plan.stats()
     plan.close()
+    den = sum(float(torch.linalg.vector_norm(r.double()) ** 2) for r in ref)
+
+    def err(its):
+        num = sum(float(torch.linalg.vector_norm((b.C - r).double()) ** 2)
+                  for b, r in zip(its, ref))
+        return float(f"{(num / max(den, 1e-300)) ** 0.5:.3e}")
+
+    # BF16 values (FP32 right-hand sides), then BF16 values and BF16 right-hand
+    # sides; both accumulate in FP32 and write FP32 C.
     items16 = [batch.BatchItem(it.A.to_values(torch.bfloat16), it.B) for it in items]
-    alg16 = alg - sum(it.A.nnz for it in items) * 2
+    alg16 = alg - nnz * 2
     plan16 = batch.BatchPlan(items16)
     plan16_ms = _events_ms(plan16.run, 5)
     plan16.close()
-    num = sum(float(torch.linalg.vector_norm((b.C - r).double()) ** 2) for b, r in zip(items16, ref))
-    den = sum(float(torch.linalg.vector_norm(r.double()) ** 2) for r in ref)
+    err16 = err(items16)
+    items16i = [batch.BatchItem(a.A, a.B.to(torch.bfloat16), a.C) for a in items16]
+    alg16i = alg16 - sum(it.A.n_inner * it.n_rhs for it in items) * 2
+    plan16i = batch.BatchPlan(items16i)
+    plan16i_ms = _events_ms(plan16i.run, 5)
+    plan16i.close()
+    err16i = err(items16i)
+    # Per class (one kernel per right-hand-side width): items, nnz and the
+    # algorithmic bytes of each variant, for per-class rooflines from a kernel trace.
+    classes = {}
+    for it in items:
+        c = classes.setdefault(str(it.n_rhs), {"items": 0, "nnz": 0, "alg_fp32": 0,
+                                               "alg_bf16_values": 0, "alg_bf16_values_inputs": 0})
+        a32 = algorithmic_bytes(it.A.n_outer, it.A.n_inner, it.A.nnz, it.n_rhs, "spmm")
+        c["items"] += 1
+        c["nnz"] += it.A.nnz
+        c["alg_fp32"] += a32
+        c["alg_bf16_values"] += a32 - 2 * it.A.nnz
+        c["alg_bf16_values_inputs"] += a32 - 2 * it.A.nnz - 2 * it.A.n_inner * it.n_rhs
     gbs = lambda b, ms: round(b / (ms * 1e-3) / 1e9, 1)
     out = {"workload": f"{count} CSR matrices, nnz={nnz}, FP32 values, Int32 idx",
            "plan_graph_ms": round(plan_ms, 3), "plan_graph_gbs": gbs(alg, plan_ms),
-           "loop_ms": round(loop_ms, 3), "loop_gbs": gbs(alg, loop_ms),
-           "speedup_vs_loop": round(loop_ms / plan_ms, 2),
+           "loop_ms": round(loop_ms, 3) if loop_ms else None,
+           "loop_gbs": gbs(alg, loop_ms) if loop_ms else None,
+           "speedup_vs_loop": round(loop_ms / plan_ms, 2) if loop_ms else None,
            "bf16_plan_ms": round(plan16_ms, 3), "bf16_plan_gbs": gbs(alg16, plan16_ms),
-           "bf16_normwise_err_vs_fp32": float(f"{(num / max(den, 1e-300)) ** 0.5:.3e}"),
-           "algorithmic_bytes": alg, "plan_stats": stats}
-    del items, items16, ref
+           "bf16_normwise_err_vs_fp32": err16,
+           "bf16_inputs_plan_ms": round(plan16i_ms, 3),
+           "bf16_inputs_plan_gbs": gbs(alg16i, plan16i_ms),
+           "bf16_inputs_normwise_err_vs_fp32": err16i,
+           "algorithmic_bytes": alg, "plan_stats": stats, "classes": classes}
+    del items, items16, items16i, ref
     torch.cuda.empty_cache()
     return out
 

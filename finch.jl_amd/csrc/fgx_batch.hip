@@ -493,6 +493,8 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   p->count = count;
   std::vector<BatchDesc> desc(static_cast<size_t>(count));
   IdxCombo combo = IDX_32_32;
+  const bool bf16_in = (flags & FGX_BATCH_BF16_INPUTS) != 0;
+  p->dense_type = bf16_in ? FGX_BF16 : FGX_F32;
   for (int64_t i = 0; i < count; i++) {
     const fgx_batch_item& it = items[i];
     int rc = check_compressed(&it.A, "fgx_batch_plan_create");
@@ -510,8 +512,14 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: n_rhs must be 1 or a multiple of 4 up to 256");
     if (rc == FGX_OK && (!it.C || (it.A.nnz > 0 && !it.B)))
       rc = set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL dense operand");
-    if (rc == FGX_OK && it.n_rhs > 1 && (!aligned16(it.B) || !aligned16(it.C)))
-      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned");
+    if (rc == FGX_OK && bf16_in && it.A.val_type != FGX_BF16)
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: BF16 inputs need BF16 values");
+    // F32 rows are read 16 bytes per 4-column slice, BF16 rows 8 bytes.
+    if (rc == FGX_OK && it.n_rhs > 1 &&
+        (!aligned16(it.C) || (bf16_in ? (reinterpret_cast<uintptr_t>(it.B) & 7u) != 0
+                                      : !aligned16(it.B))))
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned "
+                                 "(BF16 inputs 8-byte)");
     if (rc != FGX_OK) {
       delete p;
       return rc;
@@ -568,7 +576,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     BatchDesc& d = desc[static_cast<size_t>(i)];
     p->total_nnz += d.nnz;
     p->alg_bytes += d.nnz * (vbytes + ibytes) + (d.m + 1) * elem_size(items[i].A.ptr_type) +
-                    d.n * d.k * 4 + d.m * d.k * 4;
+                    d.n * d.k * (bf16_in ? 2 : 4) + d.m * d.k * 4;
     if (d.m == 0) continue;
     TunePick pk = autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags);
     if (pk.kind == BK_TILES) pk.unit = std::min(pk.unit, unit_cap);

@@ -24,7 +24,8 @@ from .ops import Compressed, _ptr, _stream, context
 
 
 class BatchItem:
-    """C = A * B for one operand (B: n_inner x n_rhs, C: n_outer x n_rhs, row-major FP32)."""
+    """C = A * B for one operand (B: n_inner x n_rhs, C: n_outer x n_rhs, row-major; C FP32,
+    B FP32, or BF16 when A's values are BF16)."""
 
     def __init__(self, A: Compressed, B: torch.Tensor, C: torch.Tensor | None = None):
         if B.dim() == 1:
@@ -37,6 +38,8 @@ class BatchItem:
             raise L.DimensionMismatch(L.EDIM, "batch item shapes do not match")
         if not (B.is_contiguous() and C.is_contiguous()):
             raise L.UnsupportedError(L.EUNSUP, "batch dense operands must be contiguous row-major")
+        if C.dtype != torch.float32 or B.dtype not in (torch.float32, torch.bfloat16):
+            raise L.UnsupportedError(L.EUNSUP, "batch: C must be FP32 and B FP32 or BF16")
         self.A, self.B, self.C = A, B, C
 
     @property
@@ -55,10 +58,18 @@ def _items_array(items):
 
 
 class BatchPlan:
-    """One autotuned persistent launch (replayed from a hipGraph) over all items."""
+    """One autotuned persistent launch (replayed from a hipGraph) over all items.
+
+    BF16 right-hand sides (every item's B, with BF16 values) select the
+    BF16-input kernels (``BATCH_BF16_INPUTS``); mixing FP32 and BF16 B raises."""
 
     def __init__(self, items, flags: int = 0):
         self.items = list(items)
+        bf16 = {it.B.dtype == torch.bfloat16 for it in self.items}
+        if len(bf16) > 1:
+            raise L.UnsupportedError(L.EUNSUP, "batch: items mix FP32 and BF16 right-hand sides")
+        if bf16 == {True}:
+            flags |= L.BATCH_BF16_INPUTS
         self.device = self.items[0].A.device if self.items else torch.device("cuda")
         self._ctx = context(self.device)
         self._arr = _items_array(self.items)
@@ -94,6 +105,8 @@ def run_loop(items):
     """Plain path: one fgx_spmv_csr / fgx_spmm_csr call per item."""
     if not items:
         return
+    if any(it.B.dtype != torch.float32 for it in items):
+        raise L.UnsupportedError(L.EUNSUP, "batch loop: right-hand sides must be FP32")
     arr = _items_array(items)
     dev = items[0].A.device
     L.check(L.load().fgx_batch_run_loop(context(dev).handle, arr, len(items), _stream(dev)),

@@ -904,11 +904,13 @@ struct FgxBatchItem           # struct fgx_batch_item
     C::Ptr{Cvoid}
 end
 
-const FGX_BATCH_FORCE_ROWSPLIT, FGX_BATCH_FORCE_NNZSPLIT, FGX_BATCH_NO_GRAPH = 1, 2, 4
+const FGX_BATCH_FORCE_ROWSPLIT, FGX_BATCH_FORCE_NNZSPLIT, FGX_BATCH_NO_GRAPH,
+      FGX_BATCH_BF16_INPUTS = 1, 2, 4, 8
 
 """Many independent `C = A*B` (CSR A, row-major F32 B/C with `n_rhs` columns)
 in one persistent launch, replayed from a hipGraph.  The items' buffers must
-outlive the plan."""
+outlive the plan.  With `flags = FGX_BATCH_BF16_INPUTS` every B is BF16
+(the items' values BF16 too); C stays F32."""
 mutable struct BatchPlan
     dev::GPU
     handle::Ptr{Cvoid}

@@ -29,7 +29,7 @@ def host_ref(item):
     A = item.A
     ptr, idx = A.ptr.cpu().numpy(), A.idx.cpu().numpy()
     val = A.val.float().cpu().numpy().astype(np.float64)
-    B = item.B.cpu().numpy().astype(np.float64)
+    B = item.B.float().cpu().numpy().astype(np.float64)
     if B.shape[1] == 1:
         return oracle.spmv_csr(ptr, idx, val, B[:, 0].copy())[:, None]
     return oracle.spmm_csr(ptr, idx, val, B)
@@ -131,3 +131,40 @@ def test_bf16_values(dev, small_batch):
         assert rel_err(b.C.cpu().numpy(), host_ref(b)) <= FP32_NORMWISE
     plan32.close()
     plan16.close()
+
+
+@pytest.mark.parametrize("flags", [0, L.BATCH_FORCE_NNZSPLIT, L.BATCH_FORCE_ROWSPLIT])
+def test_bf16_values_and_inputs(dev, small_batch, flags):
+    """BF16 values and BF16 right-hand sides, FP32 accumulation: every item equals
+    the FP32 oracle on the BF16-rounded operands, and stays within 1e-2 of the
+    FP32 plan."""
+    items = small_batch + skewed_items(dev)
+    plan32 = batch.BatchPlan(items)
+    plan32.run()
+    items16 = [batch.BatchItem(it.A.to_values(torch.bfloat16), it.B.to(torch.bfloat16))
+               for it in items]
+    plan16 = batch.BatchPlan(items16, flags)
+    plan16.run()
+    torch.cuda.synchronize()
+    for a, b in zip(items, items16):
+        assert rel_err(b.C.cpu().numpy(), host_ref(b)) <= FP32_NORMWISE
+        assert rel_err(b.C.cpu().numpy(), a.C.cpu().numpy()) <= 1e-2
+    first = [it.C.clone() for it in items16]
+    plan16.run()
+    torch.cuda.synchronize()
+    for it, f in zip(items16, first):
+        assert torch.equal(it.C, f)
+    plan32.close()
+    plan16.close()
+
+
+def test_bf16_inputs_refused_without_bf16_values(dev, small_batch):
+    it = small_batch[0]
+    with pytest.raises(L.UnsupportedError):
+        batch.BatchPlan([batch.BatchItem(it.A, it.B.to(torch.bfloat16))])
+    a16 = it.A.to_values(torch.bfloat16)
+    mixed = [batch.BatchItem(a16, it.B.to(torch.bfloat16)), batch.BatchItem(a16, it.B)]
+    with pytest.raises(L.UnsupportedError):
+        batch.BatchPlan(mixed)
+    with pytest.raises(L.UnsupportedError):
+        batch.run_loop([batch.BatchItem(a16, it.B.to(torch.bfloat16))])
