@@ -228,8 +228,16 @@ __device__ __forceinline__ void run_unit(const BatchDesc& d, int32_t kind, int64
 //     hardware can interleave the blocks of launches on different streams.
 // The two forms are separate instantiations (SEG), so the single-tile
 // segmented body is not held to the registers of the persistent unit loop.
+// Every instantiation is held to 128 VGPRs, 4 waves per SIMD: the FP32
+// persistent G=8 and G=16 bodies otherwise take 131-133 and run 3 (the few
+// values that no longer fit are loop invariants, spilled once and reloaded
+// per work unit).
+#ifndef FGX_BATCH_MIN_WAVES
+#define FGX_BATCH_MIN_WAVES 4
+#endif
 template <typename TP, typename TI, typename TV, typename TD, int CLS, bool SEG>
-__global__ __launch_bounds__(kBatchThreads) void batch_kernel(
+__global__ __launch_bounds__(kBatchThreads)
+__attribute__((amdgpu_waves_per_eu(FGX_BATCH_MIN_WAVES))) void batch_kernel(
     const BatchDesc* __restrict__ desc, const BatchWork* __restrict__ work, int64_t w0,
     int64_t w1, const int64_t* __restrict__ seg_off, int64_t total,
     int64_t* __restrict__ carry_row, float* __restrict__ carry_val, SegInline inl) {

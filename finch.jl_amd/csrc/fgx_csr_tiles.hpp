@@ -423,6 +423,27 @@ struct DenseRaw<KV, bf16_t> {
 template <int KV, typename TD>
 __device__ __forceinline__ void ld_raw(const TD* __restrict__ p, int valid_cols,
                                        DenseRaw<KV, TD>& o) {
+#ifdef FGX_GATHER_NT
+  // Measurement build only (cache-policy comparison): full 4-column slices
+  // gathered with the non-temporal (streaming) policy instead of the default.
+  if constexpr (KV == 4 && std::is_same<TD, float>::value) {
+    if (valid_cols >= 4) {
+      typedef float f4n __attribute__((ext_vector_type(4)));
+      const f4n t = __builtin_nontemporal_load(reinterpret_cast<const f4n*>(p));
+      o.v[0] = t[0]; o.v[1] = t[1]; o.v[2] = t[2]; o.v[3] = t[3];
+      return;
+    }
+  }
+  if constexpr (KV == 4 && std::is_same<TD, bf16_t>::value) {
+    if (valid_cols >= 4) {
+      typedef unsigned u2n __attribute__((ext_vector_type(2)));
+      const u2n u = __builtin_nontemporal_load(reinterpret_cast<const u2n*>(p));
+      o.w[0] = u[0];
+      o.w[1] = u[1];
+      return;
+    }
+  }
+#endif
   if constexpr (std::is_same<TD, bf16_t>::value) {
     if constexpr (KV == 4) {
       if (valid_cols >= 4) {

@@ -39,6 +39,57 @@ __device__ __forceinline__ void sddmm_sub_mfma(const TD* __restrict__ X, int64_t
   constexpr int kRowsPerLoad = kWave / kLanesPerRow;   // rows one wave load covers
   const int lane = lane_id();
   const int part = lane % kLanesPerRow;
+#ifdef FGX_SDDMM_LDS_DIRECT
+  // Measurement build only (staging comparison): FP32 Y rows go straight from
+  // memory to LDS with global_load_lds_dwordx4 (no VGPR round trip, no LDS
+  // write instructions).  Each load writes lane l's 16 bytes at byte 16*l of
+  // its 1 KB block, so rows are packed 128 bytes apart; lane l of row t loads
+  // slice (l % 8) ^ (t % 8), and the read-back of slice q of row t finds it at
+  // slot q ^ (t % 8), which spreads eight consecutive rows' reads over all
+  // banks.  Only when every 32-column chunk is full (no zero padding needed).
+  if constexpr (std::is_same<TD, float>::value) {
+    if (k % kSdChunk == 0) {
+      float* flat = &ys[0][0];
+      for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
+        float4 xv[kSdChunk / 4];
+#pragma unroll
+        for (int q = 0; q < kSdChunk / 4; q++)
+          xv[q] = in ? *reinterpret_cast<const float4*>(X + i * ldx + c0 + 4 * q)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int s = 0; s < kLanesPerRow; s++) {
+          const int t = lane / kLanesPerRow + kRowsPerLoad * s;
+          const int64_t jt = shfl(j, t);
+          const int tin = shfl(static_cast<int>(in), t);
+          const int slice = part ^ (t % kLanesPerRow);
+          if (tin)
+            __builtin_amdgcn_global_load_lds(
+                static_cast<const void*>(Y + jt * ldy + c0 + 4 * slice),
+                (__attribute__((address_space(3))) void*)(flat + s * kWave * 4),
+                16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int q = 0; q < kSdChunk / 4; q++) {
+          const float4 a4 = xv[q];
+          const float4 b4 = *reinterpret_cast<const float4*>(
+              flat + lane * kSdChunk + 4 * (q ^ (lane % kLanesPerRow)));
+          c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.x, b4.x, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.y, b4.y, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.z, b4.z, c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_4x4x1f32(a4.w, b4.w, c, 0, 0, 0);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      return;
+    }
+  }
+#endif
   for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
     // Loads first (storage form), conversions after every load is issued.
     DenseRaw<4, TD> yr[kLanesPerRow];
