@@ -252,7 +252,11 @@ def test_powerlaw_1m_spmv_spmm_sddmm(dev, powerlaw_1m):
     A = operand(ptr, idx, val, m, n, dev, torch.int32)
     nth = max(1, min(16, oracle.max_threads()))
     x = synth.dense(n, 1, seed=1)[:, 0].copy()
-    y = ops.spmv_csr(A, up(x, dev)).cpu().numpy()
+    yd = ops.spmv_csr(A, up(x, dev))
+    # The entry point that allocates nothing (table and carries in the
+    # library's static array) gives the same bits at full size.
+    assert torch.equal(yd, ops.spmv_csr_noalloc(A, up(x, dev), torch.empty_like(yd)))
+    y = yd.cpu().numpy()
     ref = oracle.spmv_csr(ptr, idx, val.astype(np.float64), x.astype(np.float64), nthreads=nth)
     assert rel_err(y, ref) <= FP32_NORMWISE
     B = synth.dense(n, 64, seed=2)
