@@ -14,14 +14,14 @@ namespace fgx {
 // nullptr): one wave per unit, over the unit's chunks.  Segmented form:
 // entries are whole requests' tile ranges, one wave per chunk.
 __device__ __forceinline__ int item_fix_width(const BatchDesc& d) {
-  return d.k == 1 ? 1 : group_for_k(d.k) * 4;
+  return fix_width_for_k(d.k, d.cpl);
 }
 
 __device__ __forceinline__ void fixup_item_chunk(const BatchDesc& d, int64_t t0, int64_t t1,
                                                  const int64_t* __restrict__ carry_row,
                                                  const float* __restrict__ carry_val) {
   const int width = item_fix_width(d);
-  const int64_t W = tile_width_for_k(d.k);
+  const int64_t W = tile_width_for_k(d.k, d.cpl);
   const int64_t ntiles = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
   const int64_t ld = d.k == 1 ? 1 : d.ldc;
   fixup_chunk<float>(carry_row + d.carry_tile0, carry_val + d.carry_val0, ntiles, t0, t1, width,
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
       for (int64_t e = 1; e < inl.n; e++) u += inl.seg[e] <= c ? 1 : 0;
       // Inline fix-up entries cover whole requests: tiles [0, ntiles).
       const BatchDesc& d = inl.desc[u];
-      const int64_t W = tile_width_for_k(d.k);
+      const int64_t W = tile_width_for_k(d.k, d.cpl);
       const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
       fixup_entry_chunk(d, 0, nt, c - inl.seg[u], carry_row, carry_val);
       return;
@@ -307,7 +307,7 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
     seg.push_back(facc);
     // One wave per chunk of the entry's tiles.
     const BatchDesc& fd = h_desc[h_fix[u].item];
-    const int64_t ct = fixup_chunk_tiles(fd.k == 1 ? 1 : group_for_k(fd.k) * 4);
+    const int64_t ct = fixup_chunk_tiles(fix_width_for_k(fd.k, fd.cpl));
     facc += (h_fix[u].end - h_fix[u].begin + ct - 1) / ct;
   }
   seg.push_back(facc);
@@ -443,7 +443,7 @@ int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* ou
 // and row-length statistics.  Units aim at ~256 KB of traffic so that
 // thousands of them spread over the device.
 TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
-                              int flags) {
+                              int flags, int cpl) {
   const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
   // Measured on MI355X (graph replays of 64-256 same-class items, row-split
   // time / tile time): row-split wins only for wide dense operands (k >= 128,
@@ -459,12 +459,12 @@ TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int v
   TunePick pk;
   if (tiles) {
     const double tile_bytes =
-        static_cast<double>(tile_width_for_k(k)) * (4.0 + vbytes + 4.0 * static_cast<double>(k));
+        static_cast<double>(tile_width_for_k(k, cpl)) * (4.0 + vbytes + 4.0 * static_cast<double>(k));
     pk.kind = BK_TILES;
     pk.unit = std::max<int64_t>(1, static_cast<int64_t>(target / tile_bytes));
   } else {
     const double row_bytes = mean * (4.0 + vbytes + 4.0 * static_cast<double>(k)) + 4.0 * k + 4.0;
-    const int64_t per_iter = k == 1 ? kWave / kSpmvRowGroup : kWave / group_for_k(k);
+    const int64_t per_iter = k == 1 ? kWave / kSpmvRowGroup : kWave / group_for_k(k, cpl);
     int64_t rows = static_cast<int64_t>(target / std::max(row_bytes, 1.0));
     rows = std::max<int64_t>(per_iter, (rows / per_iter) * per_iter);
     pk.kind = BK_ROWS;
@@ -495,6 +495,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   IdxCombo combo = IDX_32_32;
   const bool bf16_in = (flags & FGX_BATCH_BF16_INPUTS) != 0;
   p->dense_type = bf16_in ? FGX_BF16 : FGX_F32;
+  const int cpl = cols_per_lane(p->dense_type);
   for (int64_t i = 0; i < count; i++) {
     const fgx_batch_item& it = items[i];
     int rc = check_compressed(&it.A, "fgx_batch_plan_create");
@@ -514,12 +515,12 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       rc = set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL dense operand");
     if (rc == FGX_OK && bf16_in && it.A.val_type != FGX_BF16)
       rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: BF16 inputs need BF16 values");
-    // F32 rows are read 16 bytes per 4-column slice, BF16 rows 8 bytes.
-    if (rc == FGX_OK && it.n_rhs > 1 &&
-        (!aligned16(it.C) || (bf16_in ? (reinterpret_cast<uintptr_t>(it.B) & 7u) != 0
-                                      : !aligned16(it.B))))
-      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned "
-                                 "(BF16 inputs 8-byte)");
+    // Rows are read 16 bytes per lane: 4 F32 or 8 BF16 columns.
+    if (rc == FGX_OK && bf16_in && it.n_rhs > 1 && it.n_rhs % 8 != 0)
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: BF16 inputs need n_rhs 1 or a multiple "
+                                 "of 8");
+    if (rc == FGX_OK && it.n_rhs > 1 && (!aligned16(it.C) || !aligned16(it.B)))
+      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned");
     if (rc != FGX_OK) {
       delete p;
       return rc;
@@ -542,6 +543,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     d.op = BO_SPMM;
     d.vec = aligned16(it.A.val) && aligned16(it.A.idx);
     d.mfma = 0;
+    d.cpl = cpl;
     d.table = nullptr;
   }
   p->combo = combo;
@@ -567,7 +569,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   int64_t total_tiles = 0;
   for (int64_t i = 0; i < count; i++) {
     const BatchDesc& d = desc[static_cast<size_t>(i)];
-    const int64_t W = tile_width_for_k(d.k);
+    const int64_t W = tile_width_for_k(d.k, cpl);
     total_tiles += d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
   }
   const int64_t unit_cap = std::max<int64_t>(
@@ -578,12 +580,13 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     p->alg_bytes += d.nnz * (vbytes + ibytes) + (d.m + 1) * elem_size(items[i].A.ptr_type) +
                     d.n * d.k * (bf16_in ? 2 : 4) + d.m * d.k * 4;
     if (d.m == 0) continue;
-    TunePick pk = autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags);
+    TunePick pk =
+        autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags, cpl);
     if (pk.kind == BK_TILES) pk.unit = std::min(pk.unit, unit_cap);
     if (pk.kind == BK_TILES) {
-      const int64_t W = tile_width_for_k(d.k);
+      const int64_t W = tile_width_for_k(d.k, cpl);
       const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
-      const int width = d.k == 1 ? 1 : group_for_k(d.k) * 4;
+      const int width = fix_width_for_k(d.k, cpl);
       d.carry_tile0 = carry_tiles;
       d.carry_val0 = carry_vals;
       carry_tiles += nt;
@@ -604,7 +607,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   for (int64_t i = 0; i < count; i++) {
     const BatchDesc& d = desc[static_cast<size_t>(i)];
     if (!is_tiles[static_cast<size_t>(i)]) continue;
-    const int64_t W = tile_width_for_k(d.k);
+    const int64_t W = tile_width_for_k(d.k, cpl);
     table_off[static_cast<size_t>(i)] = table_entries;
     table_entries += (d.nnz > 0 ? (d.nnz + W - 1) / W : 1) + 1;
   }
@@ -621,7 +624,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       if (off < 0) continue;
       BatchDesc& d = desc[static_cast<size_t>(i)];
       d.table = p->d_tables + off;
-      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k), s, p->d_tables + off);
+      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k, cpl), s, p->d_tables + off);
     }
   }
   if (rc == FGX_OK)

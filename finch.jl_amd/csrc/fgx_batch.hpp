@@ -38,7 +38,7 @@ struct BatchDesc {
   int32_t vec;     // 16-byte aligned ptr/idx/val runs (SpMV tiles)
   int32_t mfma;    // SDDMM on matrix cores (k % 4 == 0, aligned rows)
   int32_t kind;    // segmented launches: the BatchKind of this entry
-  int32_t pad;
+  int32_t cpl;     // dense columns per lane of SpMM: 4 (FP32 rows), 8 (BF16 rows)
 };
 
 // Entries of a segmented launch passed by value as a kernel argument (when a
@@ -62,13 +62,17 @@ constexpr int kBatchThreads = 256;
 constexpr int kSpmvRowGroup = 8;   // lanes per row, row-split SpMV
 
 // Lanes per row (row-split) or per position (TILES) for a dense width k:
-// 16-byte column slices, rounded up to a power of two.
-__host__ __device__ inline int group_for_k(int64_t k) {
-  const int64_t lanes = (k + 3) / 4;
+// 16-byte column slices of cpl columns (4 FP32 or 8 BF16 values), rounded up
+// to a power of two.
+__host__ __device__ inline int group_for_k(int64_t k, int cpl = 4) {
+  const int64_t lanes = (k + cpl - 1) / cpl;
   int g = 2;
   while (g < lanes && g < 64) g <<= 1;
   return g;
 }
+
+// Dense columns per lane for a dense input type.
+__host__ __device__ inline int cols_per_lane(int dense_type) { return dense_type == FGX_BF16 ? 8 : 4; }
 
 
 
@@ -80,7 +84,7 @@ constexpr int kClsSpmv = 1;
 constexpr int kMaxClasses = 8;
 
 __host__ __device__ inline int unit_class(const BatchDesc& d) {
-  return d.op == BO_SDDMM ? kClsSddmm : (d.k == 1 ? kClsSpmv : group_for_k(d.k));
+  return d.op == BO_SDDMM ? kClsSddmm : (d.k == 1 ? kClsSpmv : group_for_k(d.k, d.cpl));
 }
 
 // Positions per lane run of a batched SpMV (width 1) tile.
@@ -90,10 +94,15 @@ __host__ __device__ inline int unit_class(const BatchDesc& d) {
 constexpr int kBatchSpmvRun = FGX_BATCH_SPMV_RUN;
 
 // Positions per TILES tile for a width k.
-__host__ __device__ inline int64_t tile_width_for_k(int64_t k) {
+__host__ __device__ inline int64_t tile_width_for_k(int64_t k, int cpl = 4) {
   if (k == 1) return kWave * kBatchSpmvRun;
-  const int g = group_for_k(k);
+  const int g = group_for_k(k, cpl);
   return static_cast<int64_t>(kWave / g) * tile_e_for_g(g);
+}
+
+// Carry values per tile of a width-k item: one per column slot of a group.
+__host__ __device__ inline int fix_width_for_k(int64_t k, int cpl = 4) {
+  return k == 1 ? 1 : group_for_k(k, cpl) * cpl;
 }
 
 }  // namespace fgx
@@ -155,7 +164,8 @@ struct TunePick {
   int kind;
   int64_t unit;
 };
-TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes, int flags);
+TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes, int flags,
+                       int cpl = 4);
 // Uploads host descriptor/work arrays into p's (grown) device buffers on
 // stream s and launches; the previous launch on p must have finished.
 int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,

@@ -59,6 +59,11 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
   }
 }
 
+// Dense columns per lane of the SpMM bodies: one 16-byte slice of a B row,
+// 4 FP32 or 8 BF16 values (cols_per_lane on the host).
+template <typename TD>
+constexpr int spmm_kv() { return std::is_same<TD, bf16_t>::value ? 8 : 4; }
+
 // SpMM: G lanes per row, each lane a 16-byte slice of the dense width.  A
 // row is walked in chunks of G positions (lane gl loads position c + gl),
 // broadcast within the group, with U rows of B gathered at a time; the sum
@@ -66,6 +71,7 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
 template <int G, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_t r1) {
   constexpr int U = G < 8 ? G : 8;
+  constexpr int KV = spmm_kv<TD>();
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
   const int gbase = g * G;
@@ -75,38 +81,40 @@ __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_
   const int64_t base = d.base;
   const TD* Bd = static_cast<const TD*>(d.B);
   float* Cd = static_cast<float*>(d.C);
-  const int64_t col0 = static_cast<int64_t>(gl) * 4;
-  const int vc = static_cast<int>(col0 < d.k ? (d.k - col0 < 4 ? d.k - col0 : 4) : 0);
+  const int64_t col0 = static_cast<int64_t>(gl) * KV;
+  const int vc = static_cast<int>(col0 < d.k ? (d.k - col0 < KV ? d.k - col0 : KV) : 0);
   for (int64_t r = r0 + g; r < r1; r += kWave / G) {
     const int64_t q0 = pos_at(ptr, base, r), q1 = pos_at(ptr, base, r + 1);
-    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    float acc[KV];
+#pragma unroll
+    for (int cc = 0; cc < KV; cc++) acc[cc] = 0.f;
     for (int64_t c = q0; c < q1; c += G) {
       const bool inc = c + gl < q1;
       const int64_t jc = inc ? static_cast<int64_t>(ld_stream(idx + c + gl)) - base : 0;
       const float vcv = inc ? widen<float>(ld_stream(val + c + gl)) : 0.f;
       const int cnt = static_cast<int>(q1 - c < G ? q1 - c : G);
       for (int e0 = 0; e0 < cnt; e0 += U) {
-        DenseRaw<4, TD> bv[U];
+        DenseRaw<KV, TD> bv[U];
         float ve[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int src = gbase + ((e0 + u) % G);
           const int64_t je = shfl(jc, src);
           ve[u] = shfl(vcv, src);
-          if (e0 + u < cnt && vc > 0) ld_raw<4, TD>(Bd + je * d.ldb + col0, vc, bv[u]);
-          else zero_raw<4, TD>(bv[u]);
+          if (e0 + u < cnt && vc > 0) ld_raw<KV, TD>(Bd + je * d.ldb + col0, vc, bv[u]);
+          else zero_raw<KV, TD>(bv[u]);
         }
 #pragma unroll
         for (int u = 0; u < U; u++)
           if (e0 + u < cnt) {
-            float bw[4];
-            widen_raw<4, TD, float>(bv[u], bw);
+            float bw[KV];
+            widen_raw<KV, TD, float>(bv[u], bw);
 #pragma unroll
-            for (int cc = 0; cc < 4; cc++) acc[cc] += ve[u] * bw[cc];
+            for (int cc = 0; cc < KV; cc++) acc[cc] += ve[u] * bw[cc];
           }
       }
     }
-    if (vc > 0) st_dense<4, float>(Cd + r * d.ldc + col0, vc, acc);
+    if (vc > 0) st_dense<KV, float>(Cd + r * d.ldc + col0, vc, acc);
   }
 }
 
@@ -137,16 +145,17 @@ template <int G, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
   constexpr int U = G < 8 ? G : 8;
+  constexpr int KV = spmm_kv<TD>();
   constexpr int W = tile_w_for_g<G>();
   static_assert(sizeof(TI) + sizeof(float) <= 12, "TILES staging holds 12 bytes per position");
   TI* lj = reinterpret_cast<TI*>(lds + kWinCap);
   float* lv = reinterpret_cast<float*>(lj + W);
   for (int64_t t = t0; t < t1; t++)
-    spmm_tile<TP, TI, TV, float, G, 4, tile_e_for_g(G), U, TD>(
+    spmm_tile<TP, TI, TV, float, G, KV, tile_e_for_g(G), U, TD>(
         static_cast<const TP*>(d.ptr), static_cast<const TI*>(d.idx),
         static_cast<const TV*>(d.val), d.base, d.m, d.nnz, d.k, 0, static_cast<const TD*>(d.B),
         d.ldb, static_cast<float*>(d.C), d.ldc, t,
-        d.table, lds, carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * 4),
+        d.table, lds, carry_row + d.carry_tile0 + t, carry_val + d.carry_val0 + t * (G * KV),
         lj, lv);
 }
 

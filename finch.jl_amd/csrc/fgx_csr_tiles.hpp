@@ -453,6 +453,16 @@ __device__ __forceinline__ void ld_raw(const TD* __restrict__ p, int valid_cols,
         return;
       }
     }
+    if constexpr (KV == 8) {
+      if (valid_cols >= 8) {
+        const uint4 u = *reinterpret_cast<const uint4*>(p);
+        o.w[0] = u.x;
+        o.w[1] = u.y;
+        o.w[2] = u.z;
+        o.w[3] = u.w;
+        return;
+      }
+    }
 #pragma unroll
     for (int c = 0; c < (KV + 1) / 2; c++) {
       const uint32_t lo = 2 * c < valid_cols ? p[2 * c].bits : 0u;
@@ -496,6 +506,13 @@ __device__ __forceinline__ void st_dense(TA* __restrict__ p, int valid_cols, con
       return;
     }
   }
+  if constexpr (KV == 8 && sizeof(TA) == 4) {
+    if (valid_cols >= 8) {
+      *reinterpret_cast<float4*>(p) = make_float4(o[0], o[1], o[2], o[3]);
+      *reinterpret_cast<float4*>(p + 4) = make_float4(o[4], o[5], o[6], o[7]);
+      return;
+    }
+  }
   if constexpr (KV == 2 && sizeof(TA) == 8) {
     if (valid_cols >= 2) {
       *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
@@ -516,6 +533,16 @@ __device__ __forceinline__ void st_dense_nt(TA* __restrict__ p, int valid_cols, 
       typedef float f4v __attribute__((ext_vector_type(4)));
       f4v t = {o[0], o[1], o[2], o[3]};
       __builtin_nontemporal_store(t, reinterpret_cast<f4v*>(p));
+      return;
+    }
+  }
+  if constexpr (KV == 8 && sizeof(TA) == 4) {
+    if (valid_cols >= 8) {
+      typedef float f4v __attribute__((ext_vector_type(4)));
+      f4v t0 = {o[0], o[1], o[2], o[3]};
+      f4v t1 = {o[4], o[5], o[6], o[7]};
+      __builtin_nontemporal_store(t0, reinterpret_cast<f4v*>(p));
+      __builtin_nontemporal_store(t1, reinterpret_cast<f4v*>(p + 4));
       return;
     }
   }

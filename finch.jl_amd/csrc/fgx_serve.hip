@@ -204,14 +204,22 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
                                    "of 4 up to 256", (long long)r);
     if (!q.in || (q.kind == FGX_REQ_SDDMM && !q.in2))
       return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: NULL input", (long long)r);
-    // BF16 rows are read 8 bytes per 4-column slice.
-    if (dense_type == FGX_BF16 && ((reinterpret_cast<uintptr_t>(q.in) & 7u) ||
-                                   (reinterpret_cast<uintptr_t>(q.in2) & 7u)))
+    // SpMM rows are read and written 16 bytes per lane (4 FP32 or 8 BF16
+    // columns); BF16 SDDMM rows are read 8 bytes per 4 columns.
+    if (q.kind == FGX_REQ_SPMM && (!aligned16(q.in) || !aligned16(q.out)))
+      return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: SpMM rows must be 16-byte "
+                                   "aligned", (long long)r);
+    if (dense_type == FGX_BF16 && q.kind == FGX_REQ_SPMM && k % 8 != 0)
+      return set_error(FGX_EUNSUP, "fgx_serve_run: request %lld: BF16 inputs need SpMM widths "
+                                   "that are multiples of 8", (long long)r);
+    if (dense_type == FGX_BF16 && q.kind == FGX_REQ_SDDMM &&
+        ((reinterpret_cast<uintptr_t>(q.in) & 7u) || (reinterpret_cast<uintptr_t>(q.in2) & 7u)))
       return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: BF16 inputs must be 8-byte "
                                    "aligned", (long long)r);
   }
   FGX_HIP(hipSetDevice(ctx->device));
   const int vbytes = n_operands > 0 ? elem_size(operands[0].val_type) : 4;
+  const int cpl = cols_per_lane(dense_type);
 
   // ---- set-up (untimed): streams, operand statistics, tile tables, arena ----
   std::vector<Launch> L(kServeStreams);
@@ -236,8 +244,9 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       int64_t W = 0;
       if (q.kind == FGX_REQ_SDDMM) {
         W = kWave * kSddmmSub;
-      } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0).kind == BK_TILES) {
-        W = tile_width_for_k(k);
+      } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0, cpl).kind ==
+                 BK_TILES) {
+        W = tile_width_for_k(k, cpl);
       }
       if (W > 0 && !oi.table_for(W)) {
         int64_t* t = nullptr;
@@ -259,11 +268,13 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       if (q.kind == FGX_REQ_SDDMM) continue;
       const fgx_compressed& A = operands[q.operand];
       const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
-      if (autotune_pick(A.n_outer, A.nnz, k, maxrow[static_cast<size_t>(q.operand)], vbytes, 0)
+      if (autotune_pick(A.n_outer, A.nnz, k, maxrow[static_cast<size_t>(q.operand)], vbytes, 0,
+                        cpl)
               .kind != BK_TILES)
         continue;
-      const int64_t nt = A.nnz > 0 ? (A.nnz + tile_width_for_k(k) - 1) / tile_width_for_k(k) : 1;
-      const int64_t nv = nt * (k == 1 ? 1 : group_for_k(k) * 4);
+      const int64_t W = tile_width_for_k(k, cpl);
+      const int64_t nt = A.nnz > 0 ? (A.nnz + W - 1) / W : 1;
+      const int64_t nv = nt * fix_width_for_k(k, cpl);
       top_t = std::max(top_t, nt);
       top_v = std::max(top_v, nv);
       if (request_cost(q, A, vbytes, dbytes) <= kLargeRequestBytes) need.push_back({nt, nv});
@@ -445,6 +456,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           d.ldc = k;
           d.base = A.index_base;
           d.vec = aligned16(A.val) && aligned16(A.idx);
+          d.cpl = cpl;
           const int32_t item = static_cast<int32_t>(hdesc.size());
           if (q.kind == FGX_REQ_SDDMM) {
             d.op = BO_SDDMM;
@@ -456,11 +468,11 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
             hwork.push_back(BatchWork{item, BK_SDDMM, 0, (A.nnz + W - 1) / W});
           } else {
             d.op = BO_SPMM;
-            const TunePick pk = autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0);
+            const TunePick pk = autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0, cpl);
             if (pk.kind == BK_TILES) {
-              const int64_t W = tile_width_for_k(k);
+              const int64_t W = tile_width_for_k(k, cpl);
               const int64_t nt = A.nnz > 0 ? (A.nnz + W - 1) / W : 1;
-              const int width = k == 1 ? 1 : group_for_k(k) * 4;
+              const int width = fix_width_for_k(k, cpl);
               d.table = oi.table_for(W);
               d.carry_tile0 = carry_tiles;
               d.carry_val0 = carry_vals;

@@ -168,3 +168,7 @@ def test_bf16_inputs_refused_without_bf16_values(dev, small_batch):
         batch.BatchPlan(mixed)
     with pytest.raises(L.UnsupportedError):
         batch.run_loop([batch.BatchItem(a16, it.B.to(torch.bfloat16))])
+    # BF16 rows are read 8 columns per lane: n_rhs must be 1 or a multiple of 8.
+    B12 = torch.zeros((it.A.n_inner, 12), dtype=torch.bfloat16, device=dev)
+    with pytest.raises(L.UnsupportedError):
+        batch.BatchPlan([batch.BatchItem(a16, B12)])

@@ -127,6 +127,13 @@ def test_bf16_inputs_refused_where_unsupported(dev, operands):
     ops16 = [A.to_values(torch.bfloat16) for A in operands]
     with pytest.raises(L.UnsupportedError):          # the plain path is FP32 only
         serve.serve(ops16, r16, "plain", input_dtype=torch.bfloat16)
+    # BF16 rows are read 8 columns per lane: SpMM widths must be multiples of 8.
+    A = ops16[0]
+    odd = serve.Request("spmm", 0, 12, 0.0,
+                        torch.zeros((A.n_inner, 12), dtype=torch.bfloat16, device=dev),
+                        None, torch.empty((A.n_outer, 12), device=dev))
+    with pytest.raises(L.UnsupportedError):
+        serve.serve(ops16, [odd], "batched", input_dtype=torch.bfloat16)
 
 
 def test_admission_budget_and_slots(dev, operands):
