@@ -403,7 +403,9 @@ def bench_cpu(oracle, ptr, idx, val, Bh, nnz, by_spmm, by_spmv, seconds):
     the whole matrix.  Each figure is the median of at least 10 warm
     repetitions, at 1 thread and at every available thread, with Finch's
     static fld blocks (the `parallel` default) and with its greedy schedule
-    at its default chunk of 1 row."""
+    at its default chunk of 1 row; CSC SpMV with AtomicElement adds and with a
+    Mutex per y element (the Mutex leg, 16M lock round trips, is the median
+    of 3)."""
     import numpy as np
 
     nproc = os.cpu_count()
@@ -438,6 +440,8 @@ def bench_cpu(oracle, ptr, idx, val, Bh, nnz, by_spmm, by_spmv, seconds):
     res["all_threads"]["spmv_greedy_chunk1_ms"] = round(g * 1e3, 2)
     c = med(lambda: oracle.spmv_csc_atomic(M, ptr, idx, val, xh, nthreads=threads))
     res["all_threads"]["csc_atomic_spmv_ms"] = round(c * 1e3, 2)
+    c = med(lambda: oracle.spmv_csc_mutex(M, ptr, idx, val, xh, nthreads=threads), reps=3)
+    res["all_threads"]["csc_mutex_spmv_ms"] = round(c * 1e3, 2)
     allt = res["all_threads"]
     return {"value": allt["spmm_gbs"], "unit": "GB/s", "cores": threads, "kind": "port",
             "nproc": nproc,

@@ -165,6 +165,42 @@ DEF_SPMV_CSC_ATOMIC(double, f64)
 DEF_SPMV_CSC_ATOMIC(float, f32)
 
 /* ---------------------------------------------------------------------------
+ * CSC SpMV with a Mutex output level (benchmark/benchmarks.jl:414-426:
+ * y = Tensor(Dense(Mutex(Element(0.0))))): the same static column blocks,
+ * and each update of y[i] is done under y[i]'s own lock, taken before the
+ * read and released after the write (src/tensors/levels/mutex_levels.jl:
+ * 304-352, aquire_lock! / release_lock! around the assignment).
+ * ------------------------------------------------------------------------- */
+#define DEF_SPMV_CSC_MUTEX(T, SFX)                                                       \
+  void or_spmv_csc_mutex_##SFX(i64 n_inner, i64 n_outer, const i64* ptr, const i64* idx, \
+                               const T* val, int base, const T* x, T* y, int nthreads) {  \
+    omp_lock_t* locks = (omp_lock_t*)malloc(sizeof(omp_lock_t) * (size_t)(n_inner > 0 ? n_inner : 1)); \
+    for (i64 i = 0; i < n_inner; i++) {                                                  \
+      y[i] = (T)0;                                                                       \
+      omp_init_lock(&locks[i]);                                                          \
+    }                                                                                    \
+    if (nthreads < 1) nthreads = 1;                                                      \
+    _Pragma("omp parallel for schedule(static, 1) num_threads(nthreads)")               \
+    for (int tid = 1; tid <= nthreads; tid++) {                                          \
+      i64 j0 = fld(n_outer * (tid - 1), nthreads);                                       \
+      i64 j1 = fld(n_outer * tid, nthreads);                                             \
+      for (i64 j = j0; j < j1; j++) {                                                    \
+        const T xj = x[j];                                                               \
+        for (i64 q = ptr[j] - base; q < ptr[j + 1] - base; q++) {                        \
+          const i64 i = idx[q] - base;                                                   \
+          omp_set_lock(&locks[i]);                                                       \
+          y[i] = xj * val[q] + y[i];                                                     \
+          omp_unset_lock(&locks[i]);                                                     \
+        }                                                                                \
+      }                                                                                  \
+    }                                                                                    \
+    for (i64 i = 0; i < n_inner; i++) omp_destroy_lock(&locks[i]);                       \
+    free(locks);                                                                         \
+  }
+DEF_SPMV_CSC_MUTEX(double, f64)
+DEF_SPMV_CSC_MUTEX(float, f32)
+
+/* ---------------------------------------------------------------------------
  * Greedy schedule of the threaded CSR loops (src/architecture.jl:721-767):
  * threads repeatedly take the next chunk of `chk` rows from a shared atomic
  * counter until the rows run out; each row is summed as in the static

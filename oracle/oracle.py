@@ -88,6 +88,20 @@ def spmv_csc_atomic(n_inner, ptr, idx, val, x, base=1, nthreads=1):
     return y
 
 
+def spmv_csc_mutex(n_inner, ptr, idx, val, x, base=1, nthreads=1):
+    """y = A*x, A column-compressed, columns split over threads, each update
+    of y[i] under y[i]'s lock (benchmark/benchmarks.jl:414-426, Mutex)."""
+    val = np.ascontiguousarray(val)
+    s = _sfx(val.dtype)
+    x = np.ascontiguousarray(x, dtype=val.dtype)
+    ptr, idx = _i64(ptr), _i64(idx)
+    y = np.empty(n_inner, dtype=val.dtype)
+    f = getattr(lib(), f"or_spmv_csc_mutex_{s}")
+    f(_I64(n_inner), _I64(len(ptr) - 1), _p(ptr), _p(idx), _p(val), _INT(base), _p(x), _p(y),
+      _INT(nthreads))
+    return y
+
+
 def spmv_csr_greedy(ptr, idx, val, x, chunk, base=1, nthreads=1):
     """CSR SpMV with Finch's greedy schedule (src/architecture.jl:721-767)."""
     val = np.ascontiguousarray(val)

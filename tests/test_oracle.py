@@ -434,6 +434,29 @@ def test_spmv_csc_atomic_golden_and_scipy(goldens):
     assert np.linalg.norm(y8 - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+def test_spmv_csc_mutex_golden_and_scipy(goldens):
+    # benchmark/benchmarks.jl:414-426 (Mutex(Element) y, parallel columns):
+    # the golden bit for bit at any thread count, the serial loop's bits at
+    # one thread, and scipy at eight.
+    g = goldens["spmv_csc"]
+    ptr, idx, val = dense_to_csc(np.array(g["A_dense"]))
+    for nth in (1, 2, 3):
+        y = oracle.spmv_csc_mutex(4, ptr, idx, val, np.array(g["x"], np.float64), nthreads=nth)
+        assert y.tolist() == g["y"]
+    rng = np.random.default_rng(53)
+    S = sp.random(3000, 2000, density=0.01, format="csc", random_state=rng)
+    x = rng.standard_normal(2000)
+    ref = S @ x
+    serial = oracle.spmv_csc(3000, S.indptr + 1, S.indices + 1, S.data, x)
+    assert np.array_equal(oracle.spmv_csc_mutex(3000, S.indptr + 1, S.indices + 1, S.data, x,
+                                                nthreads=1), serial)
+    y8 = oracle.spmv_csc_mutex(3000, S.indptr + 1, S.indices + 1, S.data, x, nthreads=8)
+    assert np.linalg.norm(y8 - ref) <= 1e-12 * np.linalg.norm(ref)
+    y32 = oracle.spmv_csc_mutex(3000, S.indptr + 1, S.indices + 1, S.data.astype(np.float32),
+                                x.astype(np.float32), nthreads=4)
+    assert np.linalg.norm(y32 - ref) <= 1e-5 * np.linalg.norm(ref)
+
+
 @pytest.mark.parametrize("chunk", [1, 7, 1000])
 @pytest.mark.parametrize("nth", [1, 3, 8])
 def test_greedy_schedule_equals_serial_bit_for_bit(chunk, nth):
