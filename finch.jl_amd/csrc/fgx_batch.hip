@@ -195,7 +195,7 @@ static int dev_alloc_copy(T** out, const T* src, int64_t n, hipStream_t s) {
   return FGX_OK;
 }
 
-void order_by_class(fgx_batch_plan* p, const BatchDesc* desc, BatchWork* work, int64_t n) {
+int order_by_class(fgx_batch_plan* p, const BatchDesc* desc, BatchWork* work, int64_t n) {
   std::stable_sort(work, work + n, [&](const BatchWork& a, const BatchWork& b) {
     return unit_class(desc[a.item]) < unit_class(desc[b.item]);
   });
@@ -204,19 +204,22 @@ void order_by_class(fgx_batch_plan* p, const BatchDesc* desc, BatchWork* work, i
     const int cls = unit_class(desc[work[u].item]);
     int64_t v = u;
     while (v < n && unit_class(desc[work[v].item]) == cls) v++;
+    if (p->n_cls >= kMaxClasses)
+      return set_error(FGX_EUNSUP, "batch: more than %d unit classes", kMaxClasses);
     p->cls_id[p->n_cls] = cls;
     p->cls_w0[p->n_cls] = u;
     p->cls_w1[p->n_cls] = v;
     p->n_cls++;
     u = v;
   }
+  return FGX_OK;
 }
 
 int batch_upload(fgx_batch_plan* p, const BatchDesc* desc, int64_t n_desc, const BatchWork* work_in,
                  int64_t n_work, int64_t carry_tiles, int64_t carry_vals, int num_cus,
                  hipStream_t s) {
   std::vector<BatchWork> work(work_in, work_in + n_work);
-  order_by_class(p, desc, work.data(), n_work);
+  FGX_TRY(order_by_class(p, desc, work.data(), n_work));
   std::vector<BatchWork> fix;
   for (int64_t u = 0; u < n_work; u++)
     if (work[static_cast<size_t>(u)].kind == BK_TILES) fix.push_back(work[static_cast<size_t>(u)]);
@@ -255,7 +258,7 @@ int batch_launch_staged(fgx_batch_plan* p, const BatchDesc* h_desc, int64_t nd,
                         hipStream_t s) {
   FGX_TRY(grow_device(p, &p->d_carry_row, &p->cap_crow, carry_tiles));
   FGX_TRY(grow_device(p, &p->d_carry_val, &p->cap_cval, carry_vals));
-  order_by_class(p, h_desc, h_work, nw);
+  FGX_TRY(order_by_class(p, h_desc, h_work, nw));
   p->segmented = true;
   // Descriptors in entry order (entry u's at u, with its kind), so a wave
   // reads its descriptor with one load; fix-up entries follow the renumbering.
@@ -495,7 +498,6 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   IdxCombo combo = IDX_32_32;
   const bool bf16_in = (flags & FGX_BATCH_BF16_INPUTS) != 0;
   p->dense_type = bf16_in ? FGX_BF16 : FGX_F32;
-  const int cpl = cols_per_lane(p->dense_type);
   for (int64_t i = 0; i < count; i++) {
     const fgx_batch_item& it = items[i];
     int rc = check_compressed(&it.A, "fgx_batch_plan_create");
@@ -515,10 +517,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       rc = set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL dense operand");
     if (rc == FGX_OK && bf16_in && it.A.val_type != FGX_BF16)
       rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: BF16 inputs need BF16 values");
-    // Rows are read 16 bytes per lane: 4 F32 or 8 BF16 columns.
-    if (rc == FGX_OK && bf16_in && it.n_rhs > 1 && it.n_rhs % 8 != 0)
-      rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: BF16 inputs need n_rhs 1 or a multiple "
-                                 "of 8");
+    // Rows are read up to 32 bytes per lane (cols_per_lane_for).
     if (rc == FGX_OK && it.n_rhs > 1 && (!aligned16(it.C) || !aligned16(it.B)))
       rc = set_error(FGX_EUNSUP, "fgx_batch_plan_create: dense operands must be 16-byte aligned");
     if (rc != FGX_OK) {
@@ -543,7 +542,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     d.op = BO_SPMM;
     d.vec = aligned16(it.A.val) && aligned16(it.A.idx);
     d.mfma = 0;
-    d.cpl = cpl;
+    d.cpl = cols_per_lane_for(it.n_rhs);
     d.table = nullptr;
   }
   p->combo = combo;
@@ -569,7 +568,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   int64_t total_tiles = 0;
   for (int64_t i = 0; i < count; i++) {
     const BatchDesc& d = desc[static_cast<size_t>(i)];
-    const int64_t W = tile_width_for_k(d.k, cpl);
+    const int64_t W = tile_width_for_k(d.k, d.cpl);
     total_tiles += d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
   }
   const int64_t unit_cap = std::max<int64_t>(
@@ -581,12 +580,12 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
                     d.n * d.k * (bf16_in ? 2 : 4) + d.m * d.k * 4;
     if (d.m == 0) continue;
     TunePick pk =
-        autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags, cpl);
+        autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags, d.cpl);
     if (pk.kind == BK_TILES) pk.unit = std::min(pk.unit, unit_cap);
     if (pk.kind == BK_TILES) {
-      const int64_t W = tile_width_for_k(d.k, cpl);
+      const int64_t W = tile_width_for_k(d.k, d.cpl);
       const int64_t nt = d.nnz > 0 ? (d.nnz + W - 1) / W : 1;
-      const int width = fix_width_for_k(d.k, cpl);
+      const int width = fix_width_for_k(d.k, d.cpl);
       d.carry_tile0 = carry_tiles;
       d.carry_val0 = carry_vals;
       carry_tiles += nt;
@@ -607,7 +606,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   for (int64_t i = 0; i < count; i++) {
     const BatchDesc& d = desc[static_cast<size_t>(i)];
     if (!is_tiles[static_cast<size_t>(i)]) continue;
-    const int64_t W = tile_width_for_k(d.k, cpl);
+    const int64_t W = tile_width_for_k(d.k, d.cpl);
     table_off[static_cast<size_t>(i)] = table_entries;
     table_entries += (d.nnz > 0 ? (d.nnz + W - 1) / W : 1) + 1;
   }
@@ -624,7 +623,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
       if (off < 0) continue;
       BatchDesc& d = desc[static_cast<size_t>(i)];
       d.table = p->d_tables + off;
-      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k, cpl), s, p->d_tables + off);
+      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k, d.cpl), s, p->d_tables + off);
     }
   }
   if (rc == FGX_OK)

@@ -59,19 +59,20 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
   }
 }
 
-// Dense columns per lane of the SpMM bodies: one 16-byte slice of a B row,
-// 4 FP32 or 8 BF16 values (cols_per_lane on the host).
-template <typename TD>
-constexpr int spmm_kv() { return std::is_same<TD, bf16_t>::value ? 8 : 4; }
+// Gathers in flight per lane: 8 slices of up to 16 bytes, or 4 of 32 bytes
+// (8 FP32 columns per lane), fewer when the group is narrower.
+template <int G, int KV, typename TD>
+constexpr int spmm_u() {
+  return KV * static_cast<int>(sizeof(TD)) > 16 ? (G < 4 ? G : 4) : (G < 8 ? G : 8);
+}
 
-// SpMM: G lanes per row, each lane a 16-byte slice of the dense width.  A
-// row is walked in chunks of G positions (lane gl loads position c + gl),
-// broadcast within the group, with U rows of B gathered at a time; the sum
-// runs over the row's positions in order.
-template <int G, typename TP, typename TI, typename TV, typename TD>
+// SpMM: G lanes per row, each lane KV columns of the dense width.  A row is
+// walked in chunks of G positions (lane gl loads position c + gl), broadcast
+// within the group, with U rows of B gathered at a time; the sum runs over
+// the row's positions in order.
+template <int G, int KV, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void rows_spmm(const BatchDesc& d, int64_t r0, int64_t r1) {
-  constexpr int U = G < 8 ? G : 8;
-  constexpr int KV = spmm_kv<TD>();
+  constexpr int U = spmm_u<G, KV, TD>();
   const int lane = lane_id();
   const int g = lane / G, gl = lane % G;
   const int gbase = g * G;
@@ -141,11 +142,10 @@ constexpr int tile_w_for_g() { return (kWave / G) * tile_e_for_g(G); }
 
 // The tile's coordinates and values are staged in LDS after the fibre window
 // (`lds` + kWinCap entries), as in the single-operand SpMM kernel.
-template <int G, typename TP, typename TI, typename TV, typename TD>
+template <int G, int KV, typename TP, typename TI, typename TV, typename TD>
 __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64_t t1,
                                            int64_t* lds, int64_t* carry_row, float* carry_val) {
-  constexpr int U = G < 8 ? G : 8;
-  constexpr int KV = spmm_kv<TD>();
+  constexpr int U = spmm_u<G, KV, TD>();
   constexpr int W = tile_w_for_g<G>();
   static_assert(sizeof(TI) + sizeof(float) <= 12, "TILES staging holds 12 bytes per position");
   TI* lj = reinterpret_cast<TI*>(lds + kWinCap);
@@ -190,7 +190,7 @@ constexpr int lds_bytes_for() {
              ? kSdWin * static_cast<int>(sizeof(int64_t)) + kWave * kSdPitch * 4
              : (CLS == kClsSpmv ? kWinCap * static_cast<int>(sizeof(int64_t))
                                 : kWinCap * static_cast<int>(sizeof(int64_t)) +
-                                      tile_w_for_g<CLS>() * 12);
+                                      tile_w_for_g<cls_g(CLS)>() * 12);
 }
 
 // Largest s in [0, n) with seg_off[s] <= t (seg_off nondecreasing,
@@ -221,8 +221,8 @@ __device__ __forceinline__ void run_unit(const BatchDesc& d, int32_t kind, int64
     if (kind == BK_ROWS) rows_spmv<TP, TI, TV, TD>(d, b, e);
     else tiles_spmv<TP, TI, TV, TD>(d, b, e, lds, carry_row, carry_val);
   } else {
-    if (kind == BK_ROWS) rows_spmm<CLS, TP, TI, TV, TD>(d, b, e);
-    else tiles_spmm<CLS, TP, TI, TV, TD>(d, b, e, lds, carry_row, carry_val);
+    if (kind == BK_ROWS) rows_spmm<cls_g(CLS), cls_kv(CLS), TP, TI, TV, TD>(d, b, e);
+    else tiles_spmm<cls_g(CLS), cls_kv(CLS), TP, TI, TV, TD>(d, b, e, lds, carry_row, carry_val);
   }
 }
 
@@ -345,6 +345,11 @@ int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_o
         case 16: rc = launch_class<TP, TI, TV, TD, 16>(p, w0, w1, seg_off, total, inl, s); break;                    \
         case 32: rc = launch_class<TP, TI, TV, TD, 32>(p, w0, w1, seg_off, total, inl, s); break;                    \
         case 64: rc = launch_class<TP, TI, TV, TD, 64>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case kClsWide | 2: rc = launch_class<TP, TI, TV, TD, kClsWide | 2>(p, w0, w1, seg_off, total, inl, s); break;   \
+        case kClsWide | 4: rc = launch_class<TP, TI, TV, TD, kClsWide | 4>(p, w0, w1, seg_off, total, inl, s); break;   \
+        case kClsWide | 8: rc = launch_class<TP, TI, TV, TD, kClsWide | 8>(p, w0, w1, seg_off, total, inl, s); break;   \
+        case kClsWide | 16: rc = launch_class<TP, TI, TV, TD, kClsWide | 16>(p, w0, w1, seg_off, total, inl, s); break; \
+        case kClsWide | 32: rc = launch_class<TP, TI, TV, TD, kClsWide | 32>(p, w0, w1, seg_off, total, inl, s); break; \
         default: break;                                                                     \
       }                                                                                     \
     });                                                                                     \

@@ -396,6 +396,15 @@ __device__ __forceinline__ void ld_dense(const TA* __restrict__ p, int valid_col
       return;
     }
   }
+  if constexpr (KV == 8 && sizeof(TA) == 4) {
+    if (valid_cols >= 8) {
+      float4 t0 = *reinterpret_cast<const float4*>(p);
+      float4 t1 = *reinterpret_cast<const float4*>(p + 4);
+      o[0] = t0.x; o[1] = t0.y; o[2] = t0.z; o[3] = t0.w;
+      o[4] = t1.x; o[5] = t1.y; o[6] = t1.z; o[7] = t1.w;
+      return;
+    }
+  }
   if constexpr (KV == 2 && sizeof(TA) == 8) {
     if (valid_cols >= 2) {
       double2 t = *reinterpret_cast<const double2*>(p);

@@ -209,9 +209,6 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
     if (q.kind == FGX_REQ_SPMM && (!aligned16(q.in) || !aligned16(q.out)))
       return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: SpMM rows must be 16-byte "
                                    "aligned", (long long)r);
-    if (dense_type == FGX_BF16 && q.kind == FGX_REQ_SPMM && k % 8 != 0)
-      return set_error(FGX_EUNSUP, "fgx_serve_run: request %lld: BF16 inputs need SpMM widths "
-                                   "that are multiples of 8", (long long)r);
     if (dense_type == FGX_BF16 && q.kind == FGX_REQ_SDDMM &&
         ((reinterpret_cast<uintptr_t>(q.in) & 7u) || (reinterpret_cast<uintptr_t>(q.in2) & 7u)))
       return set_error(FGX_EINVAL, "fgx_serve_run: request %lld: BF16 inputs must be 8-byte "
@@ -219,7 +216,6 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
   }
   FGX_HIP(hipSetDevice(ctx->device));
   const int vbytes = n_operands > 0 ? elem_size(operands[0].val_type) : 4;
-  const int cpl = cols_per_lane(dense_type);
 
   // ---- set-up (untimed): streams, operand statistics, tile tables, arena ----
   std::vector<Launch> L(kServeStreams);
@@ -244,9 +240,9 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       int64_t W = 0;
       if (q.kind == FGX_REQ_SDDMM) {
         W = kWave * kSddmmSub;
-      } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0, cpl).kind ==
-                 BK_TILES) {
-        W = tile_width_for_k(k, cpl);
+      } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0,
+                               cols_per_lane_for(k)).kind == BK_TILES) {
+        W = tile_width_for_k(k, cols_per_lane_for(k));
       }
       if (W > 0 && !oi.table_for(W)) {
         int64_t* t = nullptr;
@@ -268,6 +264,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       if (q.kind == FGX_REQ_SDDMM) continue;
       const fgx_compressed& A = operands[q.operand];
       const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
+      const int cpl = cols_per_lane_for(k);
       if (autotune_pick(A.n_outer, A.nnz, k, maxrow[static_cast<size_t>(q.operand)], vbytes, 0,
                         cpl)
               .kind != BK_TILES)
@@ -456,6 +453,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           d.ldc = k;
           d.base = A.index_base;
           d.vec = aligned16(A.val) && aligned16(A.idx);
+          const int cpl = cols_per_lane_for(k);
           d.cpl = cpl;
           const int32_t item = static_cast<int32_t>(hdesc.size());
           if (q.kind == FGX_REQ_SDDMM) {

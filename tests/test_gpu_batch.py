@@ -168,7 +168,25 @@ def test_bf16_inputs_refused_without_bf16_values(dev, small_batch):
         batch.BatchPlan(mixed)
     with pytest.raises(L.UnsupportedError):
         batch.run_loop([batch.BatchItem(a16, it.B.to(torch.bfloat16))])
-    # BF16 rows are read 8 columns per lane: n_rhs must be 1 or a multiple of 8.
-    B12 = torch.zeros((it.A.n_inner, 12), dtype=torch.bfloat16, device=dev)
-    with pytest.raises(L.UnsupportedError):
-        batch.BatchPlan([batch.BatchItem(a16, B12)])
+
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_widths_of_four_and_eight_columns_per_lane(dev, bf16):
+    """Widths of 16 and more that are multiples of 8 read 8 columns per lane,
+    the others 4 (k = 20: 8 per lane with a 4-column tail lane): every item
+    equals the oracle, on both the nnz-split and the row-split bodies."""
+    items = []
+    for i, k in enumerate((4, 12, 16, 20, 24, 40, 136)):
+        A = batch.synth_matrix(700 + 37 * i, 30.0, 200 + i, dev)
+        if bf16:
+            A = A.to_values(torch.bfloat16)
+        B = torch.randn((A.n_inner, k), device=dev)
+        items.append(batch.BatchItem(A, B.to(torch.bfloat16) if bf16 else B))
+    for flags in (L.BATCH_FORCE_NNZSPLIT, L.BATCH_FORCE_ROWSPLIT):
+        plan = batch.BatchPlan(items, flags)
+        plan.run()
+        torch.cuda.synchronize()
+        for it in items:
+            assert rel_err(it.C.cpu().numpy(), host_ref(it)) <= FP32_NORMWISE, (flags, it.n_rhs)
+        plan.close()

@@ -127,13 +127,24 @@ def test_bf16_inputs_refused_where_unsupported(dev, operands):
     ops16 = [A.to_values(torch.bfloat16) for A in operands]
     with pytest.raises(L.UnsupportedError):          # the plain path is FP32 only
         serve.serve(ops16, r16, "plain", input_dtype=torch.bfloat16)
-    # BF16 rows are read 8 columns per lane: SpMM widths must be multiples of 8.
-    A = ops16[0]
-    odd = serve.Request("spmm", 0, 12, 0.0,
-                        torch.zeros((A.n_inner, 12), dtype=torch.bfloat16, device=dev),
-                        None, torch.empty((A.n_outer, 12), device=dev))
-    with pytest.raises(L.UnsupportedError):
-        serve.serve(ops16, [odd], "batched", input_dtype=torch.bfloat16)
+
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_widths_of_four_and_eight_columns_per_lane(dev, operands, bf16):
+    # SpMM widths 12 (4 columns per lane), 24 and 40 (8 per lane) and SDDMM
+    # 24, in FP32 and in the all-BF16 form, each equal to the oracle.
+    ops_ = [A.to_values(torch.bfloat16) for A in operands] if bf16 else operands
+    dt = torch.bfloat16 if bf16 else torch.float32
+    inp = serve.Inputs(operands, seed=3, dtype=dt)
+    reqs = serve.make_trace(operands, inp, qps=50_000, n=120, seed=8, outputs=True,
+                            spmm_k=(12, 24, 40), sddmm_k=24)
+    res = serve.serve(ops_, reqs, "batched", max_batch=64, input_dtype=dt)
+    assert (res.status == 0).all()
+    for r in reqs:
+        r64 = serve.Request(r.kind, r.operand, r.k, 0.0, r.inp.float(),
+                            r.inp2.float() if r.inp2 is not None else None)
+        assert rel_err(r.out.cpu().numpy(), reference(ops_[r.operand], r64)) <= FP32_NORMWISE
 
 
 def test_admission_budget_and_slots(dev, operands):
