@@ -15,6 +15,8 @@
 // one side stored -> the value itself (ADD) or max against the fill 0 (MAX).
 #include "fgx_scan.hpp"
 
+#include <algorithm>
+
 namespace fgx {
 
 // Julia's `max` (base/math.jl): for floats, a NaN on either side gives NaN
@@ -354,6 +356,61 @@ int fgx_ewise_symbolic(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_
                                                       ctx->host_i64, &total, s)));
   *c_nnz = total;
   return FGX_OK;
+}
+
+int fgx_ewise_async(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_compressed* B,
+                    void* c_ptr, void* c_idx, void* c_val, int64_t c_cap, void* stream) {
+  clear_error();
+  if (!ctx) return set_error(FGX_EINVAL, "fgx_ewise_async: NULL ctx");
+  if (op < FGX_OP_MUL || op > FGX_OP_MAX) return set_error(FGX_EINVAL, "fgx_ewise_async: bad op");
+  FGX_TRY(check_pair(A, B, "fgx_ewise_async"));
+  if (!c_ptr) return set_error(FGX_EINVAL, "fgx_ewise_async: NULL c_ptr");
+  // The intersection keeps at most the shorter list of each fibre pair, the
+  // union at most both.
+  const int64_t bound = op == FGX_OP_MUL ? std::min(A->nnz, B->nnz) : A->nnz + B->nnz;
+  if (c_cap < bound || (bound > 0 && (!c_idx || !c_val)))
+    return set_error(FGX_EINVAL, "fgx_ewise_async: output capacity %lld below the bound %lld",
+                     (long long)c_cap, (long long)bound);
+  IdxCombo ic;
+  FGX_TRY(idx_combo(A, &ic));
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  hipStream_t s = pick_stream(ctx, stream);
+  FGX_TRY(admit_operand(ctx, A, s, "fgx_ewise_async"));
+  FGX_TRY(admit_operand(ctx, B, s, "fgx_ewise_async"));
+  const int64_t m = A->n_outer;
+  void* scr;
+  FGX_TRY(scratch(ctx, s, static_cast<size_t>(m + 1) * sizeof(int64_t) + scan_work_bytes(m), &scr));
+  int64_t* cnt = static_cast<int64_t*>(scr);
+  void* work = cnt + m + 1;
+  const int G = pick_group(m, A->nnz, B->nnz);
+  const int64_t base = A->index_base;
+  if (m > 0) {
+    FGX_DISPATCH_IDX(ic, {
+      if (G == 8) launch_count<8, TP, TI>(op, A, B, cnt, s);
+      else if (G == 16) launch_count<16, TP, TI>(op, A, B, cnt, s);
+      else launch_count<64, TP, TI>(op, A, B, cnt, s);
+    });
+    FGX_TRY(launch_check("ewise_count_kernel"));
+  }
+  // ptr from the counts on the device, with no read-back of the total.
+  FGX_DISPATCH_IDX(ic, FGX_TRY(scan_counts_to_ptr<TP>(cnt, m, base, static_cast<TP*>(c_ptr), work,
+                                                      ctx->host_i64, nullptr, s)));
+  if (m == 0) return FGX_OK;
+#define FGX_EW_FILL(TV_)                                                              \
+  FGX_DISPATCH_IDX(ic, {                                                              \
+    if (G == 8) launch_fill<8, TP, TI, TV_>(op, A, B, c_ptr, c_idx, c_val, s);        \
+    else if (G == 16) launch_fill<16, TP, TI, TV_>(op, A, B, c_ptr, c_idx, c_val, s); \
+    else launch_fill<64, TP, TI, TV_>(op, A, B, c_ptr, c_idx, c_val, s);              \
+  })
+  switch (A->val_type) {
+    case FGX_F64: FGX_EW_FILL(double); break;
+    case FGX_F32: FGX_EW_FILL(float); break;
+    case FGX_I32: FGX_EW_FILL(int32_t); break;
+    default: return set_error(FGX_EUNSUP, "fgx_ewise_async: value type");
+  }
+#undef FGX_EW_FILL
+  return launch_check("ewise_fill_kernel");
 }
 
 int fgx_ewise_numeric(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_compressed* B,

@@ -129,6 +129,7 @@ SIGNATURES = {
     "fgx_sddmm_csr": (_I, [_P, _PC, _I64, _P, _I64, _I, _P, _I64, _I, _P, _P]),
     "fgx_ewise_symbolic": (_I, [_P, _I, _PC, _PC, _P, ctypes.POINTER(_I64), _P]),
     "fgx_ewise_numeric": (_I, [_P, _I, _PC, _PC, _P, _P, _P, _P]),
+    "fgx_ewise_async": (_I, [_P, _I, _PC, _PC, _P, _P, _P, _I64, _P]),
     "fgx_spgemm_symbolic": (_I, [_P, _I, _PC, _PC, _P, ctypes.POINTER(_I64), _P]),
     "fgx_spgemm_numeric": (_I, [_P, _I, _PC, _PC, _P, _P, _P, _P]),
     "fgx_convert": (_I, [_P, _P, _I, _P, _I, _I64, _P]),

@@ -289,6 +289,32 @@ def ewise(op: str, A: Compressed, B: Compressed) -> Compressed:
     return Compressed(cptr, cidx, cval, A.n_outer, A.n_inner, A.base)
 
 
+def ewise_async(op: str, A: Compressed, B: Compressed, out=None):
+    """C = op(A, B) in one stream-ordered call with no host synchronisation
+    (fgx_ewise_async).  Returns (cptr, cidx, cval) with cidx/cval of the
+    bound's length (intersection: min(nnz(A), nnz(B)); union: the sum), or
+    fills the buffers `out` = (cptr, cidx, cval) given; nnz(C) is
+    cptr[-1] - base once the stream reaches it.  Capturable in a graph
+    after one call outside the capture."""
+    if (A.n_outer, A.n_inner) != (B.n_outer, B.n_inner):
+        raise L.DimensionMismatch(L.EDIM, f"shapes ({A.n_outer}, {A.n_inner}) and "
+                                          f"({B.n_outer}, {B.n_inner}) differ")
+    opc = EWISE_OPS[op]
+    if out is None:
+        cap = min(A.nnz, B.nnz) if op == "mul" else A.nnz + B.nnz
+        out = (torch.empty(A.n_outer + 1, dtype=A.ptr.dtype, device=A.device),
+               torch.empty(cap, dtype=A.idx.dtype, device=A.device),
+               torch.empty(cap, dtype=A.val.dtype, device=A.device))
+    cptr, cidx, cval = out
+    if cptr.numel() != A.n_outer + 1 or cidx.numel() != cval.numel():
+        raise L.DimensionMismatch(L.EDIM, "ewise_async: output buffer lengths")
+    da, db = A.desc(), B.desc()
+    L.check(L.load().fgx_ewise_async(context(A.device).handle, opc, ctypes.byref(da),
+                                     ctypes.byref(db), _ptr(cptr), _ptr(cidx), _ptr(cval),
+                                     cidx.numel(), _stream(A.device)), "fgx_ewise_async")
+    return cptr, cidx, cval
+
+
 def spgemm(semiring: str, A: Compressed, B: Compressed) -> Compressed:
     """Gustavson C = A*B over CSR operands: symbolic -> allocate -> numeric."""
     if A.n_inner != B.n_outer:

@@ -387,6 +387,24 @@ function ewise_buffers(dev::GPU, op::AbstractString, A::Tensor, B::Tensor)
     (ptr, idx, val, da.n_inner, da.n_outer)
 end
 
+"""Both phases in one stream-ordered call (no host synchronisation), into
+idx/val buffers of the bound's length (∩: min(nnz(A), nnz(B)); ∪: their sum);
+nnz(C) = ptr[end] - 1 once the stream reaches it.  For stream-ordered
+pipelines and graph capture."""
+function ewise_async_buffers(dev::GPU, op::AbstractString, A::Tensor, B::Tensor)
+    opc = EWISE_OPS[op]
+    da, db = descriptor(A), descriptor(B)
+    cap = op == "mul" ? min(da.nnz, db.nnz) : da.nnz + db.nnz
+    ptr = GPUVector{eltype(A.lvl.lvl.ptr)}(dev, da.n_outer + 1)
+    idx = GPUVector{eltype(A.lvl.lvl.idx)}(dev, cap)
+    val = GPUVector{eltype(values_of(A))}(dev, cap)
+    check(ccall((:fgx_ewise_async, libfgx), Cint,
+                (Ptr{Cvoid}, Cint, Ref{FgxCompressed}, Ref{FgxCompressed}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Ptr{Cvoid}, Int64, Ptr{Cvoid}),
+                dev.ctx, opc, da, db, ptr.ptr, idx.ptr, val.ptr, cap, C_NULL))
+    (ptr, idx, val, da.n_inner, da.n_outer)
+end
+
 # Gustavson over fibre-major operands: fibre r of the result is
 # sum over stored k of fibre r of A of A[k, r] * (fibre k of B)
 # (docs/examples/spgemm.jl:59-76 in its row form).

@@ -211,6 +211,17 @@ int fgx_ewise_symbolic(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_
                        void* c_ptr, int64_t* c_nnz, void* stream);
 int fgx_ewise_numeric(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_compressed* B,
                       const void* c_ptr, void* c_idx, void* c_val, void* stream);
+/* Both phases in one stream-ordered call, with no host synchronisation: the
+ * caller sizes c_idx / c_val to c_cap >= the bound (FGX_OP_MUL: min(nnz(A),
+ * nnz(B)); FGX_OP_ADD / FGX_OP_MAX: nnz(A) + nnz(B); FGX_EINVAL below it),
+ * and nnz(C) = c_ptr[n_outer] - index_base once the stream reaches it.  The
+ * same results, bit for bit, as symbolic + numeric.  For stream-ordered
+ * pipelines and graph capture (call once outside the capture first, so the
+ * context's scratch is sized).  Finch's counterpart is the same assemble
+ * loop into an output whose capacity the caller reserved (`resize_if_smaller!`,
+ * src/util/shims.jl:8-12), frozen without a shrink. */
+int fgx_ewise_async(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_compressed* B,
+                    void* c_ptr, void* c_idx, void* c_val, int64_t c_cap, void* stream);
 
 /* Gustavson SpGEMM C = A*B over CSR operands (A: m x l, B: l x n), row by row:
  * spgemm_gustavson (docs/examples/spgemm.jl:59-76; generated code
