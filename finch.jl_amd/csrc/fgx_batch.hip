@@ -542,7 +542,7 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     d.op = BO_SPMM;
     d.vec = aligned16(it.A.val) && aligned16(it.A.idx);
     d.mfma = 0;
-    d.cpl = cols_per_lane_for(it.n_rhs);
+    d.cpl = cols_per_lane_for(it.n_rhs, p->dense_type);
     d.table = nullptr;
   }
   p->combo = combo;

@@ -66,19 +66,21 @@ constexpr int kSpmvRowGroup = 8;   // lanes per row, row-split SpMV
 // to a power of two.
 __host__ __device__ inline int group_for_k(int64_t k, int cpl = 4) {
   const int64_t lanes = (k + cpl - 1) / cpl;
-  int g = 2;
+  int g = (cpl == 8 && lanes <= 1) ? 1 : 2;  // one 8-column lane covers k = 8
   while (g < lanes && g < 64) g <<= 1;
   return g;
 }
 
-// Dense columns per lane of SpMM for a width k: 8 (a lane reads 32 bytes of
-// an FP32 row or 16 of a BF16 row) when k >= 16 is a multiple of 8, else 4,
-// so that narrow widths do not leave half of each lane group idle.  Measured
-// on the 4,096-item plan: the FP32 n_rhs=32 class 5.37 -> 3.43 ms and the
-// n_rhs=128 class 19.7 -> 17.0 ms with 8; at n_rhs=8, 8 columns leave one of
-// the two lanes idle and are 5 % slower than 4.
-__host__ __device__ inline int cols_per_lane_for(int64_t k) {
-  return (k >= 16 && k % 8 == 0) ? 8 : 4;
+// Dense columns per lane of SpMM for a width k and dense type: 8 (a lane
+// reads 32 bytes of an FP32 row or 16 of a BF16 row) when k >= 16 is a
+// multiple of 8, and for BF16 rows of width 8 (one lane per position); else
+// 4.  Measured on the 4,096-item plan: the FP32 n_rhs=32 class 5.37 -> 3.43
+// ms and the n_rhs=128 class 19.7 -> 17.0 ms with 8; at n_rhs=8 one 8-column
+// lane per position is 9 % faster than two 4-column lanes for BF16 rows and
+// 10 % slower for FP32 rows.
+__host__ __device__ inline int cols_per_lane_for(int64_t k, int dense_type = FGX_F32) {
+  if (k % 8 != 0) return 4;
+  return (k >= 16 || (k == 8 && dense_type == FGX_BF16)) ? 8 : 4;
 }
 
 

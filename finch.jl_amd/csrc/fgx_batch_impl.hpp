@@ -59,11 +59,14 @@ __device__ __forceinline__ void rows_spmv(const BatchDesc& d, int64_t r0, int64_
   }
 }
 
-// Gathers in flight per lane: 8 slices of up to 16 bytes, or 4 of 32 bytes
-// (8 FP32 columns per lane), fewer when the group is narrower.
+// Gathers in flight per lane: 4 slices of 32 bytes (8 FP32 columns), or of
+// 16 bytes for groups of 8 lanes and more 8, a one-lane group 4, and narrower
+// groups one per lane of the group (4 for two lanes measured no faster).
 template <int G, int KV, typename TD>
 constexpr int spmm_u() {
-  return KV * static_cast<int>(sizeof(TD)) > 16 ? (G < 4 ? G : 4) : (G < 8 ? G : 8);
+  return KV * static_cast<int>(sizeof(TD)) > 16
+             ? 4
+             : (G >= 8 ? 8 : (G == 1 ? 4 : (G < 4 ? G : 4)));
 }
 
 // SpMM: G lanes per row, each lane KV columns of the dense width.  A row is
@@ -345,6 +348,7 @@ int launch_class(fgx_batch_plan* p, int64_t w0, int64_t w1, const int64_t* seg_o
         case 16: rc = launch_class<TP, TI, TV, TD, 16>(p, w0, w1, seg_off, total, inl, s); break;                    \
         case 32: rc = launch_class<TP, TI, TV, TD, 32>(p, w0, w1, seg_off, total, inl, s); break;                    \
         case 64: rc = launch_class<TP, TI, TV, TD, 64>(p, w0, w1, seg_off, total, inl, s); break;                    \
+        case kClsWide | 1: rc = launch_class<TP, TI, TV, TD, kClsWide | 1>(p, w0, w1, seg_off, total, inl, s); break;   \
         case kClsWide | 2: rc = launch_class<TP, TI, TV, TD, kClsWide | 2>(p, w0, w1, seg_off, total, inl, s); break;   \
         case kClsWide | 4: rc = launch_class<TP, TI, TV, TD, kClsWide | 4>(p, w0, w1, seg_off, total, inl, s); break;   \
         case kClsWide | 8: rc = launch_class<TP, TI, TV, TD, kClsWide | 8>(p, w0, w1, seg_off, total, inl, s); break;   \

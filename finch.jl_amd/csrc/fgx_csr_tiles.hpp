@@ -32,8 +32,10 @@ constexpr int kWinCap = 512;  // fibre ends cached in LDS per wave
 #ifndef FGX_TILE_E_WIDE
 #define FGX_TILE_E_WIDE 128  // positions per lane group for G >= 32
 #endif
+// One-lane groups take 8 positions each, so a tile (512 positions) stages no
+// more in LDS than a two-lane group's.
 __host__ __device__ constexpr int tile_e_for_g(int g) {
-  return g >= 32 ? FGX_TILE_E_WIDE : (g >= 16 ? 64 : (g >= 4 ? 32 : 16));
+  return g >= 32 ? FGX_TILE_E_WIDE : (g >= 16 ? 64 : (g >= 4 ? 32 : (g >= 2 ? 16 : 8)));
 }
 
 // Fibres that a wave tile [q0, qend) touches: r_lo owns q0 (0 for the first

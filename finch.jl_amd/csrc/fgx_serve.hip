@@ -241,8 +241,8 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       if (q.kind == FGX_REQ_SDDMM) {
         W = kWave * kSddmmSub;
       } else if (autotune_pick(A.n_outer, A.nnz, k, oi.max_row, vbytes, 0,
-                               cols_per_lane_for(k)).kind == BK_TILES) {
-        W = tile_width_for_k(k, cols_per_lane_for(k));
+                               cols_per_lane_for(k, dense_type)).kind == BK_TILES) {
+        W = tile_width_for_k(k, cols_per_lane_for(k, dense_type));
       }
       if (W > 0 && !oi.table_for(W)) {
         int64_t* t = nullptr;
@@ -264,7 +264,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
       if (q.kind == FGX_REQ_SDDMM) continue;
       const fgx_compressed& A = operands[q.operand];
       const int64_t k = q.kind == FGX_REQ_SPMV ? 1 : q.k;
-      const int cpl = cols_per_lane_for(k);
+      const int cpl = cols_per_lane_for(k, dense_type);
       if (autotune_pick(A.n_outer, A.nnz, k, maxrow[static_cast<size_t>(q.operand)], vbytes, 0,
                         cpl)
               .kind != BK_TILES)
@@ -453,7 +453,7 @@ extern "C" int fgx_serve_run(fgx_ctx* ctx, const fgx_compressed* operands, int64
           d.ldc = k;
           d.base = A.index_base;
           d.vec = aligned16(A.val) && aligned16(A.idx);
-          const int cpl = cols_per_lane_for(k);
+          const int cpl = cols_per_lane_for(k, dense_type);
           d.cpl = cpl;
           const int32_t item = static_cast<int32_t>(hdesc.size());
           if (q.kind == FGX_REQ_SDDMM) {

@@ -173,11 +173,12 @@ def test_bf16_inputs_refused_without_bf16_values(dev, small_batch):
 
 @pytest.mark.parametrize("bf16", [False, True])
 def test_widths_of_four_and_eight_columns_per_lane(dev, bf16):
-    """Widths of 16 and more that are multiples of 8 read 8 columns per lane,
-    the others 4 (k = 20: 8 per lane with a 4-column tail lane): every item
-    equals the oracle, on both the nnz-split and the row-split bodies."""
+    """Widths of 16 and more that are multiples of 8 read 8 columns per lane
+    (k = 20: 4 per lane; BF16 rows of width 8: one 8-column lane per
+    position), the others 4: every item equals the oracle, on both the
+    nnz-split and the row-split bodies."""
     items = []
-    for i, k in enumerate((4, 12, 16, 20, 24, 40, 136)):
+    for i, k in enumerate((4, 8, 12, 16, 20, 24, 40, 136)):
         A = batch.synth_matrix(700 + 37 * i, 30.0, 200 + i, dev)
         if bf16:
             A = A.to_values(torch.bfloat16)

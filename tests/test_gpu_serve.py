@@ -138,7 +138,7 @@ def test_widths_of_four_and_eight_columns_per_lane(dev, operands, bf16):
     dt = torch.bfloat16 if bf16 else torch.float32
     inp = serve.Inputs(operands, seed=3, dtype=dt)
     reqs = serve.make_trace(operands, inp, qps=50_000, n=120, seed=8, outputs=True,
-                            spmm_k=(12, 24, 40), sddmm_k=24)
+                            spmm_k=(8, 12, 24, 40), sddmm_k=24)
     res = serve.serve(ops_, reqs, "batched", max_batch=64, input_dtype=dt)
     assert (res.status == 0).all()
     for r in reqs:
