@@ -448,14 +448,14 @@ int max_row_lengths(const fgx_compressed* ops, int64_t n, int combo, int64_t* ou
 TunePick autotune_pick(int64_t m, int64_t nnz, int64_t k, int64_t max_row, int vbytes,
                               int flags, int cpl) {
   const double mean = m > 0 ? static_cast<double>(nnz) / static_cast<double>(m) : 0.0;
-  // Measured on MI355X (graph replays of 64-256 same-class items, row-split
-  // time / tile time): row-split wins only for wide dense operands (k >= 128,
-  // 32+ lanes per row) over even, long rows (Poisson, mean 32-128):
-  // 0.77-0.95.  Everywhere else merge-path tiles win, by 1.1-2x on even rows
-  // and by up to 40x on power-law rows, where one long row serialises a lane
-  // group.
+  // Measured on MI355X (graph replays of same-class items, row-split time /
+  // tile time): row-split wins only for wide dense operands (k >= 128) over
+  // even rows of mean length 12 and more: 0.92-0.96 on the batched workload's
+  // Poisson items (mean 12-64), 0.77-0.95 on calibration sets with mean
+  // 32-128; at mean 4-12 tiles win (1.07), at k = 32 by 1.26-1.58, and on
+  // power-law rows by up to 40x, where one long row serialises a lane group.
   const bool even = static_cast<double>(max_row) <= 2.0 * mean + 16.0;
-  bool tiles = !(k >= 128 && mean >= 24.0 && even);
+  bool tiles = !(k >= 128 && mean >= 12.0 && even);
   if (flags & FGX_BATCH_FORCE_ROWSPLIT) tiles = false;
   if (flags & FGX_BATCH_FORCE_NNZSPLIT) tiles = true;
   const double target = 256.0 * 1024.0;
