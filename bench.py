@@ -629,7 +629,11 @@ def bench_batched(dev, count, with_loop=True):
               for it in items)
     print(f"# batch generated: {count} matrices, nnz={nnz} in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     plan = batch.BatchPlan(items)
+    torch.cuda.synchronize()
+    create_ms = (time.perf_counter() - t0) * 1e3
     plan_ms = _events_ms(plan.run, 5)
     ref = [it.C.clone() for it in items]
     loop_ms = _events_ms(lambda: batch.run_loop(items), 3) if with_loop else None
@@ -679,6 +683,7 @@ def bench_batched(dev, count, with_loop=True):
            "bf16_inputs_plan_ms": round(plan16i_ms, 3),
            "bf16_inputs_plan_gbs": gbs(alg16i, plan16i_ms),
            "bf16_inputs_normwise_err_vs_fp32": err16i,
+           "plan_create_ms": round(create_ms, 1),
            "algorithmic_bytes": alg, "plan_stats": stats, "classes": classes}
     del items, items16, items16i, ref
     torch.cuda.empty_cache()

@@ -71,6 +71,63 @@ __global__ __launch_bounds__(kBatchThreads) void batch_fixup_kernel(
   }
 }
 
+// The tile tables of a plan's TILES items in one launch: thread e builds
+// entry e of the concatenated tables, where item slot j owns entries
+// [off[j], off[j+1]) (its ntiles + 1 entries, laid out as the plan's tables).
+// Entry t of an item is the fibre owning position t*W (nnz for the last),
+// as tile_table_kernel computes it for one operand.
+template <typename TP>
+__global__ __launch_bounds__(256) void plan_tables_kernel(const BatchDesc* __restrict__ desc,
+                                                          const int64_t* __restrict__ slots,
+                                                          int64_t n, int64_t total,
+                                                          int64_t* __restrict__ tables) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int64_t* off = slots + n;  // n + 1 offsets after the n item ids
+  int64_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (off[mid] <= e) lo = mid; else hi = mid;
+  }
+  const BatchDesc& d = desc[slots[lo]];
+  const int64_t W = tile_width_for_k(d.k, d.cpl);
+  const int64_t ntiles = off[lo + 1] - off[lo] - 1;
+  const int64_t t = e - off[lo];
+  const int64_t q = t < ntiles ? t * W : d.nnz;
+  const TP* ptr = static_cast<const TP*>(d.ptr);
+  int64_t r0 = 0, r1 = d.m;
+  while (r0 < r1) {
+    const int64_t mid = r0 + ((r1 - r0) >> 1);
+    if (pos_at(ptr, static_cast<int64_t>(d.base), mid + 1) <= q) r0 = mid + 1; else r1 = mid;
+  }
+  tables[e] = r0;
+}
+
+template <typename T>
+static int dev_alloc_copy(T** out, const T* src, int64_t n, hipStream_t s);
+
+static int build_plan_tables(fgx_batch_plan* p, const std::vector<int64_t>& ids,
+                             const std::vector<int64_t>& off, int64_t total, hipStream_t s) {
+  const int64_t n = static_cast<int64_t>(ids.size());
+  if (n == 0 || total == 0) return FGX_OK;
+  std::vector<int64_t> host(ids);
+  host.insert(host.end(), off.begin(), off.end());
+  int64_t* d = nullptr;
+  FGX_TRY(dev_alloc_copy(&d, host.data(), static_cast<int64_t>(host.size()), s));
+  const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
+  if (p->combo == IDX_64_64 || p->combo == IDX_64_32)
+    hipLaunchKernelGGL((plan_tables_kernel<int64_t>), dim3(blocks), dim3(256), 0, s, p->d_desc, d,
+                       n, total, p->d_tables);
+  else
+    hipLaunchKernelGGL((plan_tables_kernel<int32_t>), dim3(blocks), dim3(256), 0, s, p->d_desc, d,
+                       n, total, p->d_tables);
+  const int rc = launch_check("plan_tables_kernel");
+  const hipError_t e = hipStreamSynchronize(s);
+  (void)hipFree(d);
+  FGX_TRY(rc);
+  return e == hipSuccess ? FGX_OK : set_error(FGX_EHIP, "plan tables: %s", hipGetErrorString(e));
+}
+
 // Per-item row-length statistics for the autotuner: [0] max row length.
 template <typename TP>
 __global__ __launch_bounds__(256) void row_stats_kernel(const BatchDesc* __restrict__ desc,
@@ -621,14 +678,23 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     for (int64_t i = 0; i < count && rc == FGX_OK; i++) {
       const int64_t off = table_off[static_cast<size_t>(i)];
       if (off < 0) continue;
-      BatchDesc& d = desc[static_cast<size_t>(i)];
-      d.table = p->d_tables + off;
-      rc = fill_tile_table(&items[i].A, tile_width_for_k(d.k, d.cpl), s, p->d_tables + off);
+      desc[static_cast<size_t>(i)].table = p->d_tables + off;
     }
   }
   if (rc == FGX_OK)
     rc = batch_upload(p, desc.data(), count, work.data(), static_cast<int64_t>(work.size()),
                       carry_tiles, carry_vals, ctx->num_cus, s);
+  // All tile tables in one launch (from the uploaded descriptors).
+  if (rc == FGX_OK && table_entries > 0) {
+    std::vector<int64_t> ids, offs;
+    for (int64_t i = 0; i < count; i++) {
+      if (table_off[static_cast<size_t>(i)] < 0) continue;
+      ids.push_back(i);
+      offs.push_back(table_off[static_cast<size_t>(i)]);
+    }
+    offs.push_back(table_entries);
+    rc = build_plan_tables(p, ids, offs, table_entries, s);
+  }
   if (rc != FGX_OK) {
     free_plan_buffers(p);
     delete p;
