@@ -620,6 +620,7 @@ def bench_batched(dev, count, with_loop=True):
     loop, and BF16-stored values (FP32 accumulate) vs FP32."""
     import torch
 
+    from finchgpu import _lib as L
     from finchgpu import batch
 
     t0 = time.time()
@@ -639,6 +640,16 @@ def bench_batched(dev, count, with_loop=True):
     loop_ms = _events_ms(lambda: batch.run_loop(items), 3) if with_loop else None
     stats = plan.stats()
     plan.close()
+    # The measured autotuner: row-split vs nnz-split timed per bucket of
+    # similar items at plan creation.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mplan = batch.BatchPlan(items, L.BATCH_MEASURE)
+    torch.cuda.synchronize()
+    mcreate_ms = (time.perf_counter() - t0) * 1e3
+    mplan_ms = _events_ms(mplan.run, 5)
+    mstats = mplan.stats()
+    mplan.close()
     den = sum(float(torch.linalg.vector_norm(r.double()) ** 2) for r in ref)
 
     def err(its):
@@ -684,6 +695,8 @@ def bench_batched(dev, count, with_loop=True):
            "bf16_inputs_plan_gbs": gbs(alg16i, plan16i_ms),
            "bf16_inputs_normwise_err_vs_fp32": err16i,
            "plan_create_ms": round(create_ms, 1),
+           "measured_tuner": {"plan_graph_ms": round(mplan_ms, 3), "plan_create_ms":
+                              round(mcreate_ms, 1), "items_rowsplit": mstats["items_rowsplit"]},
            "algorithmic_bytes": alg, "plan_stats": stats, "classes": classes}
     del items, items16, items16i, ref
     torch.cuda.empty_cache()

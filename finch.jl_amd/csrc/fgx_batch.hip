@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <vector>
 
 namespace fgx {
@@ -539,14 +540,13 @@ using namespace fgx;
 
 extern "C" {
 
-int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, int flags,
-                          fgx_batch_plan** out) {
-  clear_error();
-  if (!ctx || !out) return set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL argument");
-  if (count < 0 || (count > 0 && !items)) return set_error(FGX_EINVAL, "fgx_batch_plan_create: bad items");
+// The plan of fgx_batch_plan_create, with the context's lock already held.
+// force[i] (optional): BK_ROWS or BK_TILES overrides the autotuner's choice
+// for item i (-1: the rule's); maxrow_in (optional): the items' longest rows.
+static int plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, int flags,
+                       const signed char* force, const int64_t* maxrow_in,
+                       fgx_batch_plan** out) {
   *out = nullptr;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  FGX_HIP(hipSetDevice(ctx->device));
   hipStream_t s = ctx->stream;
   auto* p = new fgx_batch_plan();
   p->device = ctx->device;
@@ -605,7 +605,9 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   p->combo = combo;
   // Row-length statistics (one block per item).
   std::vector<int64_t> maxrow(static_cast<size_t>(count), 0);
-  if (count > 0) {
+  if (maxrow_in) {
+    std::copy(maxrow_in, maxrow_in + count, maxrow.begin());
+  } else if (count > 0) {
     std::vector<fgx_compressed> ops(static_cast<size_t>(count));
     for (int64_t i = 0; i < count; i++) ops[static_cast<size_t>(i)] = items[i].A;
     int rc = max_row_lengths(ops.data(), count, combo, maxrow.data(), s);
@@ -636,8 +638,11 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
     p->alg_bytes += d.nnz * (vbytes + ibytes) + (d.m + 1) * elem_size(items[i].A.ptr_type) +
                     d.n * d.k * (bf16_in ? 2 : 4) + d.m * d.k * 4;
     if (d.m == 0) continue;
+    int fl = flags;
+    if (force && force[i] == BK_ROWS) fl = (fl & ~FGX_BATCH_FORCE_NNZSPLIT) | FGX_BATCH_FORCE_ROWSPLIT;
+    if (force && force[i] == BK_TILES) fl = (fl & ~FGX_BATCH_FORCE_ROWSPLIT) | FGX_BATCH_FORCE_NNZSPLIT;
     TunePick pk =
-        autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, flags, d.cpl);
+        autotune_pick(d.m, d.nnz, d.k, maxrow[static_cast<size_t>(i)], vbytes, fl, d.cpl);
     if (pk.kind == BK_TILES) pk.unit = std::min(pk.unit, unit_cap);
     if (pk.kind == BK_TILES) {
       const int64_t W = tile_width_for_k(d.k, d.cpl);
@@ -725,6 +730,94 @@ int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t cou
   }
   *out = p;
   return FGX_OK;
+}
+
+// FGX_BATCH_MEASURE: the autotuner's choice, measured.  Items are bucketed
+// by lane-group class and row statistics (mean row length by powers of two,
+// even or skewed rows); for every bucket of even rows at k >= 32 (where
+// row-split can win) both forms are timed on up to 512 of its items, each as
+// a plan of its own (one warm-up and three timed runs, HIP events on the
+// context's stream), and the faster form is used for the whole bucket.
+// Skewed rows and narrow widths keep nnz-split tiles, which the rule's
+// calibration shows winning there by 1.3-40x.  The timed runs write those
+// items' C.
+static int measure_choices(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, int flags,
+                           std::vector<signed char>* force, std::vector<int64_t>* maxrow) {
+  force->assign(static_cast<size_t>(count), -1);
+  maxrow->assign(static_cast<size_t>(count), 0);
+  if (count <= 0) return FGX_OK;
+  IdxCombo combo;
+  FGX_TRY(idx_combo(&items[0].A, &combo));
+  std::vector<fgx_compressed> ops(static_cast<size_t>(count));
+  for (int64_t i = 0; i < count; i++) ops[static_cast<size_t>(i)] = items[i].A;
+  hipStream_t s = ctx->stream;
+  FGX_TRY(max_row_lengths(ops.data(), count, combo, maxrow->data(), s));
+  const int dense_type = (flags & FGX_BATCH_BF16_INPUTS) ? FGX_BF16 : FGX_F32;
+  std::map<int64_t, std::vector<int64_t>> buckets;
+  for (int64_t i = 0; i < count; i++) {
+    const fgx_batch_item& it = items[i];
+    const int64_t m = it.A.n_outer;
+    if (it.n_rhs < 32 || m == 0 || it.A.nnz == 0) continue;
+    const double mean = static_cast<double>(it.A.nnz) / static_cast<double>(m);
+    if (static_cast<double>((*maxrow)[static_cast<size_t>(i)]) > 2.0 * mean + 16.0) continue;
+    const int cpl = cols_per_lane_for(it.n_rhs, dense_type);
+    const int64_t cls = group_for_k(it.n_rhs, cpl) | (cpl == 8 ? kClsWide : 0);
+    const int64_t mb = static_cast<int64_t>(std::floor(std::log2(std::max(mean, 1.0))));
+    buckets[cls * 64 + mb].push_back(i);
+  }
+  hipEvent_t e0, e1;
+  FGX_HIP(hipEventCreate(&e0));
+  FGX_HIP(hipEventCreate(&e1));
+  int rc = FGX_OK;
+  const int base = flags & ~(FGX_BATCH_FORCE_ROWSPLIT | FGX_BATCH_FORCE_NNZSPLIT | FGX_BATCH_MEASURE);
+  for (auto it = buckets.begin(); it != buckets.end() && rc == FGX_OK; ++it) {
+    const std::vector<int64_t>& members = it->second;
+    const size_t ns = std::min<size_t>(members.size(), 512);
+    std::vector<fgx_batch_item> sample(ns);
+    std::vector<int64_t> smax(ns);
+    for (size_t j = 0; j < ns; j++) {
+      sample[j] = items[members[j]];
+      smax[j] = (*maxrow)[static_cast<size_t>(members[j])];
+    }
+    float ms[2] = {0.f, 0.f};
+    const int forms[2] = {FGX_BATCH_FORCE_NNZSPLIT, FGX_BATCH_FORCE_ROWSPLIT};
+    for (int f = 0; f < 2 && rc == FGX_OK; f++) {
+      fgx_batch_plan* sp = nullptr;
+      rc = plan_create(ctx, sample.data(), static_cast<int64_t>(ns), base | forms[f] | FGX_BATCH_NO_GRAPH,
+                       nullptr, smax.data(), &sp);
+      if (rc == FGX_OK) rc = batch_launch(sp, s);
+      if (rc == FGX_OK && hipEventRecord(e0, s) != hipSuccess) rc = set_error(FGX_EHIP, "measure: event");
+      for (int r = 0; r < 3 && rc == FGX_OK; r++) rc = batch_launch(sp, s);
+      if (rc == FGX_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                           hipEventElapsedTime(&ms[f], e0, e1) != hipSuccess))
+        rc = set_error(FGX_EHIP, "measure: timing");
+      if (sp) fgx_batch_plan_destroy(sp);
+    }
+    if (rc != FGX_OK) break;
+    const signed char pick = ms[1] < ms[0] ? BK_ROWS : BK_TILES;
+    for (int64_t i : members) (*force)[static_cast<size_t>(i)] = pick;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+int fgx_batch_plan_create(fgx_ctx* ctx, const fgx_batch_item* items, int64_t count, int flags,
+                          fgx_batch_plan** out) {
+  clear_error();
+  if (!ctx || !out) return set_error(FGX_EINVAL, "fgx_batch_plan_create: NULL argument");
+  if (count < 0 || (count > 0 && !items)) return set_error(FGX_EINVAL, "fgx_batch_plan_create: bad items");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  FGX_HIP(hipSetDevice(ctx->device));
+  if (!(flags & FGX_BATCH_MEASURE) ||
+      (flags & (FGX_BATCH_FORCE_ROWSPLIT | FGX_BATCH_FORCE_NNZSPLIT)))
+    return plan_create(ctx, items, count, flags, nullptr, nullptr, out);
+  std::vector<signed char> force;
+  std::vector<int64_t> maxrow;
+  FGX_TRY(measure_choices(ctx, items, count, flags, &force, &maxrow));
+  return plan_create(ctx, items, count, flags & ~FGX_BATCH_MEASURE, force.data(), maxrow.data(),
+                     out);
 }
 
 int fgx_batch_plan_run(fgx_ctx* ctx, fgx_batch_plan* plan, void* stream) {

@@ -22,6 +22,7 @@ OP_MUL, OP_ADD, OP_MAX = 0, 1, 2
 PLUS_TIMES, MIN_PLUS = 0, 1
 REQ_SPMV, REQ_SPMM, REQ_SDDMM = 0, 1, 2
 BATCH_FORCE_ROWSPLIT, BATCH_FORCE_NNZSPLIT, BATCH_NO_GRAPH, BATCH_BF16_INPUTS = 1, 2, 4, 8
+BATCH_MEASURE = 16
 NOALLOC_SCRATCH_BYTES = 16 << 20     # FGX_NOALLOC_SCRATCH_BYTES
 
 

@@ -923,7 +923,7 @@ struct FgxBatchItem           # struct fgx_batch_item
 end
 
 const FGX_BATCH_FORCE_ROWSPLIT, FGX_BATCH_FORCE_NNZSPLIT, FGX_BATCH_NO_GRAPH,
-      FGX_BATCH_BF16_INPUTS = 1, 2, 4, 8
+      FGX_BATCH_BF16_INPUTS, FGX_BATCH_MEASURE = 1, 2, 4, 8, 16
 
 """Many independent `C = A*B` (CSR A, row-major F32 B/C with `n_rhs` columns)
 in one persistent launch, replayed from a hipGraph.  The items' buffers must

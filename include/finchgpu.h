@@ -281,8 +281,8 @@ int fgx_convert(fgx_ctx* ctx, const void* src, int src_type, void* dst, int dst_
 /* ---- batched form ------------------------------------------------------- */
 
 /* One item of a batch: C = A*B with B n_inner x n_rhs and C n_outer x n_rhs,
- * both row-major; C is F32, B is F32 (or BF16 under FGX_BATCH_BF16_INPUTS);
- * A's values F32 or BF16. */
+ * both row-major and 16-byte aligned (n_rhs > 1); C is F32, B is F32 (or BF16
+ * under FGX_BATCH_BF16_INPUTS); A's values F32 or BF16. */
 typedef struct fgx_batch_item {
   fgx_compressed A;
   int64_t n_rhs;
@@ -296,8 +296,11 @@ typedef struct fgx_batch_plan fgx_batch_plan;
 #define FGX_BATCH_FORCE_ROWSPLIT 1  /* autotuner may only pick row-split */
 #define FGX_BATCH_FORCE_NNZSPLIT 2  /* autotuner may only pick nnz-split */
 #define FGX_BATCH_NO_GRAPH 4        /* launch directly instead of replaying a hipGraph */
-#define FGX_BATCH_BF16_INPUTS 8     /* every B is BF16 (8-byte aligned when n_rhs > 1); needs
-                                       BF16 values; products accumulate in F32 */
+#define FGX_BATCH_BF16_INPUTS 8     /* every B is BF16; needs BF16 values; products
+                                       accumulate in F32 */
+#define FGX_BATCH_MEASURE 16        /* time row-split against nnz-split on a sample of each
+                                       bucket of similar items where row-split can win, and
+                                       use the faster (the timed runs write those items' C) */
 
 /* Build the work queue of one persistent launch over all items: per-item
  * algorithm (row-split vs nnz-split) and tile width picked by the autotuner

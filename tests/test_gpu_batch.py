@@ -191,3 +191,23 @@ def test_widths_of_four_and_eight_columns_per_lane(dev, bf16):
         for it in items:
             assert rel_err(it.C.cpu().numpy(), host_ref(it)) <= FP32_NORMWISE, (flags, it.n_rhs)
         plan.close()
+
+
+def test_measured_tuner_matches_oracle(dev, small_batch):
+    """FGX_BATCH_MEASURE times both forms on each bucket of row-split
+    candidates before choosing; whatever it picks, every item equals the
+    oracle, and the plan replays to the same bits."""
+    items = small_batch + wide_even_items(dev) + skewed_items(dev)
+    plan = batch.BatchPlan(items, L.BATCH_MEASURE)
+    st = plan.stats()
+    assert st["items_nnzsplit"] >= 3          # skewed rows are never measured
+    plan.run()
+    torch.cuda.synchronize()
+    first = [it.C.clone() for it in items]
+    for it in items:
+        assert rel_err(it.C.cpu().numpy(), host_ref(it)) <= FP32_NORMWISE
+    plan.run()
+    torch.cuda.synchronize()
+    for it, f in zip(items, first):
+        assert torch.equal(it.C, f)
+    plan.close()
