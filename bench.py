@@ -140,6 +140,8 @@ def main():
     sddmm = timed(lambda: ops.sddmm_csr(A, X, B, out), args.steps, args.warmup, flush)
     sddmm_warm = timed(lambda: ops.sddmm_csr(A, X, B, out), args.steps, 1)
     print("# spmv/sddmm done", file=sys.stderr, flush=True)
+    gather_floor = bench_gather_floor(A, B, flush, min(args.steps, 10))
+    print(f"# gather floor: {gather_floor and gather_floor['ms']} ms", file=sys.stderr, flush=True)
 
     by_spmm = algorithmic_bytes(M, N, nnz, K, "spmm")
     by_spmv = algorithmic_bytes(M, N, nnz, K, "spmv")
@@ -228,7 +230,10 @@ def main():
                                        "FETCH_SIZE x2.00 + WRITE_SIZE x1.00, copy-calibrated)",
                      "kernel": "spmm_csr_kernel + fixup_chunk_kernel (one fgx_spmm_csr call)",
                      "algorithmic_bytes": by_spmm, "launch_ms": round(spmm["event_ms"], 5),
-                     "warm_launch_ms": round(spmm_warm["event_ms"], 5)},
+                     "warm_launch_ms": round(spmm_warm["event_ms"], 5),
+                     "gather_floor": gather_floor,
+                     "launch_over_gather_floor": (round(spmm["event_ms"] / gather_floor["ms"], 3)
+                                                  if gather_floor else None)},
         "normwise_error_vs_fp64": errors,
         "spmv": {"gbs": round(gbs(by_spmv, spmv["event_ms"]), 2),
                  "event_ms": round(spmv["event_ms"], 5),
@@ -281,6 +286,63 @@ def timed(fn, steps, warmup, flush=None):
     per = [s.elapsed_time(e) for s, e in ev]
     return {"event_ms": float(np.mean(per)), "event_median_ms": float(np.median(per)),
             "wall_ms": wall / steps * 1e3}
+
+
+def bench_gather_floor(A, B, flush, steps):
+    """The gather floor of SpMM's B-row gathers on this device.
+
+    The probe library gathers exactly the rows an SpMM call gathers (row
+    idx[q] of B for every stored position q, 256 bytes each) and nothing
+    else, flushed between calls like the SpMM timing.  The best time over a
+    few launch shapes is the floor the gathers alone impose; the SpMM call
+    is reported against it.  None if the probe library is not built.
+    """
+    import ctypes
+
+    import torch
+
+    path = os.path.join(HERE, "finch.jl_amd", "finchgpu", "libfgx_probe.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.fgx_probe_gather_out_len.restype = ctypes.c_int64
+    lib.fgx_probe_gather_out_len.argtypes = [ctypes.c_int]
+    lib.fgx_probe_gather_rows.restype = ctypes.c_int
+    lib.fgx_probe_gather_rows.argtypes = [
+        ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+        ctypes.c_int, ctypes.c_void_p]
+    assert B.dtype == torch.float32 and B.shape[1] == 64 and B.stride(1) == 1
+    assert B.shape[0] == A.n_inner and A.n_inner < 2 ** 31
+    nnz = A.nnz
+
+    def probe(idx, Bv, u, bpc):
+        n = lib.fgx_probe_gather_out_len(bpc)
+        out = torch.empty(n, dtype=torch.float32, device=B.device)
+
+        def run():
+            rc = lib.fgx_probe_gather_rows(
+                Bv.data_ptr(), Bv.stride(0) // 4, 16, idx.data_ptr(), idx.element_size(),
+                A.base, nnz, out.data_ptr(), n, u, bpc, torch.cuda.current_stream().cuda_stream)
+            if rc != 0:
+                raise RuntimeError(f"gather probe failed ({rc})")
+        return timed(run, steps, 2, flush)["event_ms"]
+
+    shapes = {f"u{u}_b{bpc}": probe(A.idx, B, u, bpc) for u in (8, 16, 32) for bpc in (8, 16)}
+    best = min(shapes.values())
+    # The same gathers folded onto the first `rows` rows of B (idx mod rows):
+    # the rate once the gathered rows fit the Infinity Cache (262,144 rows,
+    # 64 MB) or each XCD's 4 MB L2 (16,384 rows).
+    folded = {}
+    for rows in (262_144, 16_384):
+        fidx = torch.remainder(A.idx - A.base, rows) + A.base
+        t = min(probe(fidx, B[:rows], u, 16) for u in (8, 16, 32))
+        folded[str(rows)] = {"ms": round(t, 5), "gathered_tbs": round(nnz * 256 / (t * 1e-3) / 1e12, 3)}
+        del fidx
+    return {"ms": round(best, 5), "rows_gathered": nnz, "bytes_gathered": nnz * 256,
+            "gathered_tbs": round(nnz * 256 / (best * 1e-3) / 1e12, 3),
+            "by_shape_ms": {k: round(v, 5) for k, v in shapes.items()},
+            "rows_of_B_folded_to": folded}
 
 
 def _cpu_median(fn, reps=10):

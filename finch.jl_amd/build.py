@@ -20,6 +20,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "finchgpu")
 LIB = os.path.join(OUT_DIR, "libfinchgpu.so")
+# Measurement probe (gather floor), loaded only by the benchmark.
+PROBE_SRC = os.path.join(CSRC, "probe")
+PROBE_LIB = os.path.join(OUT_DIR, "libfgx_probe.so")
 OBJ_DIR = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
@@ -73,13 +76,30 @@ def build(jobs: int | None = None, verbose: bool = False, out: str | None = None
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, lib)
+    if out is None:
+        build_probe(verbose)
     return lib
+
+
+def build_probe(verbose: bool = False) -> str:
+    """Builds the benchmark's gather-floor probe library (not the C ABI)."""
+    srcs = sorted(glob.glob(os.path.join(PROBE_SRC, "*.hip")))
+    tmp = PROBE_LIB + ".tmp"
+    cmd = [HIPCC, *CXXFLAGS, "-shared", "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed on the probe:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 def clean() -> None:
     shutil.rmtree(OBJ_DIR, ignore_errors=True)
-    if os.path.exists(LIB):
-        os.remove(LIB)
+    for f in (LIB, PROBE_LIB):
+        if os.path.exists(f):
+            os.remove(f)
 
 
 if __name__ == "__main__":
