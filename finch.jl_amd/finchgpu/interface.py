@@ -325,6 +325,10 @@ def finch(program: str, **tensors):
     out_name = parse(program)[0]
     target = tensors[out_name]
     r, views = route(program, **tensors)
+    if r["output"] == "dense" and isinstance(target, Tensor) and target.format == "separate":
+        res, _ = HANDLERS[r["handler"]](r, views)
+        _write_separate(target, res)
+        return {out_name: target}
     if r["output"] == "dense":
         kind, buf = _kind(target)
         if kind != "dense":
@@ -344,6 +348,21 @@ def finch(program: str, **tensors):
     sl.ptr, sl.idx, body.element.val = res.ptr, res.idx, res.val
     sl.shape, body.lvl.shape = res.n_inner, res.n_outer
     return {out_name: target}
+
+
+def _write_separate(t: Tensor, res: torch.Tensor):
+    """Stores a dense vector result into a Dense(Separate(Element)) output:
+    each position's own Element receives its value.  An output declared
+    without positions (Finch declares it to the loop's extent) takes the
+    result's length; on the device every position's level is a one-value
+    view of the result buffer, so the store costs no copy per position."""
+    sep = t.lvl.lvl
+    n = int(res.numel())
+    if sep.val and len(sep.val) != n:
+        raise L.DimensionMismatch(L.EDIM, f"output has {len(sep.val)} positions, result {n}")
+    proto = sep.lvl
+    sep.val = [Element(proto.fill, proto.dtype, res[p:p + 1]) for p in range(n)]
+    t.lvl.shape = n
 
 
 # ---- lazy / compute ------------------------------------------------------------------------

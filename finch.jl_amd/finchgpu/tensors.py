@@ -12,6 +12,12 @@ and write:
   sums, ``idx`` sorted within each fibre, ``val`` the 1-based positions of
   the child level's entries;
 * ``Dense(lvl, shape)`` — dense level (src/tensors/levels/dense_levels.jl:26-29);
+* ``AtomicElement(fill)``, ``Mutex(lvl)``, ``Separate(lvl)`` — the levels
+  Finch's threaded loops write through (atomic_element_levels.jl:15-37,
+  mutex_levels.jl:19-36, separate_levels.jl:22-31): as outputs of the routed
+  programs, the device kernels' own update protocol (device atomics for the
+  scatter-style loops, one writer per row for the row-parallel ones) stands
+  in for their per-element atomics, locks and per-position memory;
 * ``Tensor(lvl, arr)`` — a tensor over a level tree, built from a dense array
   the way ``dropfills!`` does (src/interface/copy.jl:113-116): column-major
   traversal, fill values dropped, every other value stored;
@@ -47,6 +53,44 @@ class Element:
 
     def __repr__(self):
         return f"Element{{{self.fill}, {self.dtype}}}({_show(self.val)})"
+
+
+class AtomicElement(Element):
+    """Leaf values updated with atomic read-modify-writes
+    (src/tensors/levels/atomic_element_levels.jl:15-37).  The values are
+    one buffer, as in Element; the device kernels that write an
+    AtomicElement output update it with device atomics."""
+
+    def __repr__(self):
+        return f"AtomicElement{{{self.fill}, {self.dtype}}}({_show(self.val)})"
+
+
+class Mutex:
+    """A lock per position over the child level
+    (src/tensors/levels/mutex_levels.jl:19-36).  The child holds the values;
+    the locks guard host threads only and are not materialised here: on the
+    device every update of a Mutex output is a device atomic, which is the
+    protocol the locks provide (one read-modify-write at a time per position)."""
+
+    def __init__(self, lvl):
+        self.lvl = lvl
+
+    def __repr__(self):
+        return f"Mutex({self.lvl!r})"
+
+
+class Separate:
+    """Each position owns a separate copy of the child level, in its own
+    memory (src/tensors/levels/separate_levels.jl:1-31): ``lvl`` is the
+    prototype and ``val`` the per-position levels.  Only Separate(Element)
+    is mirrored: ``val`` holds one single-value Element per position."""
+
+    def __init__(self, lvl, val=None):
+        self.lvl = lvl
+        self.val = list(val) if val is not None else []
+
+    def __repr__(self):
+        return f"Separate({self.lvl!r}, {len(self.val)} positions)"
 
 
 class SparseList:
@@ -137,10 +181,13 @@ class Tensor:
     @property
     def format(self) -> str:
         l = self.lvl
-        if isinstance(l, Dense) and isinstance(l.lvl, Element):
+        if isinstance(l, Dense) and _leaf(l.lvl):
             return "dense"
-        if isinstance(l, Dense) and isinstance(l.lvl, Dense) and isinstance(l.lvl.lvl, Element):
+        if isinstance(l, Dense) and isinstance(l.lvl, Dense) and _leaf(l.lvl.lvl):
             return "dense2"
+        if (isinstance(l, Dense) and isinstance(l.lvl, Separate)
+                and type(l.lvl.lvl) is Element):
+            return "separate"
         if isinstance(l, SparseList) and isinstance(l.lvl, Element):
             return "sparsevec"
         if isinstance(l, Dense) and isinstance(l.lvl, SparseList) and isinstance(l.lvl.lvl, Element):
@@ -152,7 +199,7 @@ class Tensor:
     @property
     def shape(self):
         f = self.format
-        if f == "dense" or f == "sparsevec":
+        if f in ("dense", "sparsevec", "separate"):
             return (self.lvl.shape,)
         if f == "dense2":
             return (self.lvl.lvl.shape, self.lvl.shape)
@@ -172,6 +219,14 @@ class Tensor:
     @property
     def on_device(self) -> bool:
         return _is_device(self.element.val)
+
+    def separate_values(self):
+        """The values of a Dense(Separate(Element)) tensor, one per position
+        (a host array, or a device tensor gathered from the positions)."""
+        vals = [e.val for e in self.lvl.lvl.val]
+        if vals and _is_device(vals[0]):
+            return torch.cat(vals)
+        return np.concatenate(vals) if vals else np.zeros(0, self.element.dtype)
 
     def countstored(self) -> int:
         f = self.format
@@ -201,6 +256,11 @@ class Tensor:
             return
         a = np.asarray(arr)
         dt = np.result_type(a.dtype, np.asarray(fill).dtype) if el.val.size == 0 else el.val.dtype
+        if f == "separate":
+            self.lvl.shape = a.shape[0]
+            self.lvl.lvl.val = [Element(fill, dt, np.array([v], dt)) for v in a.astype(dt)]
+            el.dtype = np.dtype(dt)
+            return
         if f == "dense":
             self.lvl.shape = a.shape[0]
             el.val = a.astype(dt).copy()
@@ -247,6 +307,8 @@ class Tensor:
         f = self.format
         el = self.element
         host = lambda a: a.cpu().numpy() if _is_device(a) else np.asarray(a)
+        if f == "separate":
+            return host(self.separate_values()).copy()
         val = host(el.val)
         if f == "dense":
             return val.copy()
@@ -311,6 +373,9 @@ def transfer(device: GPU, t):
             l.ptr, l.idx = mv(l.ptr), mv(l.idx)
         if isinstance(l, SparseDict):
             l.ptr, l.idx, l.val = mv(l.ptr), mv(l.idx), mv(l.val)
+        if isinstance(l, Separate):
+            for e in l.val:
+                e.val = mv(e.val)
         if isinstance(l, Element):
             l.val = mv(l.val)
             break
@@ -318,9 +383,20 @@ def transfer(device: GPU, t):
     return out
 
 
+def _leaf(l) -> bool:
+    """A value leaf: Element or AtomicElement, possibly under Mutex locks."""
+    while isinstance(l, Mutex):
+        l = l.lvl
+    return isinstance(l, Element)
+
+
 def _copy_levels(l):
     if isinstance(l, Element):
-        return Element(l.fill, l.dtype, l.val)
+        return type(l)(l.fill, l.dtype, l.val)
+    if isinstance(l, Mutex):
+        return Mutex(_copy_levels(l.lvl))
+    if isinstance(l, Separate):
+        return Separate(_copy_levels(l.lvl), [Element(e.fill, e.dtype, e.val) for e in l.val])
     if isinstance(l, SparseList):
         return SparseList(_copy_levels(l.lvl), l.shape, l.ptr, l.idx, l.Ti)
     if isinstance(l, SparseDict):

@@ -28,7 +28,7 @@ module FinchGPU
 using Finch
 using TOML
 using Finch: AbstractDevice, Tensor, SwizzleArray, DenseLevel, SparseListLevel, SparseDictLevel,
-    ElementLevel
+    ElementLevel, AtomicElementLevel, MutexLevel, SeparateLevel
 using Finch.FinchNotation: FinchNodeInstance, BlockInstance, LoopInstance, AssignInstance,
     AccessInstance, CallInstance, TagInstance, LiteralInstance, IndexInstance, DeclareInstance,
     FreezeInstance, ThawInstance, YieldBindInstance, VariableInstance, UpdaterInstance
@@ -244,6 +244,16 @@ Finch.transfer(dev::GPU, v::GPUVector) = v.dev == dev ? v : upload(dev, Array(v)
 Finch.transfer(mem::GPUGlobalMemory, v::GPUVector) = Finch.transfer(mem.device, v)
 # SparseDict's insertion table and pool stay on the host.
 Finch.transfer(::Union{GPU,GPUGlobalMemory}, d::Dict) = d
+# The levels of Finch's threaded loops (atomic_element_levels.jl:15-48,
+# mutex_levels.jl:19-41, separate_levels.jl:22-37).  AtomicElement's values
+# move like Element's.  A Mutex's locks guard host threads; they stay on the
+# host, since on the device every update of a routed output is a device
+# atomic.  A Separate level moves each position's own level.
+Finch.transfer(dev::Union{GPU,GPUGlobalMemory}, lvl::AtomicElementLevel{Vf,Tv,Tp}) where {Vf,Tv,Tp} =
+    AtomicElementLevel{Vf,Tv,Tp}(Finch.transfer(dev, lvl.val))
+Finch.transfer(::Union{GPU,GPUGlobalMemory}, locks::Vector{<:Base.AbstractLock}) = locks
+Finch.transfer(dev::Union{GPU,GPUGlobalMemory}, lvls::Vector{<:Finch.AbstractLevel}) =
+    [Finch.transfer(dev, l) for l in lvls]
 
 """The host: `transfer(Host(), tensor)` copies every level buffer back."""
 struct Host end
@@ -263,13 +273,26 @@ const IN_ORDER = WeakKeyDict{GPUVector,Bool}()
 sparse_level(A::Tensor) = A.lvl isa DenseLevel && (A.lvl.lvl isa SparseListLevel ||
                                                    A.lvl.lvl isa SparseDictLevel) &&
                           A.lvl.lvl.lvl isa ElementLevel
-dense_level(A::Tensor) = (A.lvl isa DenseLevel && A.lvl.lvl isa ElementLevel) ||
+# A value leaf: Element or AtomicElement, possibly under Mutex locks.  As the
+# output of a routed program, AtomicElement and Mutex(Element) receive the
+# kernel's result like Element: the device kernels' own update protocol
+# (device atomics for the column scatter, one writer per row otherwise)
+# stands in for the per-element atomics and locks.
+value_leaf(l) = l isa ElementLevel || l isa AtomicElementLevel ||
+                (l isa MutexLevel && value_leaf(l.lvl))
+dense_level(A::Tensor) = (A.lvl isa DenseLevel && value_leaf(A.lvl.lvl)) ||
                          (A.lvl isa DenseLevel && A.lvl.lvl isa DenseLevel &&
-                          A.lvl.lvl.lvl isa ElementLevel)
-gpu_resident(A::Tensor) = on_gpu(values_of(A))
+                          value_leaf(A.lvl.lvl.lvl))
+# Dense(Separate(Element)): each position owns its own Element level.
+separate_level(A::Tensor) = A.lvl isa DenseLevel && A.lvl.lvl isa SeparateLevel &&
+                            A.lvl.lvl.lvl isa ElementLevel
+gpu_resident(A::Tensor) = separate_level(A) ? all(e -> on_gpu(e.val), A.lvl.lvl.val) :
+                          on_gpu(values_of(A))
 gpu_resident(S::SwizzleArray) = gpu_resident(S.body)
 gpu_resident(x) = false
-values_of(A::Tensor) = (l = A.lvl; while !(l isa ElementLevel); l = l.lvl; end; l.val)
+values_of(A::Tensor) = (l = A.lvl;
+                        while !(l isa ElementLevel || l isa AtomicElementLevel); l = l.lvl; end;
+                        l.val)
 
 # A Dense(SparseList(Element)) or Dense(SparseDict(Element)) tensor with
 # GPUVector buffers; swizzle(A, 2, 1) is the same descriptor read row-wise.
@@ -734,6 +757,20 @@ end
 # the result, a Dense(SparseList) output takes the result's buffers (its
 # resize! and freeze), transposed first when its storage order differs.
 function write_output!(out::Tensor, r, dev, res, storage)
+    if storage == "dense" && separate_level(out)
+        # One copy into each position's own level (Separate gives every
+        # position a separate allocation, so there is no single buffer).
+        src, lvls = values_of(res), out.lvl.lvl.val
+        length(src) == length(lvls) || throw(DimensionMismatch("output has $(length(lvls)) " *
+                                                               "positions, result $(length(src))"))
+        T = eltype(src)
+        for (p, e) in enumerate(lvls)
+            on_gpu(e.val) && length(e.val) == 1 && eltype(e.val) == T ||
+                unsupported("position $p of the Separate output is not a GPU value of type $T")
+            hip_copy(e.val.ptr, src.ptr + (p - 1) * sizeof(T), sizeof(T), D2D)
+        end
+        return out
+    end
     if storage == "dense"
         dense_level(out) || unsupported("the output of $(r["program"]) must be dense")
         src, dst = values_of(res), values_of(out)

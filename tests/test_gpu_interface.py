@@ -291,3 +291,39 @@ def test_unrouted_device_programs_raise(dev):
     from finchgpu._lib import UnsupportedError
     with pytest.raises(UnsupportedError):
         fi.einsum("C[i, k] <<max>>= A[i, j] + B[j, k]", A=A, B=B)
+
+
+# ---- outputs through Finch's threaded-write levels ---------------------------------------
+
+def test_threaded_write_levels_as_outputs(dev):
+    # test/suites/parallel_tests.jl:627-690 (and benchmark/benchmarks.jl:402-426):
+    # y = Dense(AtomicElement(0.0)) or Dense(Mutex(Element(0.0))) under
+    # `for j = parallel(_), i = _; y[i] += A[i, j] * x[j]`, and
+    # y = Dense(Separate(Element(0.0))) under
+    # `for i = parallel(_), j = _; y[i] += A[j, i] * x[j]`; the reference
+    # checks norm(y - A*x) / norm(A*x) < 1e-10 (and permutedims(A) * x).
+    from finchgpu.tensors import AtomicElement, Mutex, Separate
+    S = _rand(42, 42, 0.1, 41)
+    xh = np.random.default_rng(42).random(42)
+    A = transfer(GPU(0), Tensor(Dense(SparseList(Element(0.0))), S))
+    x = transfer(GPU(0), Tensor(Dense(Element(0.0)), xh))
+    for lvl in (AtomicElement(0.0), Mutex(Element(0.0))):
+        y = transfer(GPU(0), Tensor(Dense(lvl), np.zeros(42)))
+        fi.finch("y[i] += A[i, j] * x[j]", y=y, A=A, x=x)
+        assert rel_err(y.todense(), S @ xh) < 1e-10, lvl
+    y = Tensor(Dense(Separate(Element(0.0))))
+    fi.finch("y[i] += A[j, i] * x[j]", y=y, A=A, x=x)
+    assert y.shape == (42,) and len(y.lvl.lvl.val) == 42
+    assert rel_err(y.todense(), S.T @ xh) < 1e-10
+    # A declared Separate output with another number of positions is a
+    # DimensionMismatch, as Finch's dimension check raises.
+    from finchgpu._lib import DimensionMismatch
+    z = transfer(GPU(0), Tensor(Dense(Separate(Element(0.0))), np.zeros(41)))
+    with pytest.raises(DimensionMismatch):
+        fi.finch("z[i] += A[j, i] * x[j]", z=z, A=A, x=x)
+    # SpMM into Dense(Dense(Mutex(Element))) (parallel_tests.jl:331).
+    Bh = np.random.default_rng(43).random((42, 42))
+    B = transfer(GPU(0), Tensor(Dense(Dense(Element(0.0))), Bh))
+    Ct = transfer(GPU(0), Tensor(Dense(Dense(Mutex(Element(0.0)))), np.zeros((42, 42))))
+    fi.finch("Ct[i, j] += A[i, k] * B[k, j]", Ct=Ct, A=A, B=B)
+    assert rel_err(Ct.todense(), S @ Bh) < 1e-10

@@ -64,3 +64,27 @@ def test_parse(prog, expect):
 def test_unrecognised_programs_raise():
     with pytest.raises(UnsupportedError):
         interface.parse("y[i] = sin(A[i, j])")
+
+
+def test_threaded_write_levels_are_dense_outputs():
+    # The output levels of Finch's threaded SpMV/SpMM tests
+    # (test/suites/parallel_tests.jl:603-690; benchmark/benchmarks.jl:402-426):
+    # AtomicElement and Mutex(Element) hold one value buffer like Element;
+    # Separate(Element) gives each position its own Element.
+    from finchgpu.tensors import AtomicElement, Mutex, Separate
+    x = np.array([1.5, 0.0, -2.0])
+    for lvl in (AtomicElement(0.0), Mutex(Element(0.0)), Mutex(AtomicElement(0.0))):
+        t = Tensor(Dense(lvl), x)
+        assert t.format == "dense" and t.shape == (3,)
+        assert np.array_equal(t.todense(), x)
+    M = np.arange(6.0).reshape(2, 3)
+    t = Tensor(Dense(Dense(Mutex(Element(0.0)))), M)
+    assert t.format == "dense2" and t.shape == (2, 3)
+    assert np.array_equal(t.todense(), M)
+    s = Tensor(Dense(Separate(Element(0.0))), x)
+    assert s.format == "separate" and s.shape == (3,)
+    assert [e.val.tolist() for e in s.lvl.lvl.val] == [[1.5], [0.0], [-2.0]]
+    assert np.array_equal(s.todense(), x)
+    # A Separate over anything but Element is not a backend format.
+    with pytest.raises(NotImplementedError):
+        Tensor(Dense(Separate(Dense(Element(0.0))))).format
