@@ -401,7 +401,15 @@ def bench_variants(dev, A, ptr, idx, val, B, x, flush, steps, oracle, nth):
     del A64, C
     # CSC: the same buffers as 1M column fibres; y = A^T x.
     yc = torch.empty(n, dtype=torch.float32, device=dev)
-    ta = timed(lambda: ops.spmv_csc(A, x, yc), steps, 2, flush)
+    # fgx_spmv_csc takes its row-bucket path at this size; FGX_CSC_BUCKETS=0
+    # times the atomic scatter kernel on the same call.
+    tb = timed(lambda: ops.spmv_csc(A, x, yc), steps, 2, flush)
+    yb = yc.clone()
+    os.environ["FGX_CSC_BUCKETS"] = "0"
+    try:
+        ta = timed(lambda: ops.spmv_csc(A, x, yc), steps, 2, flush)
+    finally:
+        del os.environ["FGX_CSC_BUCKETS"]
     ya = yc.clone()
     to = timed(lambda: ops.spmv_csc_ordered(A, x, yc), max(5, steps // 4), 2, flush)
     yo1 = yc.clone()
@@ -409,7 +417,10 @@ def bench_variants(dev, A, ptr, idx, val, B, x, flush, steps, oracle, nth):
     refc = oracle.spmv_csc_atomic(n, ptr, idx, v64, x.cpu().numpy().astype(np.float64),
                                   nthreads=nth)
     bc = algorithmic_bytes(n, m, nnz, 1, "spmv")
-    out["csc_spmv"] = {"atomic_ms": round(ta["event_ms"], 5), "atomic_gbs": gbs(bc, ta["event_ms"]),
+    out["csc_spmv"] = {"buckets_ms": round(tb["event_ms"], 5),
+                       "buckets_gbs": gbs(bc, tb["event_ms"]),
+                       "buckets_err_vs_fp64": _normwise(yb, refc),
+                       "atomic_ms": round(ta["event_ms"], 5), "atomic_gbs": gbs(bc, ta["event_ms"]),
                        "ordered_ms": round(to["event_ms"], 5),
                        "ordered_gbs": gbs(bc, to["event_ms"]),
                        "atomic_err_vs_fp64": _normwise(ya, refc),

@@ -1,10 +1,14 @@
 // fgx_spmv.hip — SpMV entry points (CSR and CSC).
 //
 // CSR SpMV / SpMM use the nnz-balanced wave tiles of fgx_csr_tiles.hpp plus a
-// fix-up launch for rows that span tiles.  CSC SpMV scatters into y with
-// hardware float atomics over the same nnz-balanced tiles (the form Finch
-// parallelises with AtomicElement, benchmark/benchmarks.jl:402-413).
+// fix-up launch for rows that span tiles.  CSC SpMV (the form Finch
+// parallelises with AtomicElement, benchmark/benchmarks.jl:402-413) scatters
+// into y through row buckets for large operands and with hardware float
+// atomics over the same nnz-balanced tiles for small ones.
 #include "fgx_launch.hpp"
+#include "fgx_scan.hpp"
+
+#include <cstdlib>
 
 namespace fgx {
 
@@ -87,6 +91,292 @@ __global__ __launch_bounds__(256) void spmv_csc_kernel(
       atomic_add<TA>(y + (static_cast<int64_t>(jj[s]) - base), widen<TA>(vv[s]) * xc);
     }
   }
+}
+
+// ---- CSC SpMV by row buckets --------------------------------------------------
+// Large CSC operands scatter through row buckets instead of global atomics:
+// a device float atomic add on MI355X retires at ~21 G/s whatever its scope
+// or target copy, so 16M scattered adds cost ~0.77 ms, while the same
+// products written once to per-bucket runs and summed in LDS cost a few
+// passes over 8-16 bytes per stored entry.
+//   1. count: per block of 256 * csc_run positions, how many rows fall in
+//      each bucket of 2^RB rows (LDS histogram);
+//   2. scan: bucket-major exclusive scan of the counts, so each bucket's
+//      entries are one contiguous run and each block owns a slice of it;
+//   3. scatter: the same blocks walk their columns (the wave-tile walk of
+//      the atomic kernel), form each product A[i,j]*x[j], sort it into the
+//      block's bucket order in LDS and write the block's slices out as
+//      contiguous runs;
+//   4. accumulate: S blocks per bucket add a share of its run into an LDS
+//      copy of the bucket's rows (LDS float atomics); with S = 1 they write
+//      those rows of y, otherwise S partial rows that a last pass sums in
+//      order.
+// Like the atomic kernel (Finch's AtomicElement form), the order of the adds
+// into each y[i] is not fixed.
+constexpr int64_t kCscMaxBuckets = 1024;   // scatter LDS: a cursor and a base per bucket
+constexpr int kCscWinCap = 256;            // scatter LDS: fibre window entries per wave
+constexpr int64_t kCscMinNnz = int64_t(1) << 18;  // below: atomic kernel
+
+template <typename TA>
+struct CscEntry {
+  uint32_t r;  // row within the bucket (in the scatter's LDS stage: the row)
+  TA v;        // A[i,j] * x[j]
+};
+// Rows per bucket (the accumulate block's LDS copy is 32 KB) and positions
+// per lane (the scatter's LDS stage is 32 KB).
+template <typename TA>
+constexpr int csc_row_bits() { return sizeof(TA) == 8 ? 12 : 13; }
+template <typename TA>
+constexpr int csc_run() { return sizeof(TA) == 8 ? 8 : 16; }
+
+// Rows outside [0, m) are skipped by both the count and the scatter, so the
+// slices stay consistent and no LDS slot outside the buckets is touched.
+template <typename TI, int RB, int RUN>
+__global__ __launch_bounds__(256) void csc_bucket_count_kernel(const TI* __restrict__ idx,
+                                                               int64_t base, int64_t nnz,
+                                                               int64_t m, int64_t nb, int64_t nblk,
+                                                               int32_t* __restrict__ counts) {
+  __shared__ int32_t h[kCscMaxBuckets];
+  for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) h[b] = 0;
+  __syncthreads();
+  const int64_t q0 = static_cast<int64_t>(blockIdx.x) * 256 * RUN;
+#pragma unroll 4
+  for (int s = 0; s < RUN; s++) {
+    const int64_t q = q0 + s * 256 + threadIdx.x;
+    if (q < nnz) {
+      const int64_t row = static_cast<int64_t>(ld_stream(idx + q)) - base;
+      if (static_cast<uint64_t>(row) < static_cast<uint64_t>(m)) atomicAdd(&h[row >> RB], 1);
+    }
+  }
+  __syncthreads();
+  for (int64_t b = threadIdx.x; b < nb; b += blockDim.x) counts[b * nblk + blockIdx.x] = h[b];
+}
+
+template <typename TP, typename TI, typename TV, typename TA, int RB, int RUN>
+__global__ __launch_bounds__(256) void csc_bucket_scatter_kernel(
+    const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
+    int64_t base, int64_t n_outer, int64_t nnz, const TA* __restrict__ x, int64_t m, int64_t nb,
+    int64_t nblk, const int32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+    const int64_t* __restrict__ table, int vec, CscEntry<TA>* __restrict__ out) {
+  constexpr int BP = 256 * RUN;
+  constexpr int64_t W = int64_t(kWave) * RUN;
+  constexpr int PER = static_cast<int>(kCscMaxBuckets / 256);
+  __shared__ CscEntry<TA> stage[BP];
+  __shared__ int32_t cur[kCscMaxBuckets];      // next local slot of each bucket
+  __shared__ int64_t gbase[kCscMaxBuckets];    // global slot of local slot 0, per bucket
+  __shared__ int64_t lds[kWavesPerBlock][kCscWinCap];
+  __shared__ int32_t wsum[kWavesPerBlock + 1];
+  const int tid = threadIdx.x;
+  const int w = wave_in_block();
+  const int lane = lane_id();
+  // Loads that depend only on the block index go out first: this lane's run
+  // of coordinates and values, then the block's bucket counts and slice
+  // starts; the fibre window is located while they are in flight.
+  const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
+  const int64_t q0 = tile * W;
+  const int64_t qend = (q0 + W < nnz) ? q0 + W : nnz;
+  const int64_t a = q0 + static_cast<int64_t>(lane) * RUN;
+  const int64_t b = (a + RUN < qend) ? a + RUN : qend;
+  TV vv[RUN];
+  TI jj[RUN];
+  if (b - a == RUN) {
+    ld_run<TV, RUN>(val + a, vec != 0, vv);
+    ld_run<TI, RUN>(idx + a, vec != 0, jj);
+  } else {
+#pragma unroll
+    for (int s = 0; s < RUN; s++) {
+      const bool in = a + s < b;
+      vv[s] = in ? ld_stream(val + a + s) : TV{};
+      jj[s] = in ? ld_stream(idx + a + s) : static_cast<TI>(base);
+    }
+  }
+  int cnt[PER];
+  int64_t off[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int64_t bk = static_cast<int64_t>(tid) * PER + k;
+    cnt[k] = bk < nb ? counts[bk * nblk + blockIdx.x] : 0;
+    off[k] = bk < nb ? offsets[bk * nblk + blockIdx.x] : 0;
+  }
+  FiberWindow<TP> win{};
+  int64_t c_lo = 0, c_hi = 0, pos_lo = 0;
+  if (q0 < nnz) win = tile_setup(ptr, base, n_outer, q0, qend, table, tile, lds[w], &c_lo, &c_hi,
+                                 &pos_lo, kCscWinCap);
+  // The block's bucket counts, scanned into local slots (bucket order).
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; k++) sum += cnt[k];
+  int incl = sum;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const int t = shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  if (lane == kWave - 1) wsum[w] = incl;
+  __syncthreads();
+  int ex = incl - sum;
+  for (int v = 0; v < w; v++) ex += wsum[v];
+  if (tid == 255) wsum[kWavesPerBlock] = ex + sum;  // the block's entry count
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int64_t bk = static_cast<int64_t>(tid) * PER + k;
+    if (bk < nb) {
+      cur[bk] = ex;
+      gbase[bk] = off[k] - ex;
+    }
+    ex += cnt[k];
+  }
+  __syncthreads();
+  // Products, staged in bucket order.
+  if (a < qend) {
+    // Each position's column from the window (LDS only), then every x
+    // gather issued before the first is used.
+    const int64_t c_last = c_hi < n_outer ? c_hi : n_outer - 1;
+    int64_t c = win.find(a);
+    int64_t col[RUN];
+#pragma unroll
+    for (int s = 0; s < RUN; s++) {
+      while (c < c_last && win.end_of(c) <= a + s) c++;
+      col[s] = c;
+    }
+    TA xs[RUN];
+#pragma unroll
+    for (int s = 0; s < RUN; s++) xs[s] = x[col[s]];
+#pragma unroll
+    for (int s = 0; s < RUN; s++) {
+      const int64_t row = static_cast<int64_t>(jj[s]) - base;
+      if (a + s < b && static_cast<uint64_t>(row) < static_cast<uint64_t>(m)) {
+        const int at = atomicAdd(&cur[row >> RB], 1);
+        stage[at].r = static_cast<uint32_t>(row);
+        stage[at].v = widen<TA>(vv[s]) * xs[s];
+      }
+    }
+  }
+  __syncthreads();
+  // The block's slices, each a contiguous run of its bucket.
+  const int total = wsum[kWavesPerBlock];
+  constexpr uint32_t kMask = (1u << RB) - 1;
+  for (int i = tid; i < total; i += 256) {
+    const CscEntry<TA> e = stage[i];
+    CscEntry<TA> o;
+    o.r = e.r & kMask;
+    o.v = e.v;
+    out[gbase[e.r >> RB] + i] = o;
+  }
+}
+
+template <typename TA, int RB>
+__global__ __launch_bounds__(1024) void csc_bucket_accum_kernel(
+    const CscEntry<TA>* __restrict__ in, const int64_t* __restrict__ offsets, int64_t nblk,
+    int64_t m, int split, TA* __restrict__ y, TA* __restrict__ partial) {
+  __shared__ TA acc[1 << RB];
+  for (int i = threadIdx.x; i < (1 << RB); i += blockDim.x) acc[i] = TA(0);
+  __syncthreads();
+  const int64_t b = blockIdx.x / split;
+  const int64_t sp = blockIdx.x % split;
+  const int64_t e0 = offsets[b * nblk], e1 = offsets[(b + 1) * nblk];
+  const int64_t lo = e0 + (e1 - e0) * sp / split, hi = e0 + (e1 - e0) * (sp + 1) / split;
+  int64_t q = lo + threadIdx.x;
+  constexpr int U = 8;
+  for (; q + (U - 1) * 1024 < hi; q += U * 1024) {
+    CscEntry<TA> t[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) t[u] = in[q + u * 1024];
+#pragma unroll
+    for (int u = 0; u < U; u++) atomicAdd(&acc[t[u].r], t[u].v);
+  }
+  for (; q < hi; q += 1024) {
+    const CscEntry<TA> t = in[q];
+    atomicAdd(&acc[t.r], t.v);
+  }
+  __syncthreads();
+  TA* dst = split == 1 ? y : partial + sp * m;
+  const int64_t r0 = b << RB;
+  for (int i = threadIdx.x; i < (1 << RB) && r0 + i < m; i += blockDim.x) dst[r0 + i] = acc[i];
+}
+
+// y[i] = sum of the split partial rows, in split order.
+template <typename TA>
+__global__ __launch_bounds__(256) void csc_partial_sum_kernel(const TA* __restrict__ partial,
+                                                              int split, int64_t m,
+                                                              TA* __restrict__ y) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += stride) {
+    TA acc = partial[i];
+    for (int s = 1; s < split; s++) acc += partial[s * m + i];
+    y[i] = acc;
+  }
+}
+
+// Whether a CSC SpMV takes the bucket path: large enough to amortise its
+// launches, and few enough buckets for the scatter's LDS slots.  The
+// environment variable FGX_CSC_BUCKETS forces it off (0) or on (1, when the
+// bucket count allows) for measurement and tests.
+template <typename TA>
+static bool csc_use_buckets(const fgx_compressed* A) {
+  const int64_t nb = (A->n_inner + (int64_t(1) << csc_row_bits<TA>()) - 1) >> csc_row_bits<TA>();
+  if (nb > kCscMaxBuckets || A->nnz == 0 || A->n_outer == 0 || A->n_inner == 0) return false;
+  const char* env = getenv("FGX_CSC_BUCKETS");
+  if (env && env[0] == '0') return false;
+  if (env && env[0] == '1') return true;
+  return A->nnz >= kCscMinNnz;
+}
+
+template <typename TP, typename TI, typename TV>
+static int spmv_csc_buckets(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
+                            hipStream_t s) {
+  using TA = typename acc_of<TV>::type;
+  constexpr int RB = csc_row_bits<TA>();
+  constexpr int RUN = csc_run<TA>();
+  constexpr int64_t BP = 256 * RUN;
+  const int64_t m = A->n_inner;
+  const int64_t nb = (m + (int64_t(1) << RB) - 1) >> RB;
+  const int64_t nblk = (A->nnz + BP - 1) / BP;
+  const int64_t ncnt = nb * nblk;
+  // Enough accumulate blocks for two per CU.
+  const int64_t cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+  const int64_t want = (2 * cus + nb - 1) / nb;
+  const int split = static_cast<int>(want < 1 ? 1 : (want > 16 ? 16 : want));
+  auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+  const size_t cb = up(static_cast<size_t>(ncnt) * sizeof(int32_t));
+  const size_t ob = up(static_cast<size_t>(ncnt + 1) * sizeof(int64_t));
+  const size_t wb = up(scan_work_bytes(ncnt));
+  const size_t pb = split > 1 ? up(static_cast<size_t>(split) * m * sizeof(TA)) : 0;
+  const size_t eb = static_cast<size_t>(A->nnz) * sizeof(CscEntry<TA>);
+  void* arena;
+  FGX_TRY(dense_scratch(ctx, s, cb + ob + wb + pb + eb, &arena));
+  char* p = static_cast<char*>(arena);
+  int32_t* counts = reinterpret_cast<int32_t*>(p);
+  int64_t* offsets = reinterpret_cast<int64_t*>(p + cb);
+  void* work = p + cb + ob;
+  TA* partial = reinterpret_cast<TA*>(p + cb + ob + wb);
+  CscEntry<TA>* entries = reinterpret_cast<CscEntry<TA>*>(p + cb + ob + wb + pb);
+  const int64_t ib = static_cast<int64_t>(A->index_base);
+  hipLaunchKernelGGL((csc_bucket_count_kernel<TI, RB, RUN>), dim3(static_cast<unsigned>(nblk)),
+                     dim3(256), 0, s, static_cast<const TI*>(A->idx), ib, A->nnz, m, nb, nblk,
+                     counts);
+  FGX_TRY(launch_check("csc_bucket_count_kernel"));
+  FGX_TRY((scan_counts_to_ptr<int64_t, int32_t>(counts, ncnt, 0, offsets, work, nullptr, nullptr,
+                                                s)));
+  const int vec = aligned16(A->val) && aligned16(A->idx);
+  const int64_t* table = nullptr;
+  FGX_TRY(tile_table(ctx, A, int64_t(kWave) * RUN, s, &table));
+  hipLaunchKernelGGL((csc_bucket_scatter_kernel<TP, TI, TV, TA, RB, RUN>),
+                     dim3(static_cast<unsigned>(nblk)), dim3(256), 0, s,
+                     static_cast<const TP*>(A->ptr), static_cast<const TI*>(A->idx),
+                     static_cast<const TV*>(A->val), ib, A->n_outer, A->nnz,
+                     static_cast<const TA*>(x), m, nb, nblk, counts, offsets, table, vec,
+                     entries);
+  FGX_TRY(launch_check("csc_bucket_scatter_kernel"));
+  hipLaunchKernelGGL((csc_bucket_accum_kernel<TA, RB>), dim3(static_cast<unsigned>(nb * split)),
+                     dim3(1024), 0, s, entries, offsets, nblk, m, split, static_cast<TA*>(y),
+                     partial);
+  FGX_TRY(launch_check("csc_bucket_accum_kernel"));
+  if (split == 1) return FGX_OK;
+  const int64_t blocks = ((m + 255) / 256) < 2048 ? (m + 255) / 256 : 2048;
+  hipLaunchKernelGGL((csc_partial_sum_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256),
+                     0, s, partial, split, m, static_cast<TA*>(y));
+  return launch_check("csc_partial_sum_kernel");
 }
 
 // ---------------------------------------------------------------------------
@@ -190,11 +480,12 @@ static int spmv_csr_noalloc_typed(fgx_ctx* ctx, const fgx_compressed* A, const v
 template <typename TP, typename TI, typename TV>
 static int spmv_csc_typed(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y,
                           hipStream_t s) {
-  (void)ctx;
   using TA = typename acc_of<TV>::type;
   constexpr int V = spmv_v<TV>::value;
   const int64_t W = kWave * V;
   const int64_t ni = A->n_inner;
+  // Every row of y is written by the bucket path's accumulate step.
+  if (csc_use_buckets<TA>(A)) return spmv_csc_buckets<TP, TI, TV>(ctx, A, x, y, s);
   if (ni > 0) {
     const int64_t blocks = ((ni + 255) / 256) < 4096 ? (ni + 255) / 256 : 4096;
     hipLaunchKernelGGL((fill_kernel<TA>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,

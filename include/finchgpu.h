@@ -178,7 +178,12 @@ int fgx_spmv_csr_noalloc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, v
 /* y = A*x with A read column-wise (CSC): y .= 0; for j, i: y[i] += A[i,j]*x[j].
  * Replaces test/reference64/typical/typical_spmv_csc.txt:26-83
  * (program test/suites/typical_tests.jl:47-73).  x has n_outer entries, y has
- * n_inner. */
+ * n_inner.  The adds into each y[i] come in no fixed order, as with Finch's
+ * AtomicElement form (benchmark/benchmarks.jl:402-413): operands of at least
+ * 2^18 stored entries (and at most 1024 buckets of 8192 FP32 / 4096 FP64
+ * rows) go through row buckets summed in LDS, smaller ones through device
+ * atomic adds.  The bucket path uses nnz * 8 (FP32) or 16 (FP64) bytes of the
+ * stream's scratch. */
 int fgx_spmv_csc(fgx_ctx* ctx, const fgx_compressed* A, const void* x, void* y, void* stream);
 
 /* C = A*B, A CSR (n_outer x n_inner), B n_inner x k, C n_outer x k:
