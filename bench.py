@@ -85,6 +85,8 @@ def main():
                          "inputs) and print its JSON (for per-class profiler passes)")
     ap.add_argument("--no-loop", action="store_true",
                     help="skip the batched section's per-matrix loop (for profiler passes)")
+    ap.add_argument("--atomic-probe", action="store_true",
+                    help="only time the scatter-add probes (device atomics, LDS atomics)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy B (256 MB) once: a kernel of known traffic that calibrates "
                          "rocprofv3 byte counters")
@@ -102,6 +104,9 @@ def main():
     from finchgpu import ops, synth
 
     dev = torch.device("cuda:0")
+    if args.atomic_probe:
+        print(json.dumps({"atomic_probe": bench_atomic_probe(dev)}), flush=True)
+        return
     if args.batch_only:
         print(json.dumps({"batched": bench_batched(dev, args.batch_count, not args.no_loop)}),
               flush=True)
@@ -343,6 +348,60 @@ def bench_gather_floor(A, B, flush, steps):
             "gathered_tbs": round(nnz * 256 / (best * 1e-3) / 1e12, 3),
             "by_shape_ms": {k: round(v, 5) for k, v in shapes.items()},
             "rows_of_B_folded_to": folded}
+
+
+def bench_atomic_probe(dev, n=16_000_000, m=1_000_000):
+    """The two accumulation forms a CSC SpMV can use, apart from any SpMV:
+    n random float adds into a 1M-entry y through device atomics (device
+    scope; workgroup or device scope into one copy of y per XCD), and n
+    (row, value) entries added into an 8,192-entry LDS array (float atomics,
+    integer atomics, plain writes).  Every float add is 1.0, so y sums to n."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    lib = ctypes.CDLL(os.path.join(HERE, "finch.jl_amd", "finchgpu", "libfgx_probe.so"))
+    lib.fgx_probe_atomic_scatter.restype = ctypes.c_int
+    lib.fgx_probe_atomic_scatter.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                             ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_void_p]
+    lib.fgx_probe_lds_accumulate.restype = ctypes.c_int
+    lib.fgx_probe_lds_accumulate.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                             ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                             ctypes.c_void_p]
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    rows = torch.randint(0, m, (n,), dtype=torch.int32, device=dev, generator=g)
+    vals = torch.ones(n, dtype=torch.float32, device=dev)
+    y = torch.zeros(8 * m, dtype=torch.float32, device=dev)
+    flushbuf = torch.zeros(128 << 20, dtype=torch.float32, device=dev)
+    stream = lambda: torch.cuda.current_stream().cuda_stream
+    out = {}
+    for mode, name in ((0, "device_scope"), (1, "per_xcd_copy_workgroup_scope"),
+                       (2, "per_xcd_copy_device_scope")):
+        def run():
+            if lib.fgx_probe_atomic_scatter(rows.data_ptr(), vals.data_ptr(), n, y.data_ptr(), m,
+                                            mode, 16, stream()) != 0:
+                raise RuntimeError("atomic probe failed")
+        y.zero_()
+        run()
+        total = float(y.double().sum())
+        t = timed(run, 10, 1, lambda: flushbuf.add_(1.0))["event_ms"]
+        out[f"global_atomic_{name}"] = {"ms": round(t, 4), "g_adds_per_s": round(n / t / 1e6, 1),
+                                        "sum_ok": total == float(n)}
+    entries = torch.stack([rows & 8191, vals.view(torch.int32)], dim=1).contiguous()
+    blocks = 512
+    res = torch.empty(blocks * 1024, dtype=torch.float32, device=dev)
+    for mode, name in ((0, "float_atomic"), (1, "int_atomic"), (2, "plain_write")):
+        def run():
+            if lib.fgx_probe_lds_accumulate(entries.data_ptr(), n, mode, res.data_ptr(),
+                                            res.numel(), blocks, stream()) != 0:
+                raise RuntimeError("LDS probe failed")
+        t = timed(run, 10, 1)["event_ms"]
+        out[f"lds_{name}"] = {"ms": round(t, 4), "g_ops_per_s": round(n / t / 1e6, 1)}
+    del flushbuf
+    return {"entries": n, "rows": m, **out}
 
 
 def _cpu_median(fn, reps=10):
