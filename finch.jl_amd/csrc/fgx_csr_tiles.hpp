@@ -578,7 +578,7 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
                                           const int64_t* __restrict__ table, int64_t* lds_win,
                                           int64_t* carry_row_out,
                                           TA* carry_val_out /* G*KV entries */,
-                                          TI* lds_j, TA* lds_v) {
+                                          TI* lds_j, TA* lds_v, bool b_off32) {
   constexpr int NG = kWave / G;
   constexpr int W = NG * E;
   static_assert(E % G == 0, "E must be a multiple of G");
@@ -643,37 +643,71 @@ __device__ __forceinline__ void spmm_tile(const TP* __restrict__ ptr, const TI* 
     for (int c = 0; c < KV; c++) acc[c] = 0;
   };
 
+  // The end of the row the walk is in, kept in a register so that a
+  // position that ends no row costs one compare (no LDS read).  Groups
+  // without positions never advance.
+  constexpr int64_t kNoEnd = INT64_MAX;
+  auto end_at = [&](int64_t row) {
+    return (row < m && row <= r_win_last) ? win.end_of(row) : kNoEnd;
+  };
+  int64_t r_end = active ? end_at(r) : kNoEnd;
+  auto advance = [&](int64_t q) {
+    while (r_end <= q) {
+      emit(r);
+      r++;
+      r_end = end_at(r);
+    }
+  };
+
   // Batches of U positions; the group's lanes read each position's
   // coordinate and value from LDS (broadcast) and gather their B slices.
+  // When every lane's slice is whole and the tile holds positions, the
+  // gathers take no branch (b_off32: every row of B starts within 4 GB of
+  // B, so byte offsets fit 32 bits): padding past the tile end is staged as
+  // coordinate `base` with value 0, so it reads B's first row, whose values
+  // are then masked to 0 (an Inf or NaN there must not reach acc).  Only
+  // padding positions lie past `b`, and only in the last tile, where every
+  // row ending there is the tile's to emit.
   const TI* gj = lds_j + g * E;
   const TA* gv = lds_v + g * E;
+  const bool whole = kofs + static_cast<int64_t>(G) * KV <= k && qend > q0 && b_off32;
 #pragma unroll 1
   for (int e0 = 0; e0 < E; e0 += U) {
     DenseRaw<KV, TB> bv[U];
     TA ve[U];
+    if (whole) {
+      // Byte offsets from the slice's start fit 32 bits here, so each
+      // gather takes one register for its address.
+      const char* Bs = reinterpret_cast<const char*>(B + col0);
+      const uint32_t row_bytes = static_cast<uint32_t>(ldb) * static_cast<uint32_t>(sizeof(TB));
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
-      ve[u] = gv[e0 + u];
-      const bool in = (a + e0 + u < b) && valid_cols > 0;
-      if (in) ld_raw<KV, TB>(B + je * ldb + col0, valid_cols, bv[u]);
-      else zero_raw<KV, TB>(bv[u]);
+      for (int u = 0; u < U; u++) {
+        const uint32_t je = static_cast<uint32_t>(static_cast<int64_t>(gj[e0 + u]) - base);
+        ve[u] = gv[e0 + u];
+        ld_raw<KV, TB>(reinterpret_cast<const TB*>(Bs + je * row_bytes), KV, bv[u]);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int64_t je = static_cast<int64_t>(gj[e0 + u]) - base;
+        ve[u] = gv[e0 + u];
+        const bool in = (a + e0 + u < b) && valid_cols > 0;
+        if (in) ld_raw<KV, TB>(B + je * ldb + col0, valid_cols, bv[u]);
+        else zero_raw<KV, TB>(bv[u]);
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int64_t q = a + e0 + u;
-      if (q < b) {
-        while (r < m && r <= r_win_last && win.end_of(r) <= q) { emit(r); r++; }
-        TA bw[KV];
-        widen_raw<KV, TB, TA>(bv[u], bw);
+      advance(q);
+      TA bw[KV];
+      widen_raw<KV, TB, TA>(bv[u], bw);
+      const bool in = q < b;
 #pragma unroll
-        for (int c = 0; c < KV; c++) acc[c] += ve[u] * bw[c];
-      }
+      for (int c = 0; c < KV; c++) acc[c] += ve[u] * (in ? bw[c] : TA(0));
     }
   }
-  if (active) {
-    while (r < m && r <= r_win_last && win.end_of(r) <= b) { emit(r); r++; }
-  }
+  advance(b);
 
   // Segmented scan of the groups' carries (same rule as spmv_tile).
   TA v[KV];

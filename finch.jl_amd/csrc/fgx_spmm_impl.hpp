@@ -12,15 +12,36 @@
 #ifndef FGX_SPMM_U
 #define FGX_SPMM_U 8
 #endif
+// B rows per gather batch for wide groups when B is larger than
+// FGX_SPMM_SMALL_B bytes.  Measured on the 1M-row Zipf(1.5) operand with its
+// columns folded onto fewer rows of B (k = 64): with B at 4 MB (L2-resident)
+// 8 rows per batch take 0.297 ms and 16 take 0.357 ms; with B at 64 MB
+// 0.589 vs 0.573 ms, and at 256 MB 0.617 vs 0.604 ms.  Larger batches keep
+// more gathers in flight per wave where they come from HBM; smaller ones
+// leave room for more waves where L2 serves them.
+#ifndef FGX_SPMM_U_LARGE_B
+#define FGX_SPMM_U_LARGE_B 16
+#endif
+#ifndef FGX_SPMM_SMALL_B
+#define FGX_SPMM_SMALL_B (int64_t(16) << 20)
+#endif
 
 namespace fgx {
 
+// FGX_SPMM_WAVES (measurement builds): resident waves per SIMD to hold the
+// tile kernel's register budget to.
+#ifdef FGX_SPMM_WAVES
+#define FGX_SPMM_OCCUPANCY __attribute__((amdgpu_waves_per_eu(FGX_SPMM_WAVES)))
+#else
+#define FGX_SPMM_OCCUPANCY
+#endif
+
 template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int E, int U>
-__global__ __launch_bounds__(256) void spmm_csr_kernel(
+__global__ __launch_bounds__(256) FGX_SPMM_OCCUPANCY void spmm_csr_kernel(
     const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
     int64_t base, int64_t m, int64_t nnz, int64_t k, int64_t kofs, const TA* __restrict__ B,
     int64_t ldb, TA* __restrict__ C, int64_t ldc, int64_t ntiles, const int64_t* __restrict__ table,
-    int64_t* __restrict__ carry_row, TA* __restrict__ carry_val) {
+    int64_t* __restrict__ carry_row, TA* __restrict__ carry_val, int b_off32) {
   __shared__ int64_t lds[kWavesPerBlock][kWinCap];
   const int w = wave_in_block();
   const int64_t tile = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + w;
@@ -30,7 +51,8 @@ __global__ __launch_bounds__(256) void spmm_csr_kernel(
   __shared__ TA lds_v[kWavesPerBlock][W];
   spmm_tile<TP, TI, TV, TA, G, KV, E, U>(ptr, idx, val, base, m, nnz, k, kofs, B, ldb, C, ldc,
                                          tile, table, lds[w], carry_row + tile,
-                                         carry_val + tile * (G * KV), lds_j[w], lds_v[w]);
+                                         carry_val + tile * (G * KV), lds_j[w], lds_v[w],
+                                         b_off32 != 0);
 }
 
 // General-layout SpMM (any strides): one wave per row, lanes over columns.
@@ -55,15 +77,14 @@ __global__ __launch_bounds__(256) void spmm_csr_strided_kernel(
   }
 }
 
-// SpMM dispatch over the dense width: lane groups of G lanes x KV columns.
-template <typename TP, typename TI, typename TV, typename TA, int G, int KV>
-static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t kofs,
-                       const TA* B, int64_t ldb, TA* C, int64_t ldc, hipStream_t s) {
+// SpMM dispatch over the dense width: lane groups of G lanes x KV columns,
+// U rows of B per gather batch.
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV, int U>
+static int spmm_launch_u(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t kofs,
+                         const TA* B, int64_t ldb, TA* C, int64_t ldc, hipStream_t s) {
   constexpr int NG = kWave / G;
-  // Positions per lane group (E) and B rows per gather batch (U); wide
-  // groups take FGX_SPMM_E / FGX_SPMM_U.
+  // Positions per lane group (E); wide groups take FGX_SPMM_E.
   constexpr int E = (G >= 32) ? tile_e_for_g(G) : ((G >= 16) ? FGX_SPMM_E : (G >= 4 ? 32 : 16));
-  constexpr int U = G < 8 ? G : (FGX_SPMM_U < G ? FGX_SPMM_U : G);
   constexpr int64_t W = static_cast<int64_t>(NG) * E;
   const int64_t m = A->n_outer;
   const int64_t ntiles = A->nnz > 0 ? (A->nnz + W - 1) / W : 1;
@@ -75,14 +96,28 @@ static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t
   const int64_t* table = nullptr;
   FGX_TRY(tile_table(ctx, A, W, s, &table));
   const int64_t blocks = (ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+  // Whether every row of B starts within 4 GB of the slice's start.
+  const int b_off32 = (A->n_inner > 0 ? (A->n_inner - 1) * ldb * static_cast<int64_t>(sizeof(TA))
+                                       : 0) < (int64_t(1) << 32) - 4096;
   hipLaunchKernelGGL((spmm_csr_kernel<TP, TI, TV, TA, G, KV, E, U>),
                      dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s,
                      static_cast<const TP*>(A->ptr), static_cast<const TI*>(A->idx),
                      static_cast<const TV*>(A->val), static_cast<int64_t>(A->index_base), m,
-                     A->nnz, k, kofs, B, ldb, C, ldc, ntiles, table, crow, cval);
+                     A->nnz, k, kofs, B, ldb, C, ldc, ntiles, table, crow, cval, b_off32);
   FGX_TRY(launch_check("spmm_csr_kernel"));
   if (ntiles > 1) FGX_TRY(launch_fixup<TA>(s, crow, cval, ntiles, width, C, ldc, kofs, k));
   return FGX_OK;
+}
+
+template <typename TP, typename TI, typename TV, typename TA, int G, int KV>
+static int spmm_launch(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, int64_t kofs,
+                       const TA* B, int64_t ldb, TA* C, int64_t ldc, hipStream_t s) {
+  constexpr int U = G < 8 ? G : (FGX_SPMM_U < G ? FGX_SPMM_U : G);
+  constexpr int UL = G < 16 ? U : (FGX_SPMM_U_LARGE_B < G ? FGX_SPMM_U_LARGE_B : G);
+  const int64_t b_bytes = A->n_inner * ldb * static_cast<int64_t>(sizeof(TA));
+  if (UL != U && b_bytes > FGX_SPMM_SMALL_B)
+    return spmm_launch_u<TP, TI, TV, TA, G, KV, UL>(ctx, A, k, kofs, B, ldb, C, ldc, s);
+  return spmm_launch_u<TP, TI, TV, TA, G, KV, U>(ctx, A, k, kofs, B, ldb, C, ldc, s);
 }
 
 template <typename TP, typename TI, typename TV>
