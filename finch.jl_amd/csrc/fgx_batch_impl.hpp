@@ -138,6 +138,11 @@ __device__ __forceinline__ void sddmm_unit(const BatchDesc& d, int64_t t0, int64
                                d.mfma != 0);
 }
 
+// Measurement switch: FP32 right-hand sides through the branch-free gathers too.
+#ifndef FGX_BATCH_BRANCHFREE_F32
+#define FGX_BATCH_BRANCHFREE_F32 0
+#endif
+
 // ---- TILES units ------------------------------------------------------------------------------
 // Positions per TILES tile for G lanes per position.
 template <int G>
@@ -162,7 +167,7 @@ __device__ __forceinline__ void tiles_spmm(const BatchDesc& d, int64_t t0, int64
         lj, lv,
         // Branch-free gathers for BF16 right-hand sides only: with FP32 ones
         // the batch bodies (held to 4 waves per SIMD) measured 4 % slower.
-        std::is_same<TD, bf16_t>::value &&
+        (std::is_same<TD, bf16_t>::value || FGX_BATCH_BRANCHFREE_F32) &&
             d.n * d.ldb * static_cast<int64_t>(sizeof(TD)) < (int64_t(1) << 32) - 4096);
 }
 
