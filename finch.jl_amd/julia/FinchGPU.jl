@@ -120,6 +120,18 @@ Finch.local_memory(dev::GPU) = GPUGlobalMemory(dev)
 Finch.shared_memory(dev::GPU) = GPUGlobalMemory(dev)
 Finch.global_memory(dev::GPU) = GPUGlobalMemory(dev)
 
+# Parallel regions.  Finch lowers `for i = parallel(_, device)` through
+# virtual_parallel_region (src/architecture.jl:685-824), which for a CPU
+# spreads the loop over host tasks.  A program over GPU-resident tensors never
+# reaches lowering: the execute hook below sends the whole program to its
+# routed kernel, whose launch covers the device.  A GPU handed to a parallel
+# loop of a program that is not routed therefore stops at lowering with
+# FGX_EUNSUP instead of generating host code for a device it cannot drive.
+struct VirtualGPU <: Finch.AbstractVirtualDevice end
+Finch.virtualize(ctx, ex, ::Type{GPU}) = VirtualGPU()
+Finch.virtual_parallel_region(f, ctx, ext, ::VirtualGPU, schedule) =
+    unsupported("a parallel region on a GPU device: only routed programs run on the GPU")
+
 """Release the context (its scratch and plan cache); `dev` is unusable afterwards."""
 function close!(dev::GPU)
     lock(DEVICES_LOCK) do
