@@ -30,11 +30,17 @@ constexpr int kSdWin = 256;      // fibre ends cached per wave
 // X and Y values to the matrix core.
 // TD is the dense factors' storage type: float, or BF16 (read 8 bytes per
 // 4-column slice and widened before staging).
+// `whole`: k is a multiple of 32 and the sub-tile's wave holds at least one
+// position.  The gathers then take no branch: a lane without a position reads
+// row 0 of X and Y (its i and j are 0), and only its own dot product, which is
+// never stored, sees those values (the diagonal of a 4x4 block takes each
+// lane's X and Y alone).
 template <typename TD>
 __device__ __forceinline__ void sddmm_sub_mfma(const TD* __restrict__ X, int64_t ldx,
                                                const TD* __restrict__ Y, int64_t ldy,
                                                int64_t k, bool in, int64_t i, int64_t j,
-                                               float (*ys)[kSdPitch], f32x4& c) {
+                                               float (*ys)[kSdPitch], f32x4& c,
+                                               bool whole = false) {
   constexpr int kLanesPerRow = kSdChunk / 4;          // 16-byte slices per staged row
   constexpr int kRowsPerLoad = kWave / kLanesPerRow;   // rows one wave load covers
   const int lane = lane_id();
@@ -93,21 +99,31 @@ __device__ __forceinline__ void sddmm_sub_mfma(const TD* __restrict__ X, int64_t
   for (int64_t c0 = 0; c0 < k; c0 += kSdChunk) {
     // Loads first (storage form), conversions after every load is issued.
     DenseRaw<4, TD> yr[kLanesPerRow];
-#pragma unroll
-    for (int s = 0; s < kLanesPerRow; s++) {
-      const int t = lane / kLanesPerRow + kRowsPerLoad * s;
-      const int64_t jt = shfl(j, t);
-      const int tin = shfl(static_cast<int>(in), t);
-      const int64_t col = c0 + 4 * part;
-      if (tin && col < k) ld_raw<4, TD>(Y + jt * ldy + col, 4, yr[s]);
-      else zero_raw<4, TD>(yr[s]);
-    }
     DenseRaw<4, TD> xr[kSdChunk / 4];
+    if (whole) {
 #pragma unroll
-    for (int q = 0; q < kSdChunk / 4; q++) {
-      const int64_t col = c0 + 4 * q;
-      if (in && col < k) ld_raw<4, TD>(X + i * ldx + col, 4, xr[q]);
-      else zero_raw<4, TD>(xr[q]);
+      for (int s = 0; s < kLanesPerRow; s++) {
+        const int t = lane / kLanesPerRow + kRowsPerLoad * s;
+        ld_raw<4, TD>(Y + shfl(j, t) * ldy + c0 + 4 * part, 4, yr[s]);
+      }
+#pragma unroll
+      for (int q = 0; q < kSdChunk / 4; q++) ld_raw<4, TD>(X + i * ldx + c0 + 4 * q, 4, xr[q]);
+    } else {
+#pragma unroll
+      for (int s = 0; s < kLanesPerRow; s++) {
+        const int t = lane / kLanesPerRow + kRowsPerLoad * s;
+        const int64_t jt = shfl(j, t);
+        const int tin = shfl(static_cast<int>(in), t);
+        const int64_t col = c0 + 4 * part;
+        if (tin && col < k) ld_raw<4, TD>(Y + jt * ldy + col, 4, yr[s]);
+        else zero_raw<4, TD>(yr[s]);
+      }
+#pragma unroll
+      for (int q = 0; q < kSdChunk / 4; q++) {
+        const int64_t col = c0 + 4 * q;
+        if (in && col < k) ld_raw<4, TD>(X + i * ldx + col, 4, xr[q]);
+        else zero_raw<4, TD>(xr[q]);
+      }
     }
     float4 xv[kSdChunk / 4];
 #pragma unroll
@@ -180,6 +196,7 @@ __device__ __forceinline__ void sddmm_tile(const TP* __restrict__ ptr, const TI*
   int64_t r_lo, r_hi, pos_lo;
   FiberWindow<TP> win =
       tile_setup(ptr, base, m, q0, qend, table, tile, lds_win, &r_lo, &r_hi, &pos_lo, kSdWin);
+  const bool whole = k % kSdChunk == 0 && qend > q0;  // wave-uniform
 #pragma unroll 1
   for (int sub = 0; sub < kSddmmSub; sub++) {
     const int64_t q = q0 + sub * kWave + lane;
@@ -194,7 +211,7 @@ __device__ __forceinline__ void sddmm_tile(const TP* __restrict__ ptr, const TI*
     float dot = 0.f;
     if (mfma) {
       f32x4 c = {0.f, 0.f, 0.f, 0.f};
-      sddmm_sub_mfma<TD>(X, ldx, Y, ldy, k, in, i, j, ys, c);
+      sddmm_sub_mfma<TD>(X, ldx, Y, ldy, k, in, i, j, ys, c, whole);
       dot = mfma_diag(c);
     } else if (in) {
       const TD* xr = X + i * ldx;
