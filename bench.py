@@ -87,6 +87,9 @@ def main():
                     help="skip the batched section's per-matrix loop (for profiler passes)")
     ap.add_argument("--atomic-probe", action="store_true",
                     help="only time the scatter-add probes (device atomics, LDS atomics)")
+    ap.add_argument("--serve-kinds", action="store_true",
+                    help="only measure the serving mix's device time per request by operand "
+                         "class and kind (saturated batched runs) and print its JSON")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first copy B (256 MB) once: a kernel of known traffic that calibrates "
                          "rocprofv3 byte counters")
@@ -118,6 +121,9 @@ def main():
     A = ops.Compressed(torch.from_numpy(ptr.astype(np.int32)).to(dev),
                        torch.from_numpy(idx.astype(np.int32)).to(dev),
                        torch.from_numpy(val).to(dev), M, N, base=1)
+    if args.serve_kinds:
+        print(json.dumps({"serving_by_kind": bench_serving_by_kind(dev, [A])}), flush=True)
+        return
     B = torch.from_numpy(synth.dense(N, K, seed=2)).to(dev)
     X = torch.from_numpy(synth.dense(M, K, seed=3)).to(dev)
     x = B[:, 0].contiguous()
@@ -835,14 +841,11 @@ def bench_batched(dev, count, with_loop=True):
     return out
 
 
-def bench_serving(dev, slo_ms, large):
-    """Serving form: open-loop Poisson arrivals, 60 % SpMV / 30 % SpMM (k 16 or
-    64) / 10 % SDDMM (k 32) over 256 operands of the batched size distribution
-    plus four 1M-row Zipf(1.5) matrices (the timed matrix and three row
-    permutations of it), HBM admission budget 200 GB.  For each path, the
-    offered rate is raised until p99 latency exceeds the target; the largest
-    achieved req/s within it is reported."""
-    import numpy as np
+def _serving_operands(dev, large):
+    """The serving mix's operands: 256 matrices of the batched size
+    distribution plus four 1M-row Zipf(1.5) matrices (the timed matrix and
+    three row permutations of it), with FP32 and BF16 values, and the FP32 and
+    BF16 dense inputs of every (operand, kind, width)."""
     import torch
 
     from finchgpu import ops, serve
@@ -867,27 +870,105 @@ def bench_serving(dev, slo_ms, large):
     operands16 = [A.to_values(torch.bfloat16) for A in operands32]
     inputs = serve.Inputs(operands32, seed=1)
     inputs16 = serve.Inputs(operands32, seed=1, dtype=torch.bfloat16)
+    return operands32, operands16, inputs, inputs16
+
+
+def bench_serving_by_kind(dev, large):
+    """Device time per request of the serving mix, by request kind and operand
+    class: for each kind (SpMV, SpMM k=16, SpMM k=64, SDDMM k=32) and each
+    operand class (the 256 small operands, the four 1M-row operands), a trace
+    of requests that all arrive at once is served in batched mode; requests /
+    span is the saturated rate.  The mix-weighted sum is the device time of an
+    average request, and each term's share of it shows where the device goes."""
+    import torch
+
+    from finchgpu import serve
+
+    operands32, operands16, inputs, inputs16 = _serving_operands(dev, large)
+    n_small = 256
+    p_large = (len(operands32) - n_small) / len(operands32)
+    kinds = (("spmv", (1.0, 0.0, 0.0), (16,), 0.6), ("spmm16", (0.0, 1.0, 0.0), (16,), 0.15),
+             ("spmm64", (0.0, 1.0, 0.0), (64,), 0.15), ("sddmm", (0.0, 0.0, 1.0), (16,), 0.1))
+
+    def alg_bytes(r, vb, db):
+        A = operands32[r.operand]
+        if r.kind == "sddmm":
+            io = (A.n_outer + A.n_inner) * r.k * db + A.nnz * 4
+        elif r.kind == "spmv":
+            io = A.n_inner * db + A.n_outer * 4
+        else:
+            io = A.n_inner * r.k * db + A.n_outer * r.k * 4
+        return A.nnz * (vb + 4) + 4 * (A.n_outer + 1) + io
+
+    variants = (("batched_fp32", operands32, inputs, torch.float32, 4, 4),
+                ("batched_bf16", operands16, inputs, torch.float32, 2, 4),
+                ("batched_bf16_values_and_inputs", operands16, inputs16, torch.bfloat16, 2, 2))
+    out = {}
+    for name, ops_, src, dt, vb, db in variants:
+        serve.serve(ops_, serve.make_trace(operands32, src, qps=20_000, n=2_000, seed=1), "batched",
+                    input_dtype=dt)  # warm-up (tile tables, plans)
+        rows, mean_us = {}, 0.0
+        for cls, n, p_cls in (("small", 8_000, 1.0 - p_large), ("large", 40_000, p_large)):
+            for kname, mix, ks, p_kind in kinds:
+                reqs = [r for r in serve.make_trace(operands32, src, qps=5e6, n=n, seed=3, mix=mix,
+                                                    spmm_k=ks)
+                        if (r.operand < n_small) == (cls == "small")]
+                res = serve.serve(ops_, reqs, "batched", input_dtype=dt)
+                span = float(res.done_s.max() - res.arrival_s.min())
+                us = span / len(reqs) * 1e6
+                mean_us += p_cls * p_kind * us
+                rows[f"{cls}_{kname}"] = {
+                    "requests": len(reqs), "us_per_request": round(us, 3),
+                    "alg_gbs": round(sum(alg_bytes(r, vb, db) for r in reqs) / span / 1e9, 1),
+                    "weight": round(p_cls * p_kind, 5)}
+        for r in rows.values():
+            r["share_of_device_time"] = round(r["weight"] * r["us_per_request"] / mean_us, 4)
+        out[name] = {"by_class_and_kind": rows, "mix_us_per_request": round(mean_us, 3),
+                     "saturated_req_per_s_estimate": round(1e6 / mean_us)}
+        print(f"# serving by kind {name}: {mean_us:.3f} us per request", file=sys.stderr,
+              flush=True)
+    out["method"] = ("per (operand class, kind): all requests arrive at t=0 and are served in "
+                     "batched mode; us_per_request = span / requests; weights: 60 % SpMV, "
+                     "15 % SpMM k=16, 15 % SpMM k=64, 10 % SDDMM k=32, operands uniform over "
+                     "256 small + 4 large")
+    return out
+
+
+def bench_serving(dev, slo_ms, large):
+    """Serving form: open-loop Poisson arrivals, 60 % SpMV / 30 % SpMM (k 16 or
+    64) / 10 % SDDMM (k 32) over 256 operands of the batched size distribution
+    plus four 1M-row Zipf(1.5) matrices (the timed matrix and three row
+    permutations of it), HBM admission budget 200 GB.  For each path, the
+    offered rate is raised until p99 latency exceeds the target; the largest
+    achieved req/s within it is reported, and for the batched paths also the
+    largest within the plain path's own p99 (the comparison at equal p99)."""
+    import numpy as np
+    import torch
+
+    from finchgpu import serve
+
+    operands32, operands16, inputs, inputs16 = _serving_operands(dev, large)
     in_of = {"fp32": (inputs, torch.float32), "bf16": (inputs16, torch.bfloat16)}
     qps_grid = [1_000, 2_000, 5_000, 10_000, 20_000, 30_000, 50_000, 70_000, 100_000, 140_000,
                 200_000, 300_000]
 
-    def run(ops_, mode, q, seed, inp="fp32"):
+    def run(ops_, mode, q, seed, inp, target):
         n = int(min(20_000, max(2_000, q * 0.25)))
         src, dt = in_of[inp]
         reqs = serve.make_trace(operands32, src, qps=q, n=n, seed=seed)
         s = serve.serve(ops_, reqs, mode, input_dtype=dt).summary()
         return {"offered_qps": q, "req_per_s": round(s["req_per_s"], 1),
                 "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
-                "launches": s["launches"], "ok": s["p99_ms"] <= slo_ms and s["rejected"] == 0}
+                "launches": s["launches"], "ok": s["p99_ms"] <= target and s["rejected"] == 0}
 
-    def passes(ops_, mode, q, points, inp):
+    def passes(ops_, mode, q, points, inp, target):
         # A rate passes when two runs (different arrival draws) both keep p99
         # within the target; the slower of the two is recorded.
-        a = run(ops_, mode, q, 7, inp)
+        a = run(ops_, mode, q, 7, inp, target)
         points.append(a)
         if not a["ok"]:
             return None
-        b = run(ops_, mode, q, 8, inp)
+        b = run(ops_, mode, q, 8, inp, target)
         points.append(b)
         if not b["ok"]:
             return None
@@ -930,11 +1011,11 @@ def bench_serving(dev, slo_ms, large):
         del rng
         return {"max": max(errs), "mean": float(f"{float(np.mean(errs)):.3e}"), "requests": len(errs)}
 
-    def sweep(ops_, mode, inp):
-        run(ops_, mode, 20_000, 1, inp)     # warm-up (plans, tables, scratch)
+    def sweep(ops_, mode, inp, target, grid):
+        run(ops_, mode, 20_000, 1, inp, target)     # warm-up (plans, tables, scratch)
         best, points, lo, hi = None, [], None, None
-        for q in qps_grid:
-            r = passes(ops_, mode, q, points, inp)
+        for q in grid:
+            r = passes(ops_, mode, q, points, inp, target)
             if r is None:
                 hi = q
                 break
@@ -943,7 +1024,7 @@ def bench_serving(dev, slo_ms, large):
         if lo is not None and hi is not None:
             for _ in range(4):
                 mid = (lo + hi) // 2
-                r = passes(ops_, mode, mid, points, inp)
+                r = passes(ops_, mode, mid, points, inp, target)
                 if r is None:
                     hi = mid
                 else:
@@ -956,14 +1037,25 @@ def bench_serving(dev, slo_ms, large):
                 ("batched_bf16_values_and_inputs", operands16, "batched", "bf16"))
     res = {}
     for name, ops_, mode, inp in variants:
-        best, points = sweep(ops_, mode, inp)
+        best, points = sweep(ops_, mode, inp, slo_ms, qps_grid)
         res[name] = {"best": best, "sweep": points,
                      "normwise_err_vs_fp64": serving_error(ops_, mode, inp)}
         print(f"# serving {name}: {best}", file=sys.stderr, flush=True)
     base = (res["plain_fp32"]["best"] or {}).get("req_per_s")
-    for name, *_ in variants[1:]:
+    # The comparison at equal p99: each batched path swept again with the
+    # plain path's measured p99 (at its best rate) as the target.
+    plain_p99 = (res["plain_fp32"]["best"] or {}).get("p99_ms")
+    for name, ops_, mode, inp in variants[1:]:
         b = (res[name]["best"] or {}).get("req_per_s")
         res[name]["x_plain"] = round(b / base, 2) if (b and base) else None
+        if plain_p99:
+            eq, eq_points = sweep(ops_, mode, inp, plain_p99, [q for q in qps_grid if q >= 50_000])
+            e = (eq or {}).get("req_per_s")
+            res[name]["at_plain_p99"] = {"p99_target_ms": plain_p99, "best": eq,
+                                         "x_plain": round(e / base, 2) if (e and base) else None,
+                                         "sweep": eq_points}
+            print(f"# serving {name} at the plain p99 ({plain_p99} ms): {eq}", file=sys.stderr,
+                  flush=True)
     res["p99_target_ms"] = slo_ms
     res["operands"] = ("256 batched-distribution CSR + 4 x 1M-row Zipf(1.5); FP32 values, BF16 "
                        "values, and BF16 values with BF16 dense inputs (FP32 accumulation and "
