@@ -292,10 +292,18 @@ struct WaveSortLds {
 // same as xor 8 there), all DPP register moves.  Distance 4 takes two row
 // rotations (row_ror:n hands lane i the value of lane (i - n) mod 16): lanes
 // with bit 2 clear want lane i + 4 (rotation 12), the others lane i - 4
-// (rotation 4).  16 and 32 go through the LDS crossbar.  d is a constant
-// once the network is unrolled.
+// (rotation 4).  The other distances (16, 31, 32, 63) go through the LDS
+// crossbar.  d is a constant once the network is unrolled.
 __device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int d) {
   switch (d) {
+    // Mirrors (lane ^ (2^m - 1)) of the sort's flip steps: within a quad,
+    // within 8 lanes (row_half_mirror) and within a 16-lane row (row_mirror).
+    case 3: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x1B, 0xF, 0xF, false));
+    case 7: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false));
+    case 15: return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xF, 0xF, false));
     case 4: {
       const int up = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x12C, 0xF, 0xF, false);
       const int dn = __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x124, 0xF, 0xF, false);
@@ -343,47 +351,98 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
   // 2. This lane's products (padding keys sort last).  Values stay in LDS
   // under their product index, which the key carries: the sort moves keys
   // only.
+  // The lane's products are consecutive: one search finds the first one's
+  // list, and each next product steps forward past the lists it has left
+  // (off[nl] = np > p stops the walk).  Positions are found first, then all
+  // of the lane's loads are issued, then used: the loads of one product do
+  // not wait for those of the one before.
+  int tl[PER];
+  int64_t ql[PER];
+  {
+    int t = PER * lane < np ? list_of(L.u.in.off, nl, PER * lane) : 0;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int p = PER * lane + r;
+      if (p < np && r > 0)
+        while (L.u.in.off[t + 1] <= p) t++;
+      tl[r] = t;
+      ql[r] = static_cast<int64_t>(L.u.in.bs[t]) + (p - L.u.in.off[t]);
+    }
+  }
+  TI bi[PER];
+  TV bv[PER], av[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    if (PER * lane + r < np) {
+      bi[r] = bidx[ql[r]];
+      bv[r] = bval[ql[r]];
+      av[r] = aval[a0 + tl[r]];
+    }
+  }
   uint32_t key[PER];
 #pragma unroll
   for (int r = 0; r < PER; r++) {
     const int p = PER * lane + r;
     key[r] = 0xffffffffu;
     if (p < np) {
-      const int t = list_of(L.u.in.off, nl, p);
-      const int64_t q = static_cast<int64_t>(L.u.in.bs[t]) + (p - L.u.in.off[t]);
-      const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bidx[q]) - base);
+      const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bi[r]) - base);
       key[r] = (col << PB) | static_cast<uint32_t>(p);
-      L.pv[p] = sr_mul<TV, SR>(aval[a0 + t], bval[q]);
+      L.pv[p] = sr_mul<TV, SR>(av[r], bv[r]);
     }
   }
-  // 3. Bitonic sort of the N keys (all distinct), ascending.
+  // 3. Bitonic sort of the N keys (all distinct), ascending, in the form
+  // whose compare-exchanges all put the minimum at the lower position: each
+  // merge stage of size k first pairs e with its mirror e ^ (k - 1) inside
+  // the k-block, then e with e ^ j for j = k/4 .. 1.  Pairs inside a lane
+  // are then a min and a max with no direction select; across lanes the
+  // lower lane of a pair keeps the minimum.
 #pragma unroll
   for (int k = 2; k <= N; k <<= 1) {
+    if (k <= PER) {
 #pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int r = 0; r < PER; r++) {
+        const int r2 = r ^ (k - 1);
+        if (r2 <= r) continue;
+        const uint32_t lo = key[r] < key[r2] ? key[r] : key[r2];
+        const uint32_t hi = key[r] < key[r2] ? key[r2] : key[r];
+        key[r] = lo;
+        key[r2] = hi;
+      }
+    } else {
+      // The mirror of element PER*lane + r is register PER-1-r of lane
+      // lane ^ (k/PER - 1).
+      const bool lower = (lane & (k / (2 * PER))) == 0;
+      uint32_t pk[PER];
+#pragma unroll
+      for (int r = 0; r < PER; r++) pk[r] = xor_lanes(key[PER - 1 - r], k / PER - 1);
+#pragma unroll
+      for (int r = 0; r < PER; r++) {
+        const uint32_t lo = pk[r] < key[r] ? pk[r] : key[r];
+        const uint32_t hi = pk[r] < key[r] ? key[r] : pk[r];
+        key[r] = lower ? lo : hi;
+      }
+    }
+#pragma unroll
+    for (int j = k >> 2; j > 0; j >>= 1) {
       if (j >= PER) {
         const int d = j / PER;
+        const bool lower = (lane & d) == 0;
 #pragma unroll
         for (int r = 0; r < PER; r++) {
-          const int e = PER * lane + r;
           const uint32_t pk = xor_lanes(key[r], d);
-          const bool keep_min = ((e & j) == 0) == ((e & k) == 0);
           const uint32_t lo = pk < key[r] ? pk : key[r];
           const uint32_t hi = pk < key[r] ? key[r] : pk;
-          key[r] = keep_min ? lo : hi;
+          key[r] = lower ? lo : hi;
         }
       } else {
 #pragma unroll
         for (int r = 0; r < PER; r++) {
           if (r & j) continue;
           const int r2 = r | j;
-          const int e = PER * lane + r;
-          const bool up = (e & k) == 0;
-          // Branch-free compare-exchange (ascending where up).
           const uint32_t lo = key[r] < key[r2] ? key[r] : key[r2];
           const uint32_t hi = key[r] < key[r2] ? key[r2] : key[r];
-          key[r] = up ? lo : hi;
-          key[r2] = up ? hi : lo;
+          key[r] = lo;
+          key[r2] = hi;
         }
       }
     }
@@ -404,8 +463,10 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
     const int e = r * kWave + lane;
     const uint32_t col = L.u.sk[e] >> PB;
     const bool head = e < np && (e == 0 || (L.u.sk[e - 1] >> PB) != col);
-    int tot;
-    const int ex = team_scan<64>(head ? 1 : 0, lane, nullptr, &tot);
+    // Heads before this lane in the round: a ballot and a masked popcount.
+    const uint64_t heads = ballot(head);
+    const int ex = popc(heads & ((uint64_t(1) << lane) - 1));
+    const int tot = popc(heads);
     if (head) {
       TV w = sr_zero<TV, SR>();
       for (int x = e; x < np && (L.u.sk[x] >> PB) == col; x++)
