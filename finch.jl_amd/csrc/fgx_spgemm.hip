@@ -48,17 +48,26 @@ __device__ __forceinline__ void team_sync() {
 }
 
 // Exclusive scan over a team; `tmp` holds NT/64 + 1 ints of LDS.
+// Inclusive sum over the wave's 64 lanes by DPP register moves (no LDS
+// round trips): shifts by 1, 2, 4 and 8 inside each 16-lane row (lanes
+// without a source add 0), then lane 15 of rows 0 and 2 into rows 1 and 3,
+// then lane 31 into rows 2 and 3.
+__device__ __forceinline__ int wave_inclusive_sum(int x) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);   // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);   // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);   // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);   // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+  return x;
+}
+
 template <int NT>
 __device__ __forceinline__ int team_scan(int v, int tt, int* tmp, int* total) {
   const int lane = tt & 63;
-  int x = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    int y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
-  }
+  const int x = wave_inclusive_sum(v);
   if constexpr (NT == 64) {
-    *total = __shfl(x, 63, 64);
+    *total = __builtin_amdgcn_readlane(x, 63);
     return x - v;
   } else {
     const int w = tt >> 6;
