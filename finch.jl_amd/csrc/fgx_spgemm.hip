@@ -239,30 +239,37 @@ __device__ void team_row(int64_t row, int tt, int64_t base, const TP* __restrict
 // Wave teams: 4 rows per 256-thread block.
 template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
 __global__ __launch_bounds__(256) void spgemm_wave_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
-    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
-    const TI* __restrict__ bidx, const TV* __restrict__ bval, int64_t* __restrict__ cnt,
-    const TP* __restrict__ cptr, TI* __restrict__ cidx, TV* __restrict__ cval) {
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
+    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
+    int64_t* __restrict__ cnt, const TP* __restrict__ cptr, TI* __restrict__ cidx,
+    TV* __restrict__ cval) {
   __shared__ TeamLds<kWaveCap, TV> lds[4];
-  const int w = threadIdx.x >> 6;
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
-  if (r >= nrows) return;
-  team_row<64, kWaveCap, NUMERIC, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr,
-                                      bidx, bval, cnt, cptr, cidx, cval, lds[w]);
+  const int w = wave_in_block();
+  const int64_t nrows = *nrows_p;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w; r < nrows;
+       r += static_cast<int64_t>(gridDim.x) * 4) {
+    team_row<64, kWaveCap, NUMERIC, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr,
+                                        bidx, bval, cnt, cptr, cidx, cval, lds[w]);
+    team_sync<64>();  // the wave's LDS is reused by its next row
+  }
 }
 
 template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
 __global__ __launch_bounds__(256) void spgemm_block_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
-    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
-    const TI* __restrict__ bidx, const TV* __restrict__ bval, int64_t* __restrict__ cnt,
-    const TP* __restrict__ cptr, TI* __restrict__ cidx, TV* __restrict__ cval) {
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
+    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
+    int64_t* __restrict__ cnt, const TP* __restrict__ cptr, TI* __restrict__ cidx,
+    TV* __restrict__ cval) {
   extern __shared__ __align__(16) unsigned char dyn[];
   TeamLds<kBlockCap, TV>& L = *reinterpret_cast<TeamLds<kBlockCap, TV>*>(dyn);
-  const int64_t r = blockIdx.x;
-  if (r >= nrows) return;
-  team_row<256, kBlockCap, NUMERIC, SR>(rows[r], threadIdx.x, base, aptr, aidx, aval, bptr, bidx,
-                                        bval, cnt, cptr, cidx, cval, L);
+  const int64_t nrows = *nrows_p;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    team_row<256, kBlockCap, NUMERIC, SR>(rows[r], threadIdx.x, base, aptr, aidx, aval, bptr,
+                                          bidx, bval, cnt, cptr, cidx, cval, L);
+    __syncthreads();  // the block's LDS is reused by its next row
+  }
 }
 
 // ---- wave rows, numeric: register sort ------------------------------------------------
@@ -490,16 +497,19 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
 // Four wave rows per 256-thread block.
 template <int PER, int SR, typename TP, typename TI, typename TV>
 __global__ __launch_bounds__(256) void spgemm_wave_sort_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
-    const TI* __restrict__ aidx, const TV* __restrict__ aval, const TP* __restrict__ bptr,
-    const TI* __restrict__ bidx, const TV* __restrict__ bval, const TP* __restrict__ cptr,
-    TI* __restrict__ cidx, TV* __restrict__ cval) {
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
+    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
+    const TP* __restrict__ cptr, TI* __restrict__ cidx, TV* __restrict__ cval) {
   __shared__ WaveSortLds<PER, TV, TP> lds[4];
   const int w = wave_in_block();
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
-  if (r >= nrows) return;
-  sort_row<PER, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr, bidx, bval, cptr,
-                    cidx, cval, lds[w]);
+  const int64_t nrows = *nrows_p;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w; r < nrows;
+       r += static_cast<int64_t>(gridDim.x) * 4) {
+    sort_row<PER, SR>(rows[r], threadIdx.x & 63, base, aptr, aidx, aval, bptr, bidx, bval, cptr,
+                      cidx, cval, lds[w]);
+    team_sync<64>();  // the wave's LDS is reused by its next row
+  }
 }
 
 // ---- hash teams (symbolic pass) ---------------------------------------------------------
@@ -577,28 +587,33 @@ __device__ void hash_count_row(int64_t row, int tt, int64_t base, const TP* __re
 // 256-thread block for the rows of 257..512 products).
 template <int CAP, int SLOTS, typename TP, typename TI>
 __global__ __launch_bounds__(256) void spgemm_wave_count_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
-    const TI* __restrict__ aidx, const TP* __restrict__ bptr, const TI* __restrict__ bidx,
-    int64_t* __restrict__ cnt) {
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TP* __restrict__ bptr,
+    const TI* __restrict__ bidx, int64_t* __restrict__ cnt) {
   static_assert(SLOTS >= 2 * CAP, "hash table at most half full");
   __shared__ HashLds<CAP, SLOTS, TP> lds[4];
   const int w = wave_in_block();
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w;
-  if (r >= nrows) return;
-  hash_count_row<64, CAP, SLOTS>(rows[r], threadIdx.x & 63, base, aptr, aidx, bptr, bidx, cnt,
-                                 lds[w]);
+  const int64_t nrows = *nrows_p;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + w; r < nrows;
+       r += static_cast<int64_t>(gridDim.x) * 4) {
+    hash_count_row<64, CAP, SLOTS>(rows[r], threadIdx.x & 63, base, aptr, aidx, bptr, bidx, cnt,
+                                   lds[w]);
+    team_sync<64>();  // the wave's LDS is reused by its next row
+  }
 }
 
 template <typename TP, typename TI>
 __global__ __launch_bounds__(256) void spgemm_block_count_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, const TP* __restrict__ aptr,
-    const TI* __restrict__ aidx, const TP* __restrict__ bptr, const TI* __restrict__ bidx,
-    int64_t* __restrict__ cnt) {
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TP* __restrict__ bptr,
+    const TI* __restrict__ bidx, int64_t* __restrict__ cnt) {
   __shared__ HashLds<kBlockCap, kBlockSlots, TP> L;
-  const int64_t r = blockIdx.x;
-  if (r >= nrows) return;
-  hash_count_row<256, kBlockCap, kBlockSlots>(rows[r], threadIdx.x, base, aptr, aidx, bptr, bidx,
-                                              cnt, L);
+  const int64_t nrows = *nrows_p;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    hash_count_row<256, kBlockCap, kBlockSlots>(rows[r], threadIdx.x, base, aptr, aidx, bptr,
+                                                bidx, cnt, L);
+    __syncthreads();  // the block's LDS is reused by its next row
+  }
 }
 
 // ---- dense-window fallback ---------------------------------------------------------
@@ -642,19 +657,16 @@ __device__ __forceinline__ void add_rounded_product(TV* acc, TV a, TV b) {
   *acc = prod + *acc;
 }
 
+// One row of the dense-window fallback, by the whole block (the LDS window
+// is the block's; every path through a window ends at a barrier, so the
+// next row may clear it).
 template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
-__global__ __launch_bounds__(256) void spgemm_window_kernel(
-    const int64_t* __restrict__ rows, int64_t nrows, int64_t base, int64_t n_inner,
-    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
-    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
-    int64_t* __restrict__ cnt, const TP* __restrict__ cptr, TI* __restrict__ cidx,
-    TV* __restrict__ cval) {
-  __shared__ TV acc[kWindow];
-  __shared__ int flag[kWindow];
-  __shared__ int tmp[8];
-  const int64_t rr = blockIdx.x;
-  if (rr >= nrows) return;
-  const int64_t row = rows[rr];
+__device__ void window_row(int64_t row, int64_t base, int64_t n_inner, const TP* __restrict__ aptr,
+                           const TI* __restrict__ aidx, const TV* __restrict__ aval,
+                           const TP* __restrict__ bptr, const TI* __restrict__ bidx,
+                           const TV* __restrict__ bval, int64_t* __restrict__ cnt,
+                           const TP* __restrict__ cptr, TI* __restrict__ cidx,
+                           TV* __restrict__ cval, TV* acc, int* flag, int* tmp) {
   const int tt = threadIdx.x;
   const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
   int64_t seen = 0;
@@ -730,6 +742,23 @@ __global__ __launch_bounds__(256) void spgemm_window_kernel(
   }
 }
 
+template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
+__global__ __launch_bounds__(256) void spgemm_window_kernel(
+    const int64_t* __restrict__ rows, const int64_t* __restrict__ nrows_p, int64_t base,
+    int64_t n_inner,
+    const TP* __restrict__ aptr, const TI* __restrict__ aidx, const TV* __restrict__ aval,
+    const TP* __restrict__ bptr, const TI* __restrict__ bidx, const TV* __restrict__ bval,
+    int64_t* __restrict__ cnt, const TP* __restrict__ cptr, TI* __restrict__ cidx,
+    TV* __restrict__ cval) {
+  __shared__ TV acc[kWindow];
+  __shared__ int flag[kWindow];
+  __shared__ int tmp[8];
+  const int64_t nrows = *nrows_p;
+  for (int64_t rr = blockIdx.x; rr < nrows; rr += gridDim.x)
+    window_row<NUMERIC, SR>(rows[rr], base, n_inner, aptr, aidx, aval, bptr, bidx, bval, cnt, cptr,
+                            cidx, cval, acc, flag, tmp);
+}
+
 // Row bins by products: 0 up to 256 (wave), 1 up to 512 (wide wave), 2 up
 // to 1536 (block), 3 beyond (dense window).  Slots are taken with one
 // atomic per bin and 1024-row block: the waves' counts (ballots) are
@@ -788,16 +817,30 @@ __global__ __launch_bounds__(kBinThreads) void spgemm_bin_kernel(int64_t m, int6
     bin_rows[bin * m + wbase[w][bin] + __popcll(mask[bin] & below)] = row;
 }
 
+// The bins stay on the device: count[b] is read by the row kernels, which
+// loop over their bin's rows, so no pass waits on the host for the sizes.
 struct Bins {
-  int64_t* rows;  // 4 * m
-  int64_t count[4];
+  int64_t* rows;          // 4 * m
+  const int64_t* count;   // 4, on the device
 };
 
+// Grids of the row kernels: enough blocks to fill the device several times
+// over, at most one row (block kernels) or four rows (wave kernels) each.
+constexpr int64_t kWaveRowBlocks = 8192;
+constexpr int64_t kBlockRowBlocks = 2048;
+inline unsigned wave_row_grid(int64_t m) {
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((m + 3) / 4, kWaveRowBlocks)));
+}
+inline unsigned block_row_grid(int64_t m) {
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(m, kBlockRowBlocks)));
+}
+
 template <typename TP, typename TI>
-static int make_bins(fgx_ctx* ctx, const fgx_compressed* A, const fgx_compressed* B, int64_t* dev,
-                     Bins* bins, hipStream_t s) {
+static int make_bins(const fgx_compressed* A, const fgx_compressed* B, int64_t* dev, Bins* bins,
+                     hipStream_t s) {
   const int64_t m = A->n_outer;
   int64_t* bin_count = dev;
+  bins->count = bin_count;
   bins->rows = dev + 4;
   FGX_HIP(hipMemsetAsync(bin_count, 0, 4 * sizeof(int64_t), s));
   if (m > 0) {
@@ -808,9 +851,6 @@ static int make_bins(fgx_ctx* ctx, const fgx_compressed* A, const fgx_compressed
                        static_cast<const TP*>(B->ptr), bin_count, bins->rows);
     FGX_TRY(launch_check("spgemm_bin_kernel"));
   }
-  FGX_HIP(hipMemcpyAsync(ctx->host_i64, bin_count, 4 * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  FGX_HIP(hipStreamSynchronize(s));
-  for (int b = 0; b < 4; b++) bins->count[b] = ctx->host_i64[b];
   return FGX_OK;
 }
 
@@ -818,6 +858,7 @@ template <bool NUMERIC, int SR, typename TP, typename TI, typename TV>
 static int run_bins(const fgx_compressed* A, const fgx_compressed* B, const Bins& bins,
                     int64_t* cnt, const void* cptr, void* cidx, void* cval, hipStream_t s) {
   const int64_t m = A->n_outer;
+  if (m == 0) return FGX_OK;
   const int64_t base = A->index_base;
   auto aptr = static_cast<const TP*>(A->ptr);
   auto aidx = static_cast<const TI*>(A->idx);
@@ -828,68 +869,56 @@ static int run_bins(const fgx_compressed* A, const fgx_compressed* B, const Bins
   auto cp = static_cast<const TP*>(cptr);
   auto ci = static_cast<TI*>(cidx);
   auto cv = static_cast<TV*>(cval);
+  const unsigned wg = wave_row_grid(m), bg = block_row_grid(m);
   if constexpr (!NUMERIC) {
     // Symbolic: distinct-column counts from the LDS hash teams.
-    if (bins.count[0] > 0) {
-      hipLaunchKernelGGL((spgemm_wave_count_kernel<kWaveCap, kWaveSlots, TP, TI>),
-                         dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
-                         bins.rows, bins.count[0], base, aptr, aidx, bptr, bidx, cnt);
-      FGX_TRY(launch_check("spgemm_wave_count_kernel"));
-    }
-    if (bins.count[1] > 0) {
-      hipLaunchKernelGGL((spgemm_wave_count_kernel<kWide, 2 * kWide, TP, TI>),
-                         dim3(static_cast<unsigned>((bins.count[1] + 3) / 4)), dim3(256), 0, s,
-                         bins.rows + m, bins.count[1], base, aptr, aidx, bptr, bidx, cnt);
-      FGX_TRY(launch_check("spgemm_wave_count_kernel"));
-    }
-    if (bins.count[2] > 0) {
-      hipLaunchKernelGGL((spgemm_block_count_kernel<TP, TI>),
-                         dim3(static_cast<unsigned>(bins.count[2])), dim3(256), 0, s,
-                         bins.rows + 2 * m, bins.count[2], base, aptr, aidx, bptr, bidx, cnt);
-      FGX_TRY(launch_check("spgemm_block_count_kernel"));
-    }
+    hipLaunchKernelGGL((spgemm_wave_count_kernel<kWaveCap, kWaveSlots, TP, TI>), dim3(wg),
+                       dim3(256), 0, s, bins.rows, bins.count, base, aptr, aidx, bptr, bidx, cnt);
+    FGX_TRY(launch_check("spgemm_wave_count_kernel"));
+    hipLaunchKernelGGL((spgemm_wave_count_kernel<kWide, 2 * kWide, TP, TI>), dim3(wg), dim3(256), 0,
+                       s, bins.rows + m, bins.count + 1, base, aptr, aidx, bptr, bidx, cnt);
+    FGX_TRY(launch_check("spgemm_wave_count_kernel"));
+    hipLaunchKernelGGL((spgemm_block_count_kernel<TP, TI>), dim3(bg), dim3(256), 0, s,
+                       bins.rows + 2 * m, bins.count + 2, base, aptr, aidx, bptr, bidx, cnt);
+    FGX_TRY(launch_check("spgemm_block_count_kernel"));
   } else {
     // Numeric: rows of up to 256 / 512 products by the register sort (when
     // the columns fit its key), the rest by the ordered merge.
     const bool sort4 = B->n_inner <= SortShape<4>::kMaxCols;
     const bool sort8 = B->n_inner <= SortShape<8>::kMaxCols;
-    if (bins.count[0] > 0 && sort4) {
-      hipLaunchKernelGGL((spgemm_wave_sort_kernel<4, SR, TP, TI, TV>),
-                         dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
-                         bins.rows, bins.count[0], base, aptr, aidx, aval, bptr, bidx, bval, cp,
-                         ci, cv);
+    if (sort4) {
+      hipLaunchKernelGGL((spgemm_wave_sort_kernel<4, SR, TP, TI, TV>), dim3(wg), dim3(256), 0, s,
+                         bins.rows, bins.count, base, aptr, aidx, aval, bptr, bidx, bval, cp, ci,
+                         cv);
       FGX_TRY(launch_check("spgemm_wave_sort_kernel"));
-    } else if (bins.count[0] > 0) {
-      hipLaunchKernelGGL((spgemm_wave_kernel<NUMERIC, SR, TP, TI, TV>),
-                         dim3(static_cast<unsigned>((bins.count[0] + 3) / 4)), dim3(256), 0, s,
-                         bins.rows, bins.count[0], base, aptr, aidx, aval, bptr, bidx, bval, cnt,
-                         cp, ci, cv);
+    } else {
+      hipLaunchKernelGGL((spgemm_wave_kernel<NUMERIC, SR, TP, TI, TV>), dim3(wg), dim3(256), 0, s,
+                         bins.rows, bins.count, base, aptr, aidx, aval, bptr, bidx, bval, cnt, cp,
+                         ci, cv);
       FGX_TRY(launch_check("spgemm_wave_kernel"));
     }
-    if (bins.count[1] > 0 && sort8) {
-      hipLaunchKernelGGL((spgemm_wave_sort_kernel<8, SR, TP, TI, TV>),
-                         dim3(static_cast<unsigned>((bins.count[1] + 3) / 4)), dim3(256), 0, s,
-                         bins.rows + m, bins.count[1], base, aptr, aidx, aval, bptr, bidx, bval,
+    if (sort8) {
+      hipLaunchKernelGGL((spgemm_wave_sort_kernel<8, SR, TP, TI, TV>), dim3(wg), dim3(256), 0, s,
+                         bins.rows + m, bins.count + 1, base, aptr, aidx, aval, bptr, bidx, bval,
                          cp, ci, cv);
       FGX_TRY(launch_check("spgemm_wave_sort_kernel"));
     }
     const size_t lds = sizeof(TeamLds<kBlockCap, TV>);
     for (int b = sort8 ? 2 : 1; b <= 2; b++) {
-      if (bins.count[b] == 0) continue;
-      hipLaunchKernelGGL((spgemm_block_kernel<NUMERIC, SR, TP, TI, TV>),
-                         dim3(static_cast<unsigned>(bins.count[b])), dim3(256), lds, s,
-                         bins.rows + b * m, bins.count[b], base, aptr, aidx, aval, bptr, bidx,
+      hipLaunchKernelGGL((spgemm_block_kernel<NUMERIC, SR, TP, TI, TV>), dim3(bg), dim3(256), lds,
+                         s, bins.rows + b * m, bins.count + b, base, aptr, aidx, aval, bptr, bidx,
                          bval, cnt, cp, ci, cv);
       FGX_TRY(launch_check("spgemm_block_kernel"));
     }
   }
-  if (bins.count[3] > 0) {
-    hipLaunchKernelGGL((spgemm_window_kernel<NUMERIC, SR, TP, TI, TV>),
-                       dim3(static_cast<unsigned>(bins.count[3])), dim3(256), 0, s,
-                       bins.rows + 3 * m, bins.count[3], base, B->n_inner, aptr, aidx, aval, bptr,
-                       bidx, bval, cnt, cp, ci, cv);
-    FGX_TRY(launch_check("spgemm_window_kernel"));
-  }
+  // The dense-window rows are rare: a smaller grid keeps the launch cheap
+  // when there are none.
+  const unsigned window_grid = std::min<unsigned>(bg, 1024);
+  hipLaunchKernelGGL((spgemm_window_kernel<NUMERIC, SR, TP, TI, TV>), dim3(window_grid), dim3(256),
+                     0, s,
+                     bins.rows + 3 * m, bins.count + 3, base, B->n_inner, aptr, aidx, aval, bptr,
+                     bidx, bval, cnt, cp, ci, cv);
+  FGX_TRY(launch_check("spgemm_window_kernel"));
   return FGX_OK;
 }
 
@@ -946,7 +975,7 @@ int fgx_spgemm_symbolic(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
   int64_t total = 0;
   FGX_DISPATCH_IDX(ic, {
     Bins bins;
-    FGX_TRY((make_bins<TP, TI>(ctx, A, B, binmem, &bins, s)));
+    FGX_TRY((make_bins<TP, TI>(A, B, binmem, &bins, s)));
     // Structure does not depend on values: count with the FP32 instantiation.
     FGX_TRY((run_bins<false, FGX_PLUS_TIMES, TP, TI, float>(A, B, bins, cnt, nullptr, nullptr,
                                                             nullptr, s)));
@@ -978,7 +1007,7 @@ int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
 #define FGX_GEMM_NUM(SR_, TV_)                                                            \
   FGX_DISPATCH_IDX(ic, {                                                                  \
     Bins bins;                                                                            \
-    FGX_TRY((make_bins<TP, TI>(ctx, A, B, binmem, &bins, s)));                            \
+    FGX_TRY((make_bins<TP, TI>(A, B, binmem, &bins, s)));                                 \
     FGX_TRY((run_bins<true, SR_, TP, TI, TV_>(A, B, bins, nullptr, c_ptr, c_idx, c_val, s))); \
   })
   if (semiring == FGX_PLUS_TIMES) {
