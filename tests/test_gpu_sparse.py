@@ -271,6 +271,31 @@ def test_spgemm_fp_plus_times_all_bins(dev):
     _gemm_check(dev, "plus_times", ap, ai, av, m, l, bp, bi, bv, n)
 
 
+@pytest.mark.parametrize("n", [(1 << 22) + 3, (1 << 23) + 7])
+def test_spgemm_wide_columns_merge_paths(dev, n):
+    # Column counts past the register sorts' keys (31 bits hold a column and
+    # an 8- or 9-bit product index): above 2^22 the 257..512-product rows go
+    # to the block merge, above 2^23 the short rows go to the wave merge.  FP
+    # plus-times folds stay in the reference's order there too.
+    rng = np.random.default_rng(40 + n % 11)
+    l = 2000
+    lens_a = np.concatenate([rng.integers(1, 30, 200), rng.integers(60, 80, 30),
+                             rng.integers(150, 200, 4), [0, 0]])
+    m = lens_a.size
+    ap = np.concatenate([[0], np.cumsum(lens_a)]) + 1
+    ai = np.concatenate([np.sort(rng.choice(l, k, replace=False)) for k in lens_a if k]) + 1
+    prods = None
+    for dtype, semiring in ((np.float32, "plus_times"), (np.float32, "min_plus")):
+        bp, bi, bv = synth.poisson_csr(l, n, 6.0, rng=rng, dtype=dtype, values="randn")
+        if prods is None:
+            blen = np.diff(bp)
+            prods = np.array([blen[ai[ap[r] - 1: ap[r + 1] - 1] - 1].sum() for r in range(m)])
+            assert (prods <= 256).any() and ((prods > 256) & (prods <= 512)).any()
+            assert ((prods > 512) & (prods <= 1536)).any() and prods.max() <= 1536
+        av = rng.standard_normal(ai.size).astype(dtype)
+        _gemm_check(dev, semiring, ap, ai, av, m, l, bp, bi, bv, n)
+
+
 def test_spgemm_empty(dev):
     ep = np.ones(6, np.int64)
     e = np.zeros(0, np.int64)
