@@ -335,85 +335,18 @@ __device__ __forceinline__ uint32_t xor_lanes(uint32_t v, int d) {
   }
 }
 
-template <int PER, int SR, typename TP, typename TI, typename TV>
-__device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restrict__ aptr,
-                         const TI* __restrict__ aidx, const TV* __restrict__ aval,
-                         const TP* __restrict__ bptr, const TI* __restrict__ bidx,
-                         const TV* __restrict__ bval, const TP* __restrict__ cptr,
-                         TI* __restrict__ cidx, TV* __restrict__ cval, WaveSortLds<PER, TV, TP>& L) {
-  constexpr int N = SortShape<PER>::kCap;
-  constexpr int PB = SortShape<PER>::kPBits;
-  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
-  const int nl = static_cast<int>(a1 - a0);
-  // 1. List offsets.
-  int run = 0;
-  for (int c = 0; c < nl; c += kWave) {
-    const int t = c + lane;
-    int len = 0;
-    if (t < nl) {
-      const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
-      const int64_t s0 = pos_at(bptr, base, k);
-      len = static_cast<int>(pos_at(bptr, base, k + 1) - s0);
-      L.u.in.bs[t] = static_cast<TP>(s0);
-    }
-    int tot;
-    const int ex = team_scan<64>(len, lane, nullptr, &tot);
-    if (t < nl) L.u.in.off[t] = run + ex;
-    run += tot;
-  }
-  const int np = run;
-  if (lane == 0) L.u.in.off[nl] = np;
-  team_sync<64>();
-  // 2. This lane's products (padding keys sort last).  Values stay in LDS
-  // under their product index, which the key carries: the sort moves keys
-  // only.
-  // The lane's products are consecutive: one search finds the first one's
-  // list, and each next product steps forward past the lists it has left
-  // (off[nl] = np > p stops the walk).  Positions are found first, then all
-  // of the lane's loads are issued, then used: the loads of one product do
-  // not wait for those of the one before.
-  int tl[PER];
-  int64_t ql[PER];
-  {
-    int t = PER * lane < np ? list_of(L.u.in.off, nl, PER * lane) : 0;
+// Stages K0, 2*K0, .., K1 of the bitonic sort of the 64*PER keys (element
+// e = PER*lane + r in register r of lane `lane`), in the form whose
+// compare-exchanges all put the minimum at the lower position: each merge
+// stage of size k first pairs e with its mirror e ^ (k - 1) inside the
+// k-block, then e with e ^ j for j = k/4 .. 1.  Pairs inside a lane are then
+// a min and a max with no direction select; across lanes the lower lane of a
+// pair keeps the minimum.  Stage k leaves every k-block sorted when every
+// k/2-block was.
+template <int PER, int K0, int K1>
+__device__ __forceinline__ void bitonic_stages(uint32_t (&key)[PER], int lane) {
 #pragma unroll
-    for (int r = 0; r < PER; r++) {
-      const int p = PER * lane + r;
-      if (p < np && r > 0)
-        while (L.u.in.off[t + 1] <= p) t++;
-      tl[r] = t;
-      ql[r] = static_cast<int64_t>(L.u.in.bs[t]) + (p - L.u.in.off[t]);
-    }
-  }
-  TI bi[PER];
-  TV bv[PER], av[PER];
-#pragma unroll
-  for (int r = 0; r < PER; r++) {
-    if (PER * lane + r < np) {
-      bi[r] = bidx[ql[r]];
-      bv[r] = bval[ql[r]];
-      av[r] = aval[a0 + tl[r]];
-    }
-  }
-  uint32_t key[PER];
-#pragma unroll
-  for (int r = 0; r < PER; r++) {
-    const int p = PER * lane + r;
-    key[r] = 0xffffffffu;
-    if (p < np) {
-      const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bi[r]) - base);
-      key[r] = (col << PB) | static_cast<uint32_t>(p);
-      L.pv[p] = sr_mul<TV, SR>(av[r], bv[r]);
-    }
-  }
-  // 3. Bitonic sort of the N keys (all distinct), ascending, in the form
-  // whose compare-exchanges all put the minimum at the lower position: each
-  // merge stage of size k first pairs e with its mirror e ^ (k - 1) inside
-  // the k-block, then e with e ^ j for j = k/4 .. 1.  Pairs inside a lane
-  // are then a min and a max with no direction select; across lanes the
-  // lower lane of a pair keeps the minimum.
-#pragma unroll
-  for (int k = 2; k <= N; k <<= 1) {
+  for (int k = K0; k <= K1; k <<= 1) {
     if (k <= PER) {
 #pragma unroll
       for (int r = 0; r < PER; r++) {
@@ -463,6 +396,104 @@ __device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restri
       }
     }
   }
+}
+
+template <int PER, int SR, typename TP, typename TI, typename TV>
+__device__ void sort_row(int64_t row, int lane, int64_t base, const TP* __restrict__ aptr,
+                         const TI* __restrict__ aidx, const TV* __restrict__ aval,
+                         const TP* __restrict__ bptr, const TI* __restrict__ bidx,
+                         const TV* __restrict__ bval, const TP* __restrict__ cptr,
+                         TI* __restrict__ cidx, TV* __restrict__ cval, WaveSortLds<PER, TV, TP>& L) {
+  constexpr int N = SortShape<PER>::kCap;
+  constexpr int PB = SortShape<PER>::kPBits;
+  const int64_t a0 = pos_at(aptr, base, row), a1 = pos_at(aptr, base, row + 1);
+  const int nl = static_cast<int>(a1 - a0);
+  // 1. List offsets.
+  constexpr int kRun = 32;
+  int run = 0;
+  bool long_list = false;
+  for (int c = 0; c < nl; c += kWave) {
+    const int t = c + lane;
+    int len = 0;
+    if (t < nl) {
+      const int k = static_cast<int>(static_cast<int64_t>(aidx[a0 + t]) - base);
+      const int64_t s0 = pos_at(bptr, base, k);
+      len = static_cast<int>(pos_at(bptr, base, k + 1) - s0);
+      L.u.in.bs[t] = static_cast<TP>(s0);
+    }
+    long_list = long_list || ballot(len > kRun) != 0;
+    int tot;
+    const int ex = team_scan<64>(len, lane, nullptr, &tot);
+    if (t < nl) L.u.in.off[t] = run + ex;
+    run += tot;
+  }
+  const int np = run;
+  if (lane == 0) L.u.in.off[nl] = np;
+  team_sync<64>();
+  // Each B row arrives sorted by column (and its products by index), so when
+  // every list fits a 32-key block and the blocks fit the network, list t is
+  // laid out from element 32*t, padded with keys that sort last.
+  // (Only the 256-key sort: the 512-key one measured slower with the
+  // second layout, 88 -> 96 us on rows of 257..512 products.)
+  const bool blocked = PER == 4 && nl <= N / kRun && !long_list;
+  // 2. This lane's products (padding keys sort last).  Values stay in LDS
+  // under their product index, which the key carries: the sort moves keys
+  // only.
+  // The lane's products are consecutive: one search finds the first one's
+  // list, and each next product steps forward past the lists it has left
+  // (off[nl] = np > p stops the walk).  Positions are found first, then all
+  // of the lane's loads are issued, then used: the loads of one product do
+  // not wait for those of the one before.
+  int tl[PER], pl[PER];
+  int64_t ql[PER];
+  if (PER == 4 && blocked) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int e = PER * lane + r;
+      const int t = e / kRun, j = e % kRun;
+      const int o = t < nl ? L.u.in.off[t] : 0;
+      const bool in = t < nl && o + j < L.u.in.off[t + 1];
+      tl[r] = t;
+      pl[r] = in ? o + j : np;
+      ql[r] = in ? static_cast<int64_t>(L.u.in.bs[t]) + j : 0;
+    }
+  } else {
+    int t = PER * lane < np ? list_of(L.u.in.off, nl, PER * lane) : 0;
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int p = PER * lane + r;
+      if (p < np && r > 0)
+        while (L.u.in.off[t + 1] <= p) t++;
+      tl[r] = t;
+      pl[r] = p;
+      ql[r] = static_cast<int64_t>(L.u.in.bs[t]) + (p - L.u.in.off[t]);
+    }
+  }
+  TI bi[PER];
+  TV bv[PER], av[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    if (pl[r] < np) {
+      bi[r] = bidx[ql[r]];
+      bv[r] = bval[ql[r]];
+      av[r] = aval[a0 + tl[r]];
+    }
+  }
+  uint32_t key[PER];
+#pragma unroll
+  for (int r = 0; r < PER; r++) {
+    const int p = pl[r];
+    key[r] = 0xffffffffu;
+    if (p < np) {
+      const uint32_t col = static_cast<uint32_t>(static_cast<int64_t>(bi[r]) - base);
+      key[r] = (col << PB) | static_cast<uint32_t>(p);
+      L.pv[p] = sr_mul<TV, SR>(av[r], bv[r]);
+    }
+  }
+  // 3. Bitonic sort of the N keys, ascending.  Laid out in sorted 32-key
+  // blocks, the keys skip the stages that sort those blocks.
+  if (PER != 4 || !blocked) bitonic_stages<PER, 2, kRun>(key, lane);
+  bitonic_stages<PER, 2 * kRun, N>(key, lane);
   // 4. Fold each column's run in order and write it at its rank.  The
   // sorted keys overwrite the product lists: every lane's reads of those
   // are complete (their values fed the sort) before the wave's stores.
