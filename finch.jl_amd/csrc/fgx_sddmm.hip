@@ -6,6 +6,12 @@
 // first transposed to k-fast rows in the context's dense arena.
 //
 // General path (FP64, other widths): VALU dot products, one lane per position.
+//
+// Few positions, wide rows (at most kSplitMaxNnz positions, k >= kSplitMinK):
+// a wave per position, its lanes striding over k in either layout, then a
+// butterfly sum, so a handful of long dot products still spreads over the
+// device (Finch's documented example, 100 positions of width 1000, takes one
+// wave each instead of two waves in all).
 #include "fgx_launch.hpp"
 #include "fgx_sddmm_core.hpp"
 #include "fgx_internal.hpp"
@@ -53,6 +59,28 @@ __global__ __launch_bounds__(256) void sddmm_valu_kernel(
   out[q] = widen<TA>(val[q]) * acc;
 }
 
+constexpr int64_t kSplitMaxNnz = 4096;
+constexpr int64_t kSplitMinK = 256;
+
+template <typename TP, typename TI, typename TV, typename TA>
+__global__ __launch_bounds__(256) void sddmm_ksplit_kernel(
+    const TP* __restrict__ ptr, const TI* __restrict__ idx, const TV* __restrict__ val,
+    int64_t base, int64_t m, int64_t nnz, int64_t k, const TA* __restrict__ X, int64_t x_rs,
+    int64_t x_cs, const TA* __restrict__ Y, int64_t y_rs, int64_t y_cs, TA* __restrict__ out) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * kWavesPerBlock + wave_in_block();
+  if (q >= nnz) return;  // wave-uniform
+  const int lane = lane_id();
+  const int64_t i = group_find_fiber<kWave>(ptr, base, 0, m, q);
+  const int64_t j = static_cast<int64_t>(idx[q]) - base;
+  const TA* x = X + i * x_rs;
+  const TA* y = Y + j * y_rs;
+  TA acc = 0;
+  for (int64_t c = lane; c < k; c += kWave) acc += x[c * x_cs] * y[c * y_cs];
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+  if (lane == 0) out[q] = widen<TA>(val[q]) * acc;
+}
+
 template <typename TP, typename TI, typename TV>
 static int sddmm_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const void* Xv,
                        int64_t ldx, int x_layout, const void* Yv, int64_t ldy, int y_layout,
@@ -63,6 +91,19 @@ static int sddmm_typed(fgx_ctx* ctx, const fgx_compressed* A, int64_t k, const v
   const TA* X = static_cast<const TA*>(Xv);
   const TA* Y = static_cast<const TA*>(Yv);
   TA* out = static_cast<TA*>(outv);
+  if (nnz <= kSplitMaxNnz && k >= kSplitMinK) {
+    const int64_t x_rs = x_layout == FGX_ROW_MAJOR ? ldx : 1;
+    const int64_t x_cs = x_layout == FGX_ROW_MAJOR ? 1 : ldx;
+    const int64_t y_rs = y_layout == FGX_ROW_MAJOR ? ldy : 1;
+    const int64_t y_cs = y_layout == FGX_ROW_MAJOR ? 1 : ldy;
+    const int64_t blocks = (nnz + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((sddmm_ksplit_kernel<TP, TI, TV, TA>), dim3(static_cast<unsigned>(blocks)),
+                       dim3(256), 0, s, static_cast<const TP*>(A->ptr),
+                       static_cast<const TI*>(A->idx), static_cast<const TV*>(A->val),
+                       static_cast<int64_t>(A->index_base), m, nnz, k, X, x_rs, x_cs, Y, y_rs,
+                       y_cs, out);
+    return launch_check("sddmm_ksplit_kernel");
+  }
   bool mfma = false;
   if constexpr (sizeof(TA) == 4) {
     if ((x_layout != FGX_ROW_MAJOR || y_layout != FGX_ROW_MAJOR) && k % 4 == 0) {

@@ -238,6 +238,35 @@ def test_sddmm_fp64(dev):
     assert rel_err(out, ref) <= FP64_REL
 
 
+@pytest.mark.parametrize("k", [256, 300, 1000])
+@pytest.mark.parametrize("layout", ["row", "col"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_sddmm_few_positions_wide_rows(dev, k, layout, dtype):
+    # Finch's documented example shape (getting_started.md:126-134): 100
+    # stored positions of a 1000 x 1000 pattern, dense factors of width k, in
+    # Finch's column-major Dense(Dense) layout or k-fast rows.  Few positions
+    # and wide rows take a wave per position.
+    rng = np.random.default_rng(k)
+    m = n = 1000
+    flat = np.sort(rng.choice(m * n, 100, replace=False))
+    rows, cols = flat // n, flat % n
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))]) + 1
+    idx = cols + 1
+    val = rng.random(100).astype(dtype)
+    X = rng.random((m, k)).astype(dtype)
+    Y = rng.random((n, k)).astype(dtype)
+    A = operand(ptr, idx, val, m, n, dev, torch.int32)
+    if layout == "row":
+        Xd, Yd = up(X, dev), up(Y, dev)
+    else:
+        Xd = up(np.ascontiguousarray(X.T), dev).t()
+        Yd = up(np.ascontiguousarray(Y.T), dev).t()
+    out = ops.sddmm_csr(A, Xd, Yd).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, val.astype(np.float64), X.astype(np.float64),
+                           Y.astype(np.float64))
+    assert rel_err(out, ref) <= (FP64_REL if dtype == np.float64 else FP32_NORMWISE)
+
+
 # ---- 1M x 1M, ~16M nnz, Zipf(1.5) rows, at full size --------------------------------------
 
 @pytest.fixture(scope="module")
