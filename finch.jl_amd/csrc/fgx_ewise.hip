@@ -81,30 +81,31 @@ __device__ __forceinline__ uint64_t group_mask() {
 template <int G>
 __host__ __device__ constexpr int ew_cap() { return 8 * G; }
 
+// Either side of a fibre pair: its coordinates staged in LDS (positions in
+// the fibre are ints: a staged fibre holds at most ew_cap<G>() of them) or
+// read from global memory (64-bit positions).  The kernels instantiate their
+// walk once per kind, so no access decides between the two.
 template <typename TI>
-__device__ __forceinline__ int lds_lower_bound(const TI* s, int lo, int hi, int64_t key) {
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (static_cast<int64_t>(s[mid]) < key) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
-// Either side of a fibre pair: its coordinates in global memory and, when
-// staged, the same coordinates in LDS (s != nullptr).
-template <typename TI>
-struct EwSide {
-  const TI* g;  // idx + fibre start
-  const TI* s;  // LDS copy, or nullptr
-  int64_t n;
-  __device__ __forceinline__ int64_t at(int64_t i) const {
-    return static_cast<int64_t>(s ? s[i] : g[i]);
-  }
+struct EwLds {
+  const TI* s;
+  int n;
+  __device__ __forceinline__ TI at(int i) const { return s[i]; }
   // First i in [0, n) with coordinate >= key (n if none).
-  __device__ __forceinline__ int64_t lower(int64_t key) const {
-    return s ? static_cast<int64_t>(lds_lower_bound(s, 0, static_cast<int>(n), key))
-             : lower_bound(g, 0, n, key);
+  __device__ __forceinline__ int lower(TI key) const {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
   }
+};
+template <typename TI>
+struct EwGlobal {
+  const TI* g;  // idx + fibre start
+  int64_t n;
+  __device__ __forceinline__ TI at(int64_t i) const { return g[i]; }
+  __device__ __forceinline__ int64_t lower(TI key) const { return lower_bound(g, 0, n, key); }
 };
 
 // Loads the group's two fibres into its LDS slots when both fit; the group's
@@ -141,25 +142,29 @@ __global__ __launch_bounds__(256) void ewise_count_kernel(
   const int64_t b0 = pos_at(bptr, base, row), b1 = pos_at(bptr, base, row + 1);
   const int64_t la = a1 - a0, lb = b1 - b0;
   const bool staged = ew_stage<G>(aidx, a0, la, bidx, b0, lb, s_a[grp], s_b[grp], gl);
-  EwSide<TI> A{aidx + a0, staged ? s_a[grp] : nullptr, la};
-  EwSide<TI> B{bidx + b0, staged ? s_b[grp] : nullptr, lb};
   // The common coordinates: each coordinate of the shorter fibre looks for
   // its partner in the longer one.
-  const EwSide<TI>& S = la <= lb ? A : B;
-  const EwSide<TI>& L = la <= lb ? B : A;
   int64_t found = 0;
-  if (L.n > 0) {
-    for (int64_t c = 0; c < S.n; c += G) {
-      const int64_t i = c + gl;
+  auto walk = [&](const auto& A, const auto& B) {
+    const auto& S = la <= lb ? A : B;
+    const auto& L = la <= lb ? B : A;
+    using Ix = decltype(S.n);
+    if (L.n == 0) return;
+    for (Ix c = 0; c < S.n; c += G) {
+      const Ix i = c + gl;
       bool f = false;
       if (i < S.n) {
-        const int64_t key = S.at(i);
-        const int64_t p = L.lower(key);
+        const TI key = S.at(i);
+        const Ix p = L.lower(key);
         f = p < L.n && L.at(p) == key;
       }
       found += popc(ballot(f) & gm);
     }
-  }
+  };
+  if (staged)
+    walk(EwLds<TI>{s_a[grp], static_cast<int>(la)}, EwLds<TI>{s_b[grp], static_cast<int>(lb)});
+  else
+    walk(EwGlobal<TI>{aidx + a0, la}, EwGlobal<TI>{bidx + b0, lb});
   if (gl == 0) cnt[row] = (op == FGX_OP_MUL) ? found : la + lb - found;
 }
 
@@ -202,71 +207,80 @@ __global__ __launch_bounds__(256) void ewise_fill_kernel(
     pb[j] = (uni && ib < lb) ? bval[b0 + ib] : TV(0);
   }
   const bool staged = ew_stage<G>(aidx, a0, la, bidx, b0, lb, s_a[grp], s_b[grp], gl);
-  EwSide<TI> A{aidx + a0, staged ? s_a[grp] : nullptr, la};
-  EwSide<TI> B{bidx + b0, staged ? s_b[grp] : nullptr, lb};
-  // A side: every stored a (union) or every a with a partner (intersection).
-  int64_t dup = 0;
-  for (int64_t c = 0; c < la; c += G) {
-    const int64_t i = c + gl;
-    const TV av = pa[0];
+  const TV* av_f = aval + a0;
+  const TV* bv_f = bval + b0;
+  TI* ci = cidx + c0;
+  TV* cv = cval + c0;
+  auto walk = [&](const auto& A, const auto& B) {
+    using Ix = decltype(A.n);
+    const Ix na = A.n, nb = B.n;
+    // A side: every stored a (union) or every a with a partner (intersection).
+    Ix dup = 0;
+    for (Ix c = 0; c < na; c += G) {
+      const Ix i = c + gl;
+      const TV av = pa[0];
 #pragma unroll
-    for (int j = 0; j + 1 < NPRE; j++) pa[j] = pa[j + 1];
-    {
-      const int64_t ia = i + static_cast<int64_t>(NPRE) * G;
-      pa[NPRE - 1] = (uni && ia < la) ? aval[a0 + ia] : TV(0);
-    }
-    bool f = false;
-    int64_t rb = 0;
-    int64_t key = 0;
-    if (i < la) {
-      key = A.at(i);
-      rb = B.lower(key);
-      f = rb < lb && B.at(rb) == key;
-    }
-    const uint64_t fb = ballot(f) & gm;
-    const int64_t before = dup + popc(fb & lt);
-    if (i < la) {
-      if (uni) {
-        const int64_t pos = c0 + i + rb - before;
-        cidx[pos] = static_cast<TI>(key);
-        cval[pos] = f ? ew_both<TV>(op, av, bval[b0 + rb]) : ew_only_a<TV>(op, av);
-      } else if (f) {
-        const int64_t pos = c0 + before;
-        cidx[pos] = static_cast<TI>(key);
-        cval[pos] = ew_both<TV>(op, aval[a0 + i], bval[b0 + rb]);
+      for (int j = 0; j + 1 < NPRE; j++) pa[j] = pa[j + 1];
+      {
+        const Ix ia = i + static_cast<Ix>(NPRE) * G;
+        pa[NPRE - 1] = (uni && ia < na) ? av_f[ia] : TV(0);
       }
+      bool f = false;
+      Ix rb = 0;
+      TI key = 0;
+      if (i < na) {
+        key = A.at(i);
+        rb = B.lower(key);
+        f = rb < nb && B.at(rb) == key;
+      }
+      const uint64_t fb = ballot(f) & gm;
+      const Ix before = dup + popc(fb & lt);
+      if (i < na) {
+        if (uni) {
+          const Ix pos = i + rb - before;
+          ci[pos] = key;
+          cv[pos] = f ? ew_both<TV>(op, av, bv_f[rb]) : ew_only_a<TV>(op, av);
+        } else if (f) {
+          ci[before] = key;
+          cv[before] = ew_both<TV>(op, av_f[i], bv_f[rb]);
+        }
+      }
+      dup += popc(fb);
     }
-    dup += popc(fb);
-  }
-  if (!uni) return;
-  // B side: the b without a partner in A.
-  dup = 0;
-  for (int64_t c = 0; c < lb; c += G) {
-    const int64_t i = c + gl;
-    const TV bv = pb[0];
+    if (!uni) return;
+    // B side: the b without a partner in A.
+    dup = 0;
+    for (Ix c = 0; c < nb; c += G) {
+      const Ix i = c + gl;
+      const TV bv = pb[0];
 #pragma unroll
-    for (int j = 0; j + 1 < NPRE; j++) pb[j] = pb[j + 1];
-    {
-      const int64_t ib = i + static_cast<int64_t>(NPRE) * G;
-      pb[NPRE - 1] = ib < lb ? bval[b0 + ib] : TV(0);
+      for (int j = 0; j + 1 < NPRE; j++) pb[j] = pb[j + 1];
+      {
+        const Ix ib = i + static_cast<Ix>(NPRE) * G;
+        pb[NPRE - 1] = ib < nb ? bv_f[ib] : TV(0);
+      }
+      bool f = false;
+      Ix ra = 0;
+      TI key = 0;
+      if (i < nb) {
+        key = B.at(i);
+        ra = A.lower(key);
+        f = ra < na && A.at(ra) == key;
+      }
+      const uint64_t fb = ballot(f) & gm;
+      const Ix before = dup + popc(fb & lt);
+      if (i < nb && !f) {
+        const Ix pos = i + ra - before;
+        ci[pos] = key;
+        cv[pos] = ew_only_b<TV>(op, bv);
+      }
+      dup += popc(fb);
     }
-    bool f = false;
-    int64_t ra = 0;
-    int64_t key = 0;
-    if (i < lb) {
-      key = B.at(i);
-      ra = A.lower(key);
-      f = ra < la && A.at(ra) == key;
-    }
-    const uint64_t fb = ballot(f) & gm;
-    const int64_t before = dup + popc(fb & lt);
-    if (i < lb && !f) {
-      const int64_t pos = c0 + i + ra - before;
-      cidx[pos] = static_cast<TI>(key);
-      cval[pos] = ew_only_b<TV>(op, bv);
-    }
-    dup += popc(fb);
-  }
+  };
+  if (staged)
+    walk(EwLds<TI>{s_a[grp], static_cast<int>(la)}, EwLds<TI>{s_b[grp], static_cast<int>(lb)});
+  else
+    walk(EwGlobal<TI>{aidx + a0, la}, EwGlobal<TI>{bidx + b0, lb});
 }
 
 static int pick_group(int64_t m, int64_t nnz_a, int64_t nnz_b) {
