@@ -234,7 +234,9 @@ int fgx_ewise_async(fgx_ctx* ctx, int op, const fgx_compressed* A, const fgx_com
  * test/suites/kernel_tests.jl:89-130).  Output coordinates of each fiber are
  * sorted (golden gustavsons.jl:182 `sort!`) and every touched coordinate is
  * stored.  Value types: FGX_PLUS_TIMES with I32 (exact, wrapping) or F32/F64;
- * FGX_MIN_PLUS with F32/F64 (exact).  Same two-phase protocol as ewise. */
+ * FGX_MIN_PLUS with F32/F64 (exact).  Same two-phase protocol as ewise: the
+ * symbolic call waits for nnz(C); the numeric call is stream-ordered only
+ * (no host synchronisation). */
 int fgx_spgemm_symbolic(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
                         const fgx_compressed* B, void* c_ptr, int64_t* c_nnz, void* stream);
 int fgx_spgemm_numeric(fgx_ctx* ctx, int semiring, const fgx_compressed* A,
