@@ -296,6 +296,32 @@ def test_spgemm_wide_columns_merge_paths(dev, n):
         _gemm_check(dev, semiring, ap, ai, av, m, l, bp, bi, bv, n)
 
 
+def test_spgemm_sorted_block_layout_edges(dev):
+    # Rows of at most 256 products whose lists all fit 32-product blocks (at
+    # most 8 lists) are sorted from one block per list; a list of 33, or a
+    # ninth list, takes the full network.  FP plus-times is order-sensitive,
+    # so a wrong fold order or a lost product changes the bits.
+    rng = np.random.default_rng(77)
+    l, n = 40, 5000
+    lens_b = np.array([32] * 8 + [33, 1, 0, 31, 16, 32] + [5] * 26)
+    bp = np.concatenate([[0], np.cumsum(lens_b)]) + 1
+    bi = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens_b if k]) + 1
+    rows_a = [list(range(8)),                 # 8 full blocks: 256 products
+              [8, 9, 11, 12],                 # a 33-product list among short ones
+              [9, 10, 11, 12, 14, 15, 16, 17, 18],  # nine lists, 73 products
+              [10, 11, 12],                   # an empty list among short ones
+              [13, 14, 15, 16, 17, 18, 19, 20],
+              list(range(14, 40)),            # 26 lists of 5: past 8 lists
+              list(range(8)) + [9]]           # 257 products: the 512-key sort
+    ap = np.concatenate([[0], np.cumsum([len(r) for r in rows_a])]) + 1
+    ai = np.concatenate([np.array(r) for r in rows_a]) + 1
+    for dtype, semiring in ((np.float32, "plus_times"), (np.float64, "plus_times"),
+                            (np.float32, "min_plus")):
+        av = rng.standard_normal(ai.size).astype(dtype)
+        bv = rng.standard_normal(bi.size).astype(dtype)
+        _gemm_check(dev, semiring, ap, ai, av, len(rows_a), l, bp, bi, bv, n)
+
+
 def test_spgemm_empty(dev):
     ep = np.ones(6, np.int64)
     e = np.zeros(0, np.int64)

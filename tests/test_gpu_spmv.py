@@ -267,6 +267,28 @@ def test_sddmm_few_positions_wide_rows(dev, k, layout, dtype):
     assert rel_err(out, ref) <= (FP64_REL if dtype == np.float64 else FP32_NORMWISE)
 
 
+@pytest.mark.parametrize("vtype", [torch.bfloat16, torch.float16])
+def test_sddmm_few_positions_wide_rows_half_values(dev, vtype):
+    # The wave-per-position path with 16-bit stored values: FP32 dense
+    # factors and accumulation, checked against FP64 on the rounded values.
+    rng = np.random.default_rng(3)
+    m = n = 500
+    k = 384
+    flat = np.sort(rng.choice(m * n, 60, replace=False))
+    rows, cols = flat // n, flat % n
+    ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=m))]) + 1
+    idx = cols + 1
+    v16 = torch.from_numpy(rng.standard_normal(60).astype(np.float32)).to(vtype)
+    X = rng.standard_normal((m, k)).astype(np.float32)
+    Y = rng.standard_normal((n, k)).astype(np.float32)
+    A = ops.Compressed(up(ptr.astype(np.int32), dev), up(idx.astype(np.int32), dev),
+                       v16.to(dev), m, n, 1)
+    out = ops.sddmm_csr(A, up(X, dev), up(Y, dev)).cpu().numpy()
+    ref = oracle.sddmm_csr(ptr, idx, v16.float().numpy().astype(np.float64), X.astype(np.float64),
+                           Y.astype(np.float64))
+    assert rel_err(out, ref) <= FP32_NORMWISE
+
+
 # ---- 1M x 1M, ~16M nnz, Zipf(1.5) rows, at full size --------------------------------------
 
 @pytest.fixture(scope="module")
