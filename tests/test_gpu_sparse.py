@@ -299,11 +299,12 @@ def test_spgemm_wide_columns_merge_paths(dev, n):
 def test_spgemm_sorted_block_layout_edges(dev):
     # Rows of at most 256 products whose lists all fit 32-product blocks (at
     # most 8 lists) are sorted from one block per list; a list of 33, or a
-    # ninth list, takes the full network.  FP plus-times is order-sensitive,
-    # so a wrong fold order or a lost product changes the bits.
+    # ninth list, takes the full network.  A row's key count is the larger of
+    # its products and its lists.  FP plus-times is order-sensitive, so a
+    # wrong fold order or a lost product changes the bits.
     rng = np.random.default_rng(77)
-    l, n = 40, 5000
-    lens_b = np.array([32] * 8 + [33, 1, 0, 31, 16, 32] + [5] * 26)
+    l, n = 200, 5000
+    lens_b = np.array([32] * 8 + [33, 1, 0, 31, 16, 32] + [5] * 26 + [1, 0] * 80)
     bp = np.concatenate([[0], np.cumsum(lens_b)]) + 1
     bi = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens_b if k]) + 1
     rows_a = [list(range(8)),                 # 8 full blocks: 256 products
@@ -312,7 +313,15 @@ def test_spgemm_sorted_block_layout_edges(dev):
               [10, 11, 12],                   # an empty list among short ones
               [13, 14, 15, 16, 17, 18, 19, 20],
               list(range(14, 40)),            # 26 lists of 5: past 8 lists
-              list(range(8)) + [9]]           # 257 products: the 512-key sort
+              list(range(8)) + [9],           # 257 products: the 512-key sort
+              [0, 1, 2, 3],                   # 4 full blocks of the 8: 128 products
+              [0, 1, 2, 8],                   # 129 products, a 33-product list
+              [9, 10, 11, 12, 14],            # five lists, 53 products
+              [8, 9],                         # a 33-product list, 34 products
+              [9, 11, 12, 13],                # four blocks, 80 products
+              [10],                           # one empty list: no products
+              list(range(40, 168)),           # 128 lists, 64 products
+              list(range(40, 200))]           # 160 lists, 80 products
     ap = np.concatenate([[0], np.cumsum([len(r) for r in rows_a])]) + 1
     ai = np.concatenate([np.array(r) for r in rows_a]) + 1
     for dtype, semiring in ((np.float32, "plus_times"), (np.float64, "plus_times"),
